@@ -1,0 +1,218 @@
+/*
+ * lj_hip.h -- C-ABI of the MI355X (gfx950) Lennard-Jones time step.
+ *
+ * Three groups of entry points, all with C linkage and plain pointers/sizes:
+ *
+ *  (1) Compatibility set.  The exact functions that the reference's
+ *      MDSystem.cpp declares under USE_CUDA_TOOLKIT (MDSystem.cpp:10-25) and
+ *      MDSystem.cu defines (MDSystem.cu:155-299), same names, signatures and
+ *      argument meaning.  Linking the reference's own MDSystem.cpp against
+ *      libljhip.so instead of MDSystem.cu puts the HIP path behind its
+ *      unchanged CPU/GPU switch (MDSystem.cpp:240).  They return void like
+ *      the originals; a failure is reported on stderr and kept for
+ *      lj_last_error() instead of exit()ing (set LJ_COMPAT_ABORT=1 to abort).
+ *
+ *  (2) Context API.  A context owns one box: device-resident positions,
+ *      velocities and forces (AoS float4, the reference layout), scratch,
+ *      one HIP stream and the captured step graph.  Every call returns
+ *      LJ_OK (0) or a negative status and never exits the process.
+ *      Calls are synchronous unless documented otherwise; distinct contexts
+ *      may be used from distinct host threads.
+ *
+ *  (3) Device-pointer API for callers that keep their own device buffers
+ *      (the PyTorch binding): forces and the adjoint pair kernel on caller
+ *      pointers, enqueued on a caller stream.
+ */
+#ifndef LJ_HIP_H
+#define LJ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status                                                              */
+/* ------------------------------------------------------------------ */
+#define LJ_OK 0
+#define LJ_ERR_ARG (-1)         /* invalid argument (null, size, mode) */
+#define LJ_ERR_DEVICE (-2)      /* no HIP device or a HIP runtime error */
+#define LJ_ERR_ALLOC (-3)       /* device or host allocation failed */
+#define LJ_ERR_STATE (-4)       /* call not valid in the current state */
+#define LJ_ERR_UNSUPPORTED (-5) /* mode not available for this shape */
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* lj_last_error(void);
+/* Number of visible HIP devices (0 when none; never fails). */
+int lj_device_count(void);
+/* Library version string. */
+const char* lj_version(void);
+
+/* ------------------------------------------------------------------ */
+/* (1) compatibility set -- reference MDSystem.cpp:10-25              */
+/* ------------------------------------------------------------------ */
+
+/* two device arrays of 4*numBodies floats        MDSystem.cu:158-165 */
+void allocateNBodyArrays(float* vel[2], int numBodies);
+/* one device array of 4*number floats            MDSystem.cu:167-173 */
+void allocateArray(float** dest, int number);
+/*                                                MDSystem.cu:175-179 */
+void deleteNBodyArrays(float* vel[2]);
+/*                                                MDSystem.cu:181-184 */
+void deleteArray(float* arr);
+/* device array of `number` ints                  MDSystem.cu:186-192 */
+void allocateIntArray(int** dest, int number);
+/*                                                MDSystem.cu:194-197 */
+void deleteIntArray(int* arr);
+/* 16*numBodies bytes device->host; pbo ignored   MDSystem.cu:199-210 */
+void copyArrayFromDevice(float* host, const float* device, unsigned int pbo, int numBodies);
+/* 16*numBodies bytes host->device                MDSystem.cu:212-216 */
+void copyArrayToDevice(float* device, const float* host, int numBodies);
+/* OpenGL interop: no-ops, as in the reference    MDSystem.cu:218-226 */
+void registerGLBufferObject(unsigned int pbo);
+void unregisterGLBufferObject(unsigned int pbo);
+/* device synchronise                             MDSystem.cu:228 */
+void threadSync(void);
+/*
+ * Forces, energy, virial and RDF of numBodies particles      MDSystem.cu:230-291
+ *   Pos    device float4[numBodies] (x,y,z,w; w ignored)
+ *   Force  device float4[numBodies] out: (Fx,Fy,Fz, sum_j (r^-12 - r^-6))
+ *          with F = 4 sum_j r (12 r^-14 - 6 r^-8)               MDSystem.cu:21-56,105-108
+ *   host_pressure out: sum_i (4/6) sum_j r.f_ij (virial only)   MDSystem.cu:108,136
+ *   Lperiodic  1: minimum image with box side host_L
+ *   host_RDF   out: 256 counts of ordered pairs, bin floor(fl32(r^2/dr2)) clamped to 255
+ *   pairs with r^2 <= 1e-6 are skipped, the self pair included MDSystem.cu:37
+ *   blockSize  accepted for signature compatibility (tiling is chosen internally;
+ *              the reference's per-blockSize RDF-merge stride bug, MDSystem.cu:149,
+ *              is not reproduced); q unused as in the reference.
+ * Synchronous: returns when Force, *host_pressure and host_RDF hold the results.
+ * The virial sum is a fixed-order FP64 reduction (deterministic), rounded to float.
+ */
+void calculateNForces(float* Pos, float* Force, float* host_pressure, int numBodies,
+                      float host_L, int Lperiodic, int* host_RDF, float host_dr2,
+                      int blockSize, int q);
+/* Releases the compatibility path's pooled scratch.  Unlike cudaThreadExit
+ * (MDSystem.cu:294-297) it does not reset the device, so other live arrays and
+ * contexts stay valid. */
+void threadExit(void);
+
+/* ------------------------------------------------------------------ */
+/* (2) context API                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct lj_ctx lj_ctx;
+
+enum { LJ_BC_PERIODIC = 0, LJ_BC_WALL = 1, LJ_BC_NONE = 2 }; /* MDSystem.h:24-25 */
+enum { LJ_EVN = 0, LJ_TVN = 1 };                              /* MDSystem.h:20-21 */
+enum {
+  LJ_RDF_REF_CPU = 0, /* reference CPU rule: r^2 as MDSystem.cpp:269-279 (double min image),
+                         bin = floor(r^2/dr2) exact, bins >= nbins dropped, self by index */
+  LJ_RDF_REF_GPU = 1, /* reference GPU rule: MDSystem.cu:26-42 (float lround min image,
+                         r^2 = fma(z,z,fma(y,y,x*x)), floor(fl32(r^2/dr2)), clamp to
+                         nbins-1, pairs with r^2 <= 1e-6 skipped) */
+  LJ_RDF_RBINS = 2    /* r^2 as LJ_RDF_REF_CPU, nbins uniform in r on [0, rmax):
+                         bin = max k with edges2[k] <= r^2, edges2[k] =
+                         (float)((k*(rmax/nbins))^2) in double; r^2 >= edges2[nbins] dropped */
+};
+enum {
+  LJ_PREC_FP32 = 0,       /* FP32 pair math and per-slice sums, FP64 cross-slice sums */
+  LJ_PREC_COMPENSATED = 1 /* + TwoSum-compensated per-particle sums inside each slice */
+};
+enum { LJ_ARRAY_POS = 0, LJ_ARRAY_VEL = 1, LJ_ARRAY_FORCE = 2 }; /* MDSystem.cpp:586-630 */
+
+typedef struct lj_config {
+  int N;          /* particles                               MDSystem.h:12 */
+  double T0;      /* initial / TVN temperature               MDSystem.h:15 */
+  double rho;     /* number density                          MDSystem.h:18 */
+  int canonical;  /* LJ_EVN or LJ_TVN                        MDSystem.h:21 */
+  int boundary;   /* LJ_BC_*                                 MDSystem.h:25 */
+  double eps;     /* LJ well depth (reference: 1)            */
+  double sigma;   /* LJ diameter (reference: 1)              */
+  int precision;  /* LJ_PREC_*                               */
+  int jslices;    /* j-range split of the force kernel; 0 = automatic */
+  int device;     /* HIP device ordinal                      */
+} lj_config;
+
+/* Observables of the last evaluated state.  Field meaning as the reference's
+ * public members MDSystem.h:61-66,79-80,97-99 (CalculateParameters,
+ * MDSystem.cpp:325-359): P includes the kinetic term and the volume, V is the
+ * potential energy.  W is the virial sum_{i<j} r_ij.f_ij (P = (W/3 + N T)/vol).
+ * Pshear is not computed on this path (nor on the reference's GPU branch). */
+typedef struct lj_obs {
+  double U, T, K, V, P, Pshear, t, W;
+  double av_U_tot, av_T_tot, av_p_tot;
+  int64_t av_iters;
+  int64_t step;
+} lj_obs;
+
+/* defaults of MDSystemConfiguration (MDSystem.h:32-40) plus eps = sigma = 1 */
+void lj_config_init(lj_config* cfg);
+int lj_create(const lj_config* cfg, lj_ctx** out);
+int lj_destroy(lj_ctx* ctx);
+/* box side L = (N/rho)^(1/3) (MDSystem.cpp:70) */
+double lj_box_length(const lj_ctx* ctx);
+/* reference RDF bin width in r^2 (MDSystem.cpp:93-95) */
+float lj_rdf_dr2_default(int N);
+
+/* Initial state of the benchmark configurations: simple-cubic lattice
+ * (MDSystem.cpp:150-168), Gaussian velocities from SplitMix64(seed) with
+ * Box-Muller, momentum removed and rescaled to exactly T0
+ * (MDSystem.cpp:179-180).  Computed on the host, uploaded; then forces and
+ * observables are evaluated as MDSystem::Reinitialize does (MDSystem.cpp:101-103). */
+int lj_sample_initial_state(lj_ctx* ctx, uint64_t seed);
+
+/* host float[4N] <-> device state           MDSystem.cpp:586-630 */
+int lj_set_array(lj_ctx* ctx, int type, const float* host4N);
+int lj_get_array(lj_ctx* ctx, int type, float* host4N);
+
+/* CalculateForces (MDSystem.cpp:232-311): device forces of the current
+ * positions, E_pot and virial into the observables (out may be NULL). */
+int lj_compute_forces(lj_ctx* ctx, lj_obs* out);
+/* CalculateParameters (MDSystem.cpp:325-359) from the device state. */
+int lj_compute_parameters(lj_ctx* ctx, lj_obs* out);
+int lj_get_observables(lj_ctx* ctx, lj_obs* out);
+
+/* nsteps of Integrate(dt) (MDSystem.cpp:438-583) fully on the device:
+ * velocity Verlet EVN or TVN, boundary conditions, observables; captured
+ * once into a HIP graph and replayed.  hist (may be NULL) receives 5 doubles
+ * per step {U, T, K, V, P}. */
+int lj_integrate(lj_ctx* ctx, double dt, int nsteps, double* hist);
+
+/* Histogram of ordered-pair distances of the current positions.
+ * counts: host uint64[nbins]; accumulate != 0 adds to the device histogram
+ * kept from earlier calls with the same rule/nbins, 0 clears it first.
+ * dr2 <= 0 picks the reference rule; rmax <= 0 picks L/2. */
+int lj_rdf(lj_ctx* ctx, int rule, int nbins, float dr2, double rmax, uint64_t* counts,
+           int accumulate);
+
+int lj_renormalize_velocities(lj_ctx* ctx, int recompute_tkin); /* MDSystem.cpp:375-389 */
+int lj_renormalize_to_energy(lj_ctx* ctx, double ust);          /* MDSystem.cpp:391-404 */
+int lj_reset_averaging(lj_ctx* ctx);                            /* MDSystem.cpp:701-705 */
+int lj_apply_boundary_conditions(lj_ctx* ctx);                  /* MDSystem.cpp:406-436 */
+int lj_set_mode(lj_ctx* ctx, int canonical, int boundary);      /* MDSystem.h:161-163 */
+int lj_synchronize(lj_ctx* ctx);
+
+/* device pointers of the state arrays (float4[N]) for zero-copy callers */
+int lj_device_arrays(lj_ctx* ctx, float** pos, float** vel, float** force);
+
+/* time of the force kernel alone, averaged over `reps` launches on the
+ * context stream, measured with HIP events (for roofline reporting). */
+int lj_time_force_kernel(lj_ctx* ctx, int reps, double* us_per_launch);
+
+/* ------------------------------------------------------------------ */
+/* (3) device-pointer API                                              */
+/* ------------------------------------------------------------------ */
+
+/* Forces of caller-owned positions d_pos4 (float4[N], device) into d_force4
+ * (float4[N], w = per-particle sum ((s/r)^12-(s/r)^6)); d_ev (device, may be
+ * NULL) receives {E_pot, W}.  Box, boundary, eps, sigma from ctx; N must equal
+ * the context's N.  Enqueued on `stream` (NULL = context stream), async. */
+int lj_forces_device(lj_ctx* ctx, const float* d_pos4, float* d_force4, double* d_ev,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LJ_HIP_H */

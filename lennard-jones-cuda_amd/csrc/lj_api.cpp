@@ -1,0 +1,835 @@
+// lj_api.cpp -- the C-ABI of libljhip.so: compatibility set (the reference's
+// MDSystem.cu entry points), context API and device-pointer API.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lj_internal.h"
+
+using namespace lj;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define LJ_HIP(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(LJ_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),     \
+                  __FILE__, __LINE__);                                                     \
+  } while (0)
+
+#define LJ_TRY(expr)        \
+  do {                      \
+    int rc_ = (expr);       \
+    if (rc_ != LJ_OK) return rc_; \
+  } while (0)
+
+template <class T>
+int dev_alloc(T** p, size_t count)
+{
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(LJ_ERR_ALLOC, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+  }
+  return LJ_OK;
+}
+
+template <class T>
+void dev_free(T*& p)
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int check_device()
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(LJ_ERR_DEVICE, "no HIP device available (the HIP path has no CPU fallback)");
+  return LJ_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+
+struct lj_ctx {
+  lj_config cfg{};
+  double L = 0.;
+  hipStream_t stream = nullptr;
+  ForcePlan plan;
+  PairScalars ps;
+  float4* pos = nullptr;
+  float4* vel = nullptr;
+  float4* force = nullptr;
+  float4* tF = nullptr;
+  uint4* q = nullptr;
+  float4* fpart = nullptr;
+  float* apart = nullptr;
+  double* blk = nullptr;
+  int nblk = 0;
+  lj_obs* obs = nullptr;
+  double* hist = nullptr;
+  int hist_cap = 0;
+  int64_t* hist_base = nullptr;
+  double* ev = nullptr;
+  unsigned long long* rdf = nullptr;
+  int rdf_cap = 0;
+  int rdf_nbins = 0, rdf_rule = -1;
+  float rdf_dr2 = 0.f;
+  double rdf_rmax = 0.;
+  float* edges2 = nullptr;
+  int edges_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0.;
+  int g_canon = -1, g_bc = -1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+int ctx_setdev(lj_ctx* c)
+{
+  if (!c) return fail(LJ_ERR_ARG, "null context");
+  LJ_HIP(hipSetDevice(c->cfg.device));
+  return LJ_OK;
+}
+
+void drop_graph(lj_ctx* c)
+{
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
+StepArgs step_args(lj_ctx* c, double dt)
+{
+  StepArgs a{};
+  a.pos = c->pos;
+  a.vel = c->vel;
+  a.force = c->force;
+  a.tF = c->tF;
+  a.q = c->q;
+  a.fpart = c->fpart;
+  a.apart = c->apart;
+  a.blk = c->blk;
+  a.obs = c->obs;
+  a.hist = c->hist;
+  a.hist_base = c->hist_base;
+  a.hist_cap = c->hist_cap;
+  a.N = c->cfg.N;
+  a.dt = dt;
+  a.L = c->L;
+  a.T0 = c->cfg.T0;
+  a.rho = c->cfg.rho;
+  a.eps = c->cfg.eps;
+  a.qscale = c->ps.qscale;
+  a.FS = c->ps.FS;
+  a.bc = c->cfg.boundary;
+  a.canonical = c->cfg.canonical;
+  a.S = c->plan.S;
+  a.Npad = c->plan.Npad;
+  return a;
+}
+
+void refresh_scalars(lj_ctx* c)
+{
+  c->ps = make_pair_scalars(c->L, c->cfg.boundary == LJ_BC_PERIODIC, c->cfg.eps, c->cfg.sigma,
+                            false);
+}
+
+// force pass into fpart/apart from the context positions (or external d_pos)
+int enqueue_force(lj_ctx* c, hipStream_t st, const float4* d_pos)
+{
+  if (c->ps.periodic) LJ_HIP(launch_prep_q(st, d_pos, c->q, c->cfg.N, c->ps.qscale));
+  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, d_pos, c->fpart, c->apart,
+                      c->cfg.precision == LJ_PREC_COMPENSATED));
+  return LJ_OK;
+}
+
+int enqueue_step(lj_ctx* c, hipStream_t st, double dt)
+{
+  const StepArgs a = step_args(c, dt);
+  LJ_HIP(launch_drift(st, a));
+  // the drift kernel already wrote the fixed-point copy for periodic boxes
+  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart,
+                      c->cfg.precision == LJ_PREC_COMPENSATED));
+  if (c->cfg.canonical == LJ_EVN) {
+    LJ_HIP(launch_kick_evn(st, a));
+  } else {
+    LJ_HIP(launch_tvn_a(st, a));
+    LJ_HIP(launch_tvn_b(st, a));
+  }
+  LJ_HIP(launch_finalize_step(st, a));
+  return LJ_OK;
+}
+
+int ensure_hist(lj_ctx* c, int n)
+{
+  if (n <= c->hist_cap) return LJ_OK;
+  int cap = std::max(n, 2 * c->hist_cap);
+  double* h = nullptr;
+  LJ_TRY(dev_alloc(&h, (size_t)5 * cap));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  dev_free(c->hist);
+  c->hist = h;
+  c->hist_cap = cap;
+  drop_graph(c);  // the graph holds the old pointer
+  return LJ_OK;
+}
+
+int compute_forces_internal(lj_ctx* c)
+{
+  hipStream_t st = c->stream;
+  LJ_TRY(enqueue_force(c, st, c->pos));
+  LJ_HIP(launch_reduce_forces(st, c->plan, c->ps, c->fpart, c->apart, c->force, c->blk));
+  LJ_HIP(launch_finalize_forces(st, c->blk, c->nblk, c->cfg.eps, c->ev, nullptr));
+  // V and W into the observables block (CalculateForces sets V and P, MDSystem.cpp:257-309)
+  LJ_HIP(launch_store_vw(st, c->ev, c->obs));
+  return LJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lj_last_error(void) { return g_err.c_str(); }
+
+int lj_device_count(void)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char* lj_version(void) { return "ljhip 0.1 (gfx950)"; }
+
+void lj_config_init(lj_config* cfg)
+{
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->N = 128;  // MDSystem.h:32-39
+  cfg->T0 = 1.5;
+  cfg->rho = 0.2;
+  cfg->canonical = LJ_EVN;
+  cfg->boundary = LJ_BC_PERIODIC;
+  cfg->eps = 1.0;
+  cfg->sigma = 1.0;
+  cfg->precision = LJ_PREC_FP32;
+  cfg->jslices = 0;
+  cfg->device = 0;
+}
+
+double lj_box_length(const lj_ctx* ctx) { return ctx ? ctx->L : 0.; }
+
+float lj_rdf_dr2_default(int N) { return N > 0 ? rdf_dr2_rule(N) : 0.f; }
+
+int lj_destroy(lj_ctx* c)
+{
+  if (!c) return LJ_OK;
+  (void)hipSetDevice(c->cfg.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  dev_free(c->pos);
+  dev_free(c->vel);
+  dev_free(c->force);
+  dev_free(c->tF);
+  dev_free(c->q);
+  dev_free(c->fpart);
+  dev_free(c->apart);
+  dev_free(c->blk);
+  dev_free(c->obs);
+  dev_free(c->hist);
+  dev_free(c->hist_base);
+  dev_free(c->ev);
+  dev_free(c->rdf);
+  dev_free(c->edges2);
+  if (c->e0) (void)hipEventDestroy(c->e0);
+  if (c->e1) (void)hipEventDestroy(c->e1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return LJ_OK;
+}
+
+int lj_create(const lj_config* cfg, lj_ctx** out)
+{
+  if (!cfg || !out) return fail(LJ_ERR_ARG, "lj_create: null argument");
+  *out = nullptr;
+  if (cfg->N <= 0) return fail(LJ_ERR_ARG, "lj_create: N must be positive (got %d)", cfg->N);
+  if (cfg->N > (1 << 26)) return fail(LJ_ERR_ARG, "lj_create: N too large (%d)", cfg->N);
+  if (!(cfg->rho > 0.)) return fail(LJ_ERR_ARG, "lj_create: rho must be positive");
+  if (!(cfg->T0 >= 0.)) return fail(LJ_ERR_ARG, "lj_create: T0 must be non-negative");
+  if (cfg->boundary < 0 || cfg->boundary > 2) return fail(LJ_ERR_ARG, "lj_create: boundary must be 0, 1 or 2");
+  if (cfg->canonical != 0 && cfg->canonical != 1) return fail(LJ_ERR_ARG, "lj_create: canonical must be 0 or 1");
+  if (!(cfg->eps > 0.) || !(cfg->sigma > 0.)) return fail(LJ_ERR_ARG, "lj_create: eps and sigma must be positive");
+  if (cfg->precision != LJ_PREC_FP32 && cfg->precision != LJ_PREC_COMPENSATED)
+    return fail(LJ_ERR_ARG, "lj_create: unknown precision mode %d", cfg->precision);
+  LJ_TRY(check_device());
+  int ndev = 0;
+  LJ_HIP(hipGetDeviceCount(&ndev));
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(LJ_ERR_ARG, "lj_create: no device %d", cfg->device);
+  LJ_HIP(hipSetDevice(cfg->device));
+
+  lj_ctx* c = new lj_ctx();
+  c->cfg = *cfg;
+  c->L = box_length(cfg->N, cfg->rho);
+  refresh_scalars(c);
+  c->plan = make_force_plan(cfg->N, cfg->jslices, 1);
+  const int N = cfg->N;
+  c->nblk = (N + kBlock - 1) / kBlock;
+  int rc = LJ_OK;
+  auto A = [&](int r) { if (rc == LJ_OK) rc = r; };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    c->stream = nullptr;
+    A(fail(LJ_ERR_DEVICE, "hipStreamCreate failed"));
+  }
+  A(dev_alloc(&c->pos, N));
+  A(dev_alloc(&c->vel, N));
+  A(dev_alloc(&c->force, N));
+  A(dev_alloc(&c->tF, N));
+  A(dev_alloc(&c->q, N));
+  A(dev_alloc(&c->fpart, (size_t)c->plan.S * c->plan.Npad));
+  A(dev_alloc(&c->apart, (size_t)c->plan.S * c->plan.Npad));
+  A(dev_alloc(&c->blk, (size_t)c->nblk * kBlkFields));
+  A(dev_alloc(&c->obs, 1));
+  A(dev_alloc(&c->hist_base, 1));
+  A(dev_alloc(&c->ev, 2));
+  if (rc == LJ_OK && hipEventCreate(&c->e0) != hipSuccess) A(fail(LJ_ERR_DEVICE, "hipEventCreate"));
+  if (rc == LJ_OK && hipEventCreate(&c->e1) != hipSuccess) A(fail(LJ_ERR_DEVICE, "hipEventCreate"));
+  if (rc == LJ_OK) A(ensure_hist(c, 1024));
+  if (rc == LJ_OK) {
+    // zero state: host buffers copied so that no memset node ever enters a graph
+    std::vector<float> z((size_t)4 * N, 0.f);
+    lj_obs o{};
+    const size_t bytes = sizeof(float) * 4 * (size_t)N;
+    if (hipMemcpy(c->pos, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->vel, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->force, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->tF, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->obs, &o, sizeof(o), hipMemcpyHostToDevice) != hipSuccess)
+      A(fail(LJ_ERR_DEVICE, "initial upload failed"));
+  }
+  if (rc != LJ_OK) {
+    std::string msg = g_err;
+    lj_destroy(c);
+    g_err = msg;
+    return rc;
+  }
+  *out = c;
+  return LJ_OK;
+}
+
+int lj_set_array(lj_ctx* c, int type, const float* host)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!host) return fail(LJ_ERR_ARG, "lj_set_array: null host pointer");
+  float4* d = type == LJ_ARRAY_POS ? c->pos : type == LJ_ARRAY_VEL ? c->vel : type == LJ_ARRAY_FORCE ? c->force : nullptr;
+  if (!d) return fail(LJ_ERR_ARG, "lj_set_array: unknown array type %d", type);
+  LJ_HIP(hipMemcpyAsync(d, host, sizeof(float4) * c->cfg.N, hipMemcpyHostToDevice, c->stream));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_get_array(lj_ctx* c, int type, float* host)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!host) return fail(LJ_ERR_ARG, "lj_get_array: null host pointer");
+  float4* d = type == LJ_ARRAY_POS ? c->pos : type == LJ_ARRAY_VEL ? c->vel : type == LJ_ARRAY_FORCE ? c->force : nullptr;
+  if (!d) return fail(LJ_ERR_ARG, "lj_get_array: unknown array type %d", type);
+  LJ_HIP(hipMemcpyAsync(host, d, sizeof(float4) * c->cfg.N, hipMemcpyDeviceToHost, c->stream));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_get_observables(lj_ctx* c, lj_obs* out)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!out) return fail(LJ_ERR_ARG, "lj_get_observables: null output");
+  LJ_HIP(hipMemcpyAsync(out, c->obs, sizeof(lj_obs), hipMemcpyDeviceToHost, c->stream));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_compute_forces(lj_ctx* c, lj_obs* out)
+{
+  LJ_TRY(ctx_setdev(c));
+  LJ_TRY(compute_forces_internal(c));
+  if (out) return lj_get_observables(c, out);
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_compute_parameters(lj_ctx* c, lj_obs* out)
+{
+  LJ_TRY(ctx_setdev(c));
+  const StepArgs a = step_args(c, 0.);
+  LJ_HIP(launch_kinetic_partials(c->stream, c->vel, c->cfg.N, c->blk));
+  LJ_HIP(launch_params(c->stream, a, 0));
+  if (out) return lj_get_observables(c, out);
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_reset_averaging(lj_ctx* c)
+{
+  LJ_TRY(ctx_setdev(c));
+  LJ_HIP(launch_reset_averaging(c->stream, c->obs));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_sample_initial_state(lj_ctx* c, uint64_t seed)
+{
+  LJ_TRY(ctx_setdev(c));
+  const int N = c->cfg.N;
+  std::vector<float> p((size_t)4 * N), v((size_t)4 * N);
+  lattice_positions(N, c->L, p.data());
+  gaussian_velocities(N, c->cfg.T0, seed, v.data());
+  LJ_TRY(lj_set_array(c, LJ_ARRAY_POS, p.data()));
+  LJ_TRY(lj_set_array(c, LJ_ARRAY_VEL, v.data()));
+  LJ_HIP(launch_reset_clock(c->stream, c->obs));
+  // MDSystem::Reinitialize: CalculateForces, CalculateParameters, reset averages (:101-113)
+  LJ_TRY(lj_compute_forces(c, nullptr));
+  LJ_TRY(lj_compute_parameters(c, nullptr));
+  return lj_reset_averaging(c);
+}
+
+int lj_integrate(lj_ctx* c, double dt, int nsteps, double* hist)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (nsteps < 0) return fail(LJ_ERR_ARG, "lj_integrate: nsteps < 0");
+  if (!std::isfinite(dt)) return fail(LJ_ERR_ARG, "lj_integrate: dt not finite");
+  if (nsteps == 0) return LJ_OK;
+  LJ_TRY(ensure_hist(c, nsteps));
+  hipStream_t st = c->stream;
+  LJ_HIP(launch_set_hist_base(st, c->obs, c->hist_base));
+  if (!c->gexec || c->g_dt != dt || c->g_canon != c->cfg.canonical || c->g_bc != c->cfg.boundary) {
+    drop_graph(c);
+    LJ_HIP(hipStreamSynchronize(st));
+    LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c, st, dt);
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(st, &g);
+    if (rc != LJ_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    LJ_HIP(ee);
+    c->graph = g;
+    LJ_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->g_dt = dt;
+    c->g_canon = c->cfg.canonical;
+    c->g_bc = c->cfg.boundary;
+  }
+  for (int k = 0; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(c->gexec, st));
+  if (hist)
+    LJ_HIP(hipMemcpyAsync(hist, c->hist, sizeof(double) * 5 * nsteps, hipMemcpyDeviceToHost, st));
+  LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, uint64_t* counts,
+           int accumulate)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (rule < LJ_RDF_REF_CPU || rule > LJ_RDF_RBINS) return fail(LJ_ERR_ARG, "lj_rdf: unknown rule %d", rule);
+  if (nbins <= 0 || nbins > 8000) return fail(LJ_ERR_ARG, "lj_rdf: nbins must be in [1, 8000]");
+  if (!counts) return fail(LJ_ERR_ARG, "lj_rdf: null counts");
+  if (!(dr2 > 0.f)) dr2 = rdf_dr2_rule(c->cfg.N);
+  if (!(rmax > 0.)) rmax = c->L / 2.;
+  hipStream_t st = c->stream;
+  if (nbins > c->rdf_cap) {
+    LJ_HIP(hipStreamSynchronize(st));
+    dev_free(c->rdf);
+    c->rdf_cap = 0;
+    LJ_TRY(dev_alloc(&c->rdf, nbins));
+    c->rdf_cap = nbins;
+    accumulate = 0;
+  }
+  const bool same = rule == c->rdf_rule && nbins == c->rdf_nbins && dr2 == c->rdf_dr2 && rmax == c->rdf_rmax;
+  if (!accumulate || !same) LJ_HIP(launch_fill_u64(st, c->rdf, nbins));
+  c->rdf_rule = rule;
+  c->rdf_nbins = nbins;
+  c->rdf_dr2 = dr2;
+  c->rdf_rmax = rmax;
+  RdfArgs a{};
+  a.pos = c->pos;
+  a.N = c->cfg.N;
+  a.periodic = c->cfg.boundary == LJ_BC_PERIODIC;
+  a.L = c->L;
+  a.rule = rule;
+  a.nbins = nbins;
+  a.dr2 = dr2;
+  a.hist = c->rdf;
+  cpu_minimage_thresholds(c->L, a.thr);
+  if (rule == LJ_RDF_RBINS) {
+    std::vector<float> e(nbins + 1);
+    rdf_edges2(nbins, rmax, e.data());
+    if (nbins + 1 > c->edges_cap) {
+      LJ_HIP(hipStreamSynchronize(st));
+      dev_free(c->edges2);
+      c->edges_cap = 0;
+      LJ_TRY(dev_alloc(&c->edges2, nbins + 1));
+      c->edges_cap = nbins + 1;
+    }
+    LJ_HIP(hipMemcpyAsync(c->edges2, e.data(), sizeof(float) * (nbins + 1), hipMemcpyHostToDevice, st));
+    a.edges2 = c->edges2;
+    a.rbin_inv = (float)(nbins / rmax);
+  }
+  LJ_HIP(launch_rdf(st, a));
+  LJ_HIP(hipMemcpyAsync(counts, c->rdf, sizeof(uint64_t) * nbins, hipMemcpyDeviceToHost, st));
+  LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_renormalize_velocities(lj_ctx* c, int recompute_tkin)
+{
+  LJ_TRY(ctx_setdev(c));
+  double factor = 0.;
+  if (!recompute_tkin) {
+    // Tkin = T of the last CalculateParameters (MDSystem.cpp:377)
+    lj_obs o;
+    LJ_TRY(lj_get_observables(c, &o));
+    if (!(o.T > 0.)) return fail(LJ_ERR_STATE, "lj_renormalize_velocities: T is not positive");
+    factor = std::sqrt(c->cfg.T0 / o.T);
+  } else {
+    LJ_HIP(launch_kinetic_partials(c->stream, c->vel, c->cfg.N, c->blk));
+  }
+  LJ_HIP(launch_scale_velocities(c->stream, c->vel, c->cfg.N, c->blk, c->nblk, c->cfg.T0, factor,
+                                 c->obs));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_renormalize_to_energy(lj_ctx* c, double ust)
+{
+  LJ_TRY(ctx_setdev(c));
+  lj_obs o;
+  LJ_TRY(lj_get_observables(c, &o));
+  const double Kdes = ust * c->cfg.N - o.V;  // MDSystem.cpp:393-395
+  if (!(Kdes > 0.) || !(o.K > 0.))
+    return fail(LJ_ERR_ARG, "lj_renormalize_to_energy: target kinetic energy %g not reachable", Kdes);
+  const double factor = std::sqrt(Kdes / o.K);
+  LJ_HIP(launch_scale_velocities(c->stream, c->vel, c->cfg.N, c->blk, c->nblk, c->cfg.T0, factor,
+                                 nullptr));
+  o.K = Kdes;
+  o.U = o.K + o.V;
+  LJ_HIP(hipMemcpyAsync(c->obs, &o, sizeof(o), hipMemcpyHostToDevice, c->stream));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_apply_boundary_conditions(lj_ctx* c)
+{
+  LJ_TRY(ctx_setdev(c));
+  LJ_HIP(launch_apply_bc(c->stream, c->pos, c->vel, c->cfg.N, c->cfg.boundary, c->L));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_set_mode(lj_ctx* c, int canonical, int boundary)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (canonical != 0 && canonical != 1) return fail(LJ_ERR_ARG, "lj_set_mode: canonical must be 0 or 1");
+  if (boundary < 0 || boundary > 2) return fail(LJ_ERR_ARG, "lj_set_mode: boundary must be 0, 1 or 2");
+  c->cfg.canonical = canonical;
+  c->cfg.boundary = boundary;
+  refresh_scalars(c);
+  drop_graph(c);
+  return LJ_OK;
+}
+
+int lj_synchronize(lj_ctx* c)
+{
+  LJ_TRY(ctx_setdev(c));
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_device_arrays(lj_ctx* c, float** pos, float** vel, float** force)
+{
+  if (!c) return fail(LJ_ERR_ARG, "null context");
+  if (pos) *pos = reinterpret_cast<float*>(c->pos);
+  if (vel) *vel = reinterpret_cast<float*>(c->vel);
+  if (force) *force = reinterpret_cast<float*>(c->force);
+  return LJ_OK;
+}
+
+int lj_time_force_kernel(lj_ctx* c, int reps, double* us)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (reps <= 0 || !us) return fail(LJ_ERR_ARG, "lj_time_force_kernel: bad arguments");
+  hipStream_t st = c->stream;
+  if (c->ps.periodic) LJ_HIP(launch_prep_q(st, c->pos, c->q, c->cfg.N, c->ps.qscale));
+  const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
+  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp));  // warm
+  LJ_HIP(hipEventRecord(c->e0, st));
+  for (int k = 0; k < reps; ++k)
+    LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp));
+  LJ_HIP(hipEventRecord(c->e1, st));
+  LJ_HIP(hipEventSynchronize(c->e1));
+  float ms = 0.f;
+  LJ_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+  *us = 1000.0 * ms / reps;
+  return LJ_OK;
+}
+
+int lj_forces_device(lj_ctx* c, const float* d_pos4, float* d_force4, double* d_ev, void* stream)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!d_pos4 || !d_force4) return fail(LJ_ERR_ARG, "lj_forces_device: null device pointer");
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  const float4* p = reinterpret_cast<const float4*>(d_pos4);
+  LJ_TRY(enqueue_force(c, st, p));
+  LJ_HIP(launch_reduce_forces(st, c->plan, c->ps, c->fpart, c->apart,
+                              reinterpret_cast<float4*>(d_force4), c->blk));
+  LJ_HIP(launch_finalize_forces(st, c->blk, c->nblk, c->cfg.eps, d_ev ? d_ev : c->ev, nullptr));
+  return LJ_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// compatibility set (MDSystem.cu:155-299)
+
+namespace {
+
+struct CompatScratch {
+  int device = -1;
+  hipStream_t st = nullptr;
+  int capN = 0;
+  size_t capPart = 0;
+  uint4* q = nullptr;
+  float4* fpart = nullptr;
+  float* apart = nullptr;
+  double* blk = nullptr;
+  unsigned long long* rdf = nullptr;
+  float* pressure = nullptr;
+};
+
+std::mutex g_compat_mu;
+std::vector<CompatScratch> g_compat;
+
+void compat_report(int rc)
+{
+  if (rc == LJ_OK) return;
+  std::fprintf(stderr, "ljhip: %s\n", g_err.c_str());
+  const char* ab = std::getenv("LJ_COMPAT_ABORT");
+  if (ab && ab[0] == '1') std::abort();
+}
+
+void compat_release(CompatScratch& s)
+{
+  if (s.device >= 0) (void)hipSetDevice(s.device);
+  if (s.st) (void)hipStreamSynchronize(s.st);
+  dev_free(s.q);
+  dev_free(s.fpart);
+  dev_free(s.apart);
+  dev_free(s.blk);
+  dev_free(s.rdf);
+  dev_free(s.pressure);
+  if (s.st) (void)hipStreamDestroy(s.st);
+  s = CompatScratch();
+}
+
+int compat_scratch(int N, const ForcePlan& pl, CompatScratch** out)
+{
+  int dev = 0;
+  LJ_HIP(hipGetDevice(&dev));
+  if ((int)g_compat.size() <= dev) g_compat.resize(dev + 1);
+  CompatScratch& s = g_compat[dev];
+  s.device = dev;
+  if (!s.st) LJ_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+  const size_t part = (size_t)pl.S * pl.Npad;
+  if (N > s.capN || part > s.capPart) {
+    LJ_HIP(hipStreamSynchronize(s.st));
+    dev_free(s.q);
+    dev_free(s.fpart);
+    dev_free(s.apart);
+    dev_free(s.blk);
+    s.capN = 0;
+    LJ_TRY(dev_alloc(&s.q, N));
+    LJ_TRY(dev_alloc(&s.fpart, (size_t)pl.S * pl.Npad));
+    LJ_TRY(dev_alloc(&s.apart, (size_t)pl.S * pl.Npad));
+    LJ_TRY(dev_alloc(&s.blk, (size_t)((N + kBlock - 1) / kBlock) * kBlkFields));
+    s.capN = N;
+    s.capPart = part;
+  }
+  if (!s.rdf) LJ_TRY(dev_alloc(&s.rdf, 256));
+  if (!s.pressure) LJ_TRY(dev_alloc(&s.pressure, 1));
+  *out = &s;
+  return LJ_OK;
+}
+
+int compat_forces(float* Pos, float* Force, float* host_pressure, int N, float host_L,
+                  int Lperiodic, int* host_RDF, float host_dr2)
+{
+  if (!Pos || !Force) return fail(LJ_ERR_ARG, "calculateNForces: null device array");
+  if (N <= 0) return fail(LJ_ERR_ARG, "calculateNForces: numBodies must be positive (got %d)", N);
+  if (Lperiodic && !(host_L > 0.f)) return fail(LJ_ERR_ARG, "calculateNForces: box side must be positive");
+  if (host_RDF && !(host_dr2 > 0.f)) return fail(LJ_ERR_ARG, "calculateNForces: RDF bin width must be positive");
+  LJ_TRY(check_device());
+  const ForcePlan pl = make_force_plan(N, 0, 1);
+  const PairScalars ps = make_pair_scalars((double)host_L, Lperiodic != 0, 1., 1., true);
+  CompatScratch* s = nullptr;
+  LJ_TRY(compat_scratch(N, pl, &s));
+  hipStream_t st = s->st;
+  const float4* p = reinterpret_cast<const float4*>(Pos);
+  if (ps.periodic) LJ_HIP(launch_prep_q(st, p, s->q, N, ps.qscale));
+  LJ_HIP(launch_force(st, pl, ps, s->q, p, s->fpart, s->apart, false));
+  LJ_HIP(launch_reduce_forces(st, pl, ps, s->fpart, s->apart, reinterpret_cast<float4*>(Force), s->blk));
+  LJ_HIP(launch_finalize_forces(st, s->blk, (N + kBlock - 1) / kBlock, 1., nullptr, s->pressure));
+  if (host_RDF) {
+    LJ_HIP(launch_fill_u64(st, s->rdf, 256));
+    RdfArgs a{};
+    a.pos = p;
+    a.N = N;
+    a.periodic = Lperiodic != 0;
+    a.L = (double)host_L;
+    a.rule = LJ_RDF_REF_GPU;
+    a.nbins = 256;
+    a.dr2 = host_dr2;
+    a.hist = s->rdf;
+    LJ_HIP(launch_rdf(st, a));
+  }
+  float pr = 0.f;
+  unsigned long long h[256];
+  LJ_HIP(hipMemcpyAsync(&pr, s->pressure, sizeof(float), hipMemcpyDeviceToHost, st));
+  if (host_RDF) LJ_HIP(hipMemcpyAsync(h, s->rdf, sizeof(h), hipMemcpyDeviceToHost, st));
+  LJ_HIP(hipStreamSynchronize(st));
+  if (host_pressure) *host_pressure = pr;
+  if (host_RDF)
+    for (int b = 0; b < 256; ++b) host_RDF[b] = (int)h[b];
+  return LJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void allocateNBodyArrays(float* vel[2], int numBodies)
+{
+  if (!vel) {
+    compat_report(fail(LJ_ERR_ARG, "allocateNBodyArrays: null argument"));
+    return;
+  }
+  allocateArray(&vel[0], numBodies);
+  allocateArray(&vel[1], numBodies);
+}
+
+void allocateArray(float** dest, int number)
+{
+  if (!dest || number < 0) {
+    compat_report(fail(LJ_ERR_ARG, "allocateArray: bad argument"));
+    return;
+  }
+  float4* p = nullptr;
+  int rc = check_device();
+  if (rc == LJ_OK) rc = dev_alloc(&p, (size_t)number);
+  *dest = reinterpret_cast<float*>(p);
+  compat_report(rc);
+}
+
+void deleteNBodyArrays(float* vel[2])
+{
+  if (!vel) return;
+  deleteArray(vel[0]);
+  deleteArray(vel[1]);
+}
+
+void deleteArray(float* arr)
+{
+  if (!arr) return;
+  hipError_t e = hipFree(arr);
+  if (e != hipSuccess) compat_report(fail(LJ_ERR_DEVICE, "deleteArray: %s", hipGetErrorString(e)));
+}
+
+void allocateIntArray(int** dest, int number)
+{
+  if (!dest || number < 0) {
+    compat_report(fail(LJ_ERR_ARG, "allocateIntArray: bad argument"));
+    return;
+  }
+  int* p = nullptr;
+  int rc = check_device();
+  if (rc == LJ_OK) rc = dev_alloc(&p, (size_t)number);
+  *dest = p;
+  compat_report(rc);
+}
+
+void deleteIntArray(int* arr)
+{
+  if (!arr) return;
+  hipError_t e = hipFree(arr);
+  if (e != hipSuccess) compat_report(fail(LJ_ERR_DEVICE, "deleteIntArray: %s", hipGetErrorString(e)));
+}
+
+void copyArrayFromDevice(float* host, const float* device, unsigned int /*pbo*/, int numBodies)
+{
+  if (!host || !device || numBodies < 0) {
+    compat_report(fail(LJ_ERR_ARG, "copyArrayFromDevice: bad argument"));
+    return;
+  }
+  hipError_t e = hipMemcpy(host, device, sizeof(float) * 4 * (size_t)numBodies, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) compat_report(fail(LJ_ERR_DEVICE, "copyArrayFromDevice: %s", hipGetErrorString(e)));
+}
+
+void copyArrayToDevice(float* device, const float* host, int numBodies)
+{
+  if (!host || !device || numBodies < 0) {
+    compat_report(fail(LJ_ERR_ARG, "copyArrayToDevice: bad argument"));
+    return;
+  }
+  hipError_t e = hipMemcpy(device, host, sizeof(float) * 4 * (size_t)numBodies, hipMemcpyHostToDevice);
+  if (e != hipSuccess) compat_report(fail(LJ_ERR_DEVICE, "copyArrayToDevice: %s", hipGetErrorString(e)));
+}
+
+void registerGLBufferObject(unsigned int) {}
+void unregisterGLBufferObject(unsigned int) {}
+
+void threadSync(void)
+{
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) compat_report(fail(LJ_ERR_DEVICE, "threadSync: %s", hipGetErrorString(e)));
+}
+
+void calculateNForces(float* Pos, float* Force, float* host_pressure, int numBodies, float host_L,
+                      int Lperiodic, int* host_RDF, float host_dr2, int /*blockSize*/, int /*q*/)
+{
+  std::lock_guard<std::mutex> lk(g_compat_mu);
+  compat_report(compat_forces(Pos, Force, host_pressure, numBodies, host_L, Lperiodic, host_RDF, host_dr2));
+}
+
+void threadExit(void)
+{
+  std::lock_guard<std::mutex> lk(g_compat_mu);
+  for (auto& s : g_compat) compat_release(s);
+  g_compat.clear();
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// lj_host.cpp -- host-side helpers of libljhip.so: initial states of the
+// benchmark configurations, box rules and the exact binning tables of the RDF.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "lj_internal.h"
+
+namespace lj {
+
+namespace {
+constexpr double kPi = 3.141592653589793238462643;
+
+// SplitMix64 (Steele, Lea, Flood 2014): one 64-bit draw per call
+inline uint64_t splitmix_next(uint64_t& state)
+{
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+inline double uniform01(uint64_t& state)  // [0, 1) with 53 random bits
+{
+  return (double)(splitmix_next(state) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// fast_round of the reference (MDSystem.cpp:732-739)
+inline int ref_fast_round(float x) { return x > 0 ? (int)(x + 0.5f) : (int)(x - 0.5f); }
+
+// the reference's image index of a float separation r (MDSystem.cpp:274)
+inline int ref_image_index(float r, double L) { return ref_fast_round((float)(r / L)); }
+
+inline uint32_t fbits(float f)
+{
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+inline float bitsf(uint32_t u)
+{
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+}  // namespace
+
+double box_length(int N, double rho) { return std::pow(N / rho, 1. / 3.); }  // MDSystem.cpp:70
+
+float rdf_dr2_rule(int N)  // MDSystem.cpp:93-95
+{
+  const double a = 0.2 * std::sqrt(100. / N);
+  float dr2 = (float)(a > 0.05 ? a : 0.05);
+  if (250 * dr2 < 25.0) dr2 = (float)(25.0 / 250);
+  return dr2;
+}
+
+void lattice_positions(int N, double L, float* pos4)  // MDSystem.cpp:150-168
+{
+  const int n = (int)std::ceil(std::pow((double)N, 1. / 3.));
+  const double dL = L / n;
+  for (int k = 0; k < N; ++k) {
+    const int ix = k % n, iy = (k / n) % n, iz = k / (n * n);
+    pos4[4 * k + 0] = (float)((ix + 0.5) * dL);
+    pos4[4 * k + 1] = (float)((iy + 0.5) * dL);
+    pos4[4 * k + 2] = (float)((iz + 0.5) * dL);
+    pos4[4 * k + 3] = (float)(L / 150.f);
+  }
+}
+
+void gaussian_velocities(int N, double T0, uint64_t seed, float* vel4)
+{
+  // component m = 3k + c of particle k; Box-Muller pairs in m order
+  uint64_t st = seed;
+  const double sT = std::sqrt(T0);
+  const int M = 3 * N;
+  for (int m = 0; m < M; m += 2) {
+    const double u1 = 1.0 - uniform01(st);
+    const double u2 = uniform01(st);
+    const double rad = std::sqrt(-2.0 * std::log(u1));
+    vel4[4 * (m / 3) + m % 3] = (float)(sT * (rad * std::cos(2.0 * kPi * u2)));
+    if (m + 1 < M) vel4[4 * ((m + 1) / 3) + (m + 1) % 3] = (float)(sT * (rad * std::sin(2.0 * kPi * u2)));
+  }
+  for (int k = 0; k < N; ++k) vel4[4 * k + 3] = 0.f;
+  // CorrectTotalMomentum (MDSystem.cpp:183-202)
+  double px = 0., py = 0., pz = 0., mass = 0.;
+  for (int k = 0; k < N; ++k) {
+    px += vel4[4 * k];
+    py += vel4[4 * k + 1];
+    pz += vel4[4 * k + 2];
+    mass += 1.;
+  }
+  for (int k = 0; k < N; ++k) {
+    vel4[4 * k] = (float)(vel4[4 * k] + -px / mass);
+    vel4[4 * k + 1] = (float)(vel4[4 * k + 1] + -py / mass);
+    vel4[4 * k + 2] = (float)(vel4[4 * k + 2] + -pz / mass);
+  }
+  // RenormalizeVelocities(true) (MDSystem.cpp:375-385, KineticTemperature :361-373)
+  double tk = 0.;
+  for (int k = 0; k < N; ++k) {
+    const float* v = vel4 + 4 * k;
+    const float s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    tk += s;
+  }
+  tk *= 1. / 3. / N;
+  for (int k = 0; k < 3 * N; ++k) {
+    float& v = vel4[4 * (k / 3) + k % 3];
+    v = (float)(v * std::sqrt(T0 / tk));
+  }
+}
+
+void cpu_minimage_thresholds(double L, float thr[4])
+{
+  // k(x) = fast_round((float)(x/L)) is non-decreasing in x; bisect on the
+  // ordered bit patterns of positive floats for the first x with k(x) >= m.
+  for (int m = 1; m <= 4; ++m) {
+    uint32_t lo = fbits(0.0f);
+    uint32_t hi = fbits((float)((m + 1) * L));
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (ref_image_index(bitsf(mid), L) >= m) hi = mid;
+      else lo = mid;
+    }
+    thr[m - 1] = bitsf(hi);
+  }
+}
+
+void rdf_edges2(int nbins, double rmax, float* edges2)
+{
+  const double dr = rmax / nbins;
+  for (int k = 0; k <= nbins; ++k) {
+    const double e = (double)k * dr;
+    edges2[k] = (float)(e * e);
+  }
+}
+
+}  // namespace lj

@@ -1,0 +1,127 @@
+// lj_internal.h -- shared declarations of the HIP kernels' launchers and the
+// host helpers of libljhip.so (not part of the public C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lj_hip.h"
+
+namespace lj {
+
+// Observables block kept on the device (same layout as lj_obs).
+static_assert(sizeof(lj_obs) == 13 * 8, "lj_obs layout");
+
+// Block-partial fields written by the per-particle kernels; one row of
+// kBlkFields doubles per block, reduced in block order by the finalisers.
+enum BlkField {
+  BLK_K2 = 0,   // sum_i fl32(vx^2+vy^2+vz^2)          (2 K)
+  BLK_E = 1,    // sum_i e_i, e_i = sum_j (s/r)^12-(s/r)^6
+  BLK_B = 2,    // sum_i b_i, b_i = sum_j (s/r)^6 (2 (s/r)^6 - 1)
+  BLK_TV2 = 3,  // TVN: sum_i fl32(|v'|^2)
+  kBlkFields = 4
+};
+
+constexpr int kBlock = 256;  // threads per workgroup of the per-particle kernels
+
+// Geometry of one force evaluation (i-tiles x j-slices).
+struct ForcePlan {
+  int N = 0;
+  int S = 1;        // j slices
+  int jchunk = 0;   // j per slice
+  int Npad = 0;     // row pitch of the partial buffers
+  int ipl = 1;      // i per lane
+  int gx = 0;       // workgroups along i
+};
+ForcePlan make_force_plan(int N, int jslices_override, int ipl);
+
+// Scalars of the pair kernel for one box.
+struct PairScalars {
+  int periodic = 1;   // fixed-point minimum image
+  double L = 1.0;
+  double qscale = 0.; // 2^32 / L (periodic)
+  float C1 = 1.f;     // (s/r)^2 = C1 * rcp(r2_internal)
+  double FS = 1.;     // force = FS * sum s_raw * d_raw
+  float thr = 0.f;    // guard-all threshold in internal r^2 units (0 = self by index only)
+  double eps = 1., sigma = 1.;
+};
+PairScalars make_pair_scalars(double L, int periodic, double eps, double sigma, bool guard_all);
+
+// Kernel launchers (enqueue only).
+hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, double qscale);
+hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                        const uint4* q, const float4* posf, float4* fpart, float* apart,
+                        bool compensated);
+// sum slices -> force (w = e_i); block partials of e, b
+hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                                const float4* fpart, const float* apart, float4* force,
+                                double* blk);
+// sum the rows of blk (nblk x kBlkFields) -> ev = {E_pot, W}; pressure_f (may be
+// null) = (float)(4 eps sum b)  [reference GPU 'pressure' convention]
+hipError_t launch_finalize_forces(hipStream_t st, const double* blk, int nblk, double eps,
+                                  double* ev, float* pressure_f);
+
+struct StepArgs {
+  float4* pos;
+  float4* vel;
+  float4* force;
+  float4* tF;
+  uint4* q;
+  const float4* fpart;
+  const float* apart;
+  double* blk;
+  lj_obs* obs;
+  double* hist;      // 5 per step
+  int64_t* hist_base;
+  int hist_cap;
+  int N;
+  double dt, L, T0, rho, eps, qscale, FS;
+  int bc, canonical, S, Npad;
+};
+hipError_t launch_drift(hipStream_t st, const StepArgs& a);
+hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a);
+hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a);
+hipError_t launch_tvn_b(hipStream_t st, const StepArgs& a);
+// observables of the step; advance t/step, write history
+hipError_t launch_finalize_step(hipStream_t st, const StepArgs& a);
+// observables of the current state (no time advance): K from vel, V/W from blk
+hipError_t launch_params(hipStream_t st, const StepArgs& a, int include_force_terms);
+hipError_t launch_set_hist_base(hipStream_t st, lj_obs* obs, int64_t* base);
+// velocity rescale by sqrt(T0/Tkin) or by a host factor (factor > 0)
+hipError_t launch_kinetic_partials(hipStream_t st, const float4* vel, int N, double* blk);
+hipError_t launch_scale_velocities(hipStream_t st, float4* vel, int N, const double* blk,
+                                   int nblk, double T0, double factor, lj_obs* obs);
+hipError_t launch_apply_bc(hipStream_t st, float4* pos, float4* vel, int N, int bc, double L);
+hipError_t launch_reset_averaging(hipStream_t st, lj_obs* obs);
+// obs->V, obs->W from ev = {E_pot, W}
+hipError_t launch_store_vw(hipStream_t st, const double* ev, lj_obs* obs);
+// obs->t = 0, obs->step = 0
+hipError_t launch_reset_clock(hipStream_t st, lj_obs* obs);
+
+// RDF
+struct RdfArgs {
+  const float4* pos;
+  int N;
+  int periodic;
+  double L;
+  int rule;
+  int nbins;
+  float dr2;
+  const float* edges2;  // RBINS
+  float rbin_inv;       // nbins / rmax
+  float thr[4];         // REF_CPU minimum-image thresholds
+  unsigned long long* hist;
+};
+hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
+hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);
+
+// host helpers (lj_host.cpp)
+void lattice_positions(int N, double L, float* pos4);
+void gaussian_velocities(int N, double T0, uint64_t seed, float* vel4);
+double box_length(int N, double rho);
+float rdf_dr2_rule(int N);
+// smallest positive float x with fast_round((float)(x / L)) >= m, m = 1..4
+void cpu_minimage_thresholds(double L, float thr[4]);
+void rdf_edges2(int nbins, double rmax, float* edges2);
+
+}  // namespace lj

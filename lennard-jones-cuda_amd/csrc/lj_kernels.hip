@@ -1,0 +1,928 @@
+// lj_kernels.hip -- gfx950 kernels of the Lennard-Jones time step.
+//
+// Force pass (the hot path; reference MDSystem.cu:21-138, MDSystem.cpp:260-309):
+//   * one i particle per lane (or IPL per lane), wave64, 4 waves per workgroup;
+//   * the j range is split into S slices (grid.y) so that N = 16k yields
+//     8 waves per SIMD on 256 CUs; each wave streams its slice's j particles
+//     through SGPRs (uniform scalar loads; every VALU op takes the j operand
+//     straight from an SGPR, so there is no LDS traffic and no barrier);
+//   * periodic boxes use 32-bit fixed-point coordinates q = x * 2^32 / L:
+//     the unsigned difference wraps modulo 2^32, which *is* the minimum image
+//     (no round/div per pair); the wall / open box uses the float coordinates;
+//   * per pair: 3 sub + 3 cvt + 3 (r^2) + v_rcp_f32 + 1 (scale) + 2 ((s/r)^6)
+//     + 3 (w, s) + 3 (force) + 2 (energy sums) VALU ops;
+//   * partial sums per (slice, i) are written once; the cross-slice sum is a
+//     fixed-order FP64 pass fused into the integrator kernels.
+//   Energy and virial come from two per-i sums: a = sum (s/r)^6 and
+//   b = sum (s/r)^6 (2 (s/r)^6 - 1):  e_i = sum ((s/r)^12 - (s/r)^6) = (b - a)/2,
+//   per-i virial sum_j r.f_ij = 24 eps b.
+// RDF pass (MDSystem.cu:39-42, MDSystem.cpp:281-285): LDS-privatised uint32
+// histogram per workgroup, flushed with one 64-bit atomic per bin.
+// Integrator (MDSystem.cpp:438-510), BCs (:406-436), observables (:325-359):
+// per-particle kernels with the reference's double/float rounding pattern and
+// deterministic FP64 block reductions.
+#include "lj_internal.h"
+
+namespace lj {
+
+namespace {
+
+constexpr int kWaves = kBlock / 64;  // waves per workgroup
+constexpr int kJB = 32;              // j block of the compensated accumulation
+
+struct Acc {
+  float fx, fy, fz, a, b;
+};
+
+__device__ __forceinline__ void acc_zero(Acc& c) { c.fx = c.fy = c.fz = c.a = c.b = 0.f; }
+
+// Knuth TwoSum: hi + lo += x without loss
+__device__ __forceinline__ void two_sum(float& hi, float& lo, float x)
+{
+  float s = hi + x;
+  float bp = s - hi;
+  float err = (hi - (s - bp)) + (x - bp);
+  hi = s;
+  lo += err;
+}
+
+template <bool PERIODIC>
+struct PosT;
+template <>
+struct PosT<true> {
+  unsigned x, y, z;
+};
+template <>
+struct PosT<false> {
+  float x, y, z;
+};
+
+template <bool PERIODIC>
+__device__ __forceinline__ PosT<PERIODIC> load_pos(const uint4* q, const float4* pf, int j)
+{
+  PosT<PERIODIC> p;
+  if constexpr (PERIODIC) {
+    uint4 v = q[j];
+    p.x = v.x;
+    p.y = v.y;
+    p.z = v.z;
+  } else {
+    float4 v = pf[j];
+    p.x = v.x;
+    p.y = v.y;
+    p.z = v.z;
+  }
+  return p;
+}
+
+// GUARD: 0 no exclusion, 1 self by index (i == j), 2 pairs with r2 <= thr
+template <bool PERIODIC, int GUARD>
+__device__ __forceinline__ void pair(const PosT<PERIODIC>& pi, const PosT<PERIODIC>& pj, int i,
+                                     int j, float C1, float thr, Acc& c)
+{
+  float dx, dy, dz;
+  if constexpr (PERIODIC) {
+    dx = (float)(int)(pi.x - pj.x);  // modular difference == minimum image
+    dy = (float)(int)(pi.y - pj.y);
+    dz = (float)(int)(pi.z - pj.z);
+  } else {
+    dx = pi.x - pj.x;
+    dy = pi.y - pj.y;
+    dz = pi.z - pj.z;
+  }
+  float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+  float ir2 = __builtin_amdgcn_rcpf(r2) * C1;  // (sigma/r)^2
+  if constexpr (GUARD == 1) ir2 = (i == j) ? 0.f : ir2;
+  if constexpr (GUARD == 2) ir2 = (r2 > thr) ? ir2 : 0.f;
+  float ir6 = ir2 * ir2 * ir2;
+  float w = ir6 * __builtin_fmaf(ir6, 2.f, -1.f);  // (s/r)^6 (2 (s/r)^6 - 1)
+  float s = w * ir2;
+  c.fx = __builtin_fmaf(dx, s, c.fx);
+  c.fy = __builtin_fmaf(dy, s, c.fy);
+  c.fz = __builtin_fmaf(dz, s, c.fz);
+  c.a += ir6;
+  c.b += w;
+}
+
+template <bool PERIODIC, int GUARD, int IPL, bool COMP>
+__device__ __forceinline__ void j_range(const uint4* __restrict__ q, const float4* __restrict__ pf,
+                                        int jb, int je, const PosT<PERIODIC> (&pi)[IPL],
+                                        const int (&ii)[IPL], float C1, float thr, Acc (&hi)[IPL],
+                                        Acc (&lo)[IPL])
+{
+  for (int b0 = jb; b0 < je; b0 += kJB) {
+    const int b1 = min(je, b0 + kJB);
+    Acc t[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) acc_zero(t[k]);
+#pragma unroll 8
+    for (int j = b0; j < b1; ++j) {
+      const PosT<PERIODIC> pj = load_pos<PERIODIC>(q, pf, j);
+#pragma unroll
+      for (int k = 0; k < IPL; ++k) pair<PERIODIC, GUARD>(pi[k], pj, ii[k], j, C1, thr, t[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+      if constexpr (COMP) {
+        two_sum(hi[k].fx, lo[k].fx, t[k].fx);
+        two_sum(hi[k].fy, lo[k].fy, t[k].fy);
+        two_sum(hi[k].fz, lo[k].fz, t[k].fz);
+        two_sum(hi[k].a, lo[k].a, t[k].a);
+        two_sum(hi[k].b, lo[k].b, t[k].b);
+      } else {
+        hi[k].fx += t[k].fx;
+        hi[k].fy += t[k].fy;
+        hi[k].fz += t[k].fz;
+        hi[k].a += t[k].a;
+        hi[k].b += t[k].b;
+      }
+    }
+  }
+}
+
+// grid: (gx, S); block: kBlock.  fpart[s*Npad + i] = (fx, fy, fz, b), apart = a,
+// in internal units (force = FS * f).
+template <bool PERIODIC, bool GUARDALL, int IPL, bool COMP>
+__global__ __launch_bounds__(kBlock) void k_force(const uint4* __restrict__ q,
+                                                  const float4* __restrict__ pf,
+                                                  float4* __restrict__ fpart,
+                                                  float* __restrict__ apart, int N, int Npad,
+                                                  int jchunk, float C1, float thr)
+{
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = (blockIdx.x * kWaves + wv) * (64 * IPL);
+  if (i0 >= N) return;
+  const int s = blockIdx.y;
+  const int j0 = s * jchunk;
+  const int j1 = min(N, j0 + jchunk);
+
+  PosT<PERIODIC> pi[IPL];
+  int ii[IPL];
+  Acc hi[IPL], lo[IPL];
+#pragma unroll
+  for (int k = 0; k < IPL; ++k) {
+    ii[k] = i0 + k * 64 + lane;
+    pi[k] = load_pos<PERIODIC>(q, pf, min(ii[k], N - 1));
+    acc_zero(hi[k]);
+    acc_zero(lo[k]);
+  }
+
+  if constexpr (GUARDALL) {
+    j_range<PERIODIC, 2, IPL, COMP>(q, pf, j0, j1, pi, ii, C1, thr, hi, lo);
+  } else {
+    // the self pair lies only in [i0, i0 + 64*IPL): guard that segment alone
+    const int d0 = max(j0, i0), d1 = min(j1, i0 + 64 * IPL);
+    if (d0 >= d1) {
+      j_range<PERIODIC, 0, IPL, COMP>(q, pf, j0, j1, pi, ii, C1, thr, hi, lo);
+    } else {
+      j_range<PERIODIC, 0, IPL, COMP>(q, pf, j0, d0, pi, ii, C1, thr, hi, lo);
+      j_range<PERIODIC, 1, IPL, COMP>(q, pf, d0, d1, pi, ii, C1, thr, hi, lo);
+      j_range<PERIODIC, 0, IPL, COMP>(q, pf, d1, j1, pi, ii, C1, thr, hi, lo);
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < IPL; ++k) {
+    if (ii[k] < N) {
+      const size_t o = (size_t)s * Npad + ii[k];
+      if constexpr (COMP) {
+        fpart[o] = make_float4(hi[k].fx + lo[k].fx, hi[k].fy + lo[k].fy, hi[k].fz + lo[k].fz,
+                               hi[k].b + lo[k].b);
+        apart[o] = hi[k].a + lo[k].a;
+      } else {
+        fpart[o] = make_float4(hi[k].fx, hi[k].fy, hi[k].fz, hi[k].b);
+        apart[o] = hi[k].a;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// deterministic block reduction of NV doubles: butterfly inside the wave, then
+// the 4 wave results in wave order.  Row `out` receives the block totals.
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out, const int (&field)[NV])
+{
+  __shared__ double sh[kWaves][NV];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wv][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.;
+    for (int w = 0; w < kWaves; ++w) s += sh[w][threadIdx.x];
+    int f = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      if (k == (int)threadIdx.x) f = field[k];
+    out[f] = s;
+  }
+}
+
+// block-wide total of NV doubles, result broadcast to every thread
+template <int NV>
+__device__ __forceinline__ void block_allsum(double (&v)[NV])
+{
+  __shared__ double sh[kWaves][NV];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wv][k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.;
+    for (int w = 0; w < kWaves; ++w) s += sh[w][k];
+    v[k] = s;
+  }
+}
+
+// sum of one field over the rows of blk in a fixed order (every block of the
+// grid obtains the same bits)
+__device__ __forceinline__ double blk_total(const double* blk, int nblk, int field)
+{
+  double v[1] = {0.};
+  for (int r = threadIdx.x; r < nblk; r += kBlock) v[0] += blk[(size_t)r * kBlkFields + field];
+  block_allsum<1>(v);
+  return v[0];
+}
+
+struct Partial {
+  double fx, fy, fz, a, b;
+};
+
+__device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
+                                              const float* __restrict__ apart, int i, int S,
+                                              int Npad)
+{
+  Partial p{0., 0., 0., 0., 0.};
+  for (int s = 0; s < S; ++s) {
+    const float4 f = fpart[(size_t)s * Npad + i];
+    p.fx += f.x;
+    p.fy += f.y;
+    p.fz += f.z;
+    p.b += f.w;
+    p.a += apart[(size_t)s * Npad + i];
+  }
+  return p;
+}
+
+__device__ __forceinline__ unsigned fixq(float x, double qs)
+{
+  return (unsigned)(long long)__builtin_rint((double)x * qs);
+}
+
+__global__ __launch_bounds__(kBlock) void k_prep_q(const float4* __restrict__ pos,
+                                                   uint4* __restrict__ q, int N, double qs)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  const float4 p = pos[i];
+  q[i] = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), 0u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_forces(const float4* __restrict__ fpart,
+                                                          const float* __restrict__ apart,
+                                                          float4* __restrict__ force,
+                                                          double* __restrict__ blk, int N, int S,
+                                                          int Npad, double FS)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[2] = {0., 0.};
+  if (i < N) {
+    const Partial p = sum_slices(fpart, apart, i, S, Npad);
+    const double e = 0.5 * (p.b - p.a);
+    force[i] = make_float4((float)(FS * p.fx), (float)(FS * p.fy), (float)(FS * p.fz), (float)e);
+    v[0] = e;
+    v[1] = p.b;
+  }
+  const int fld[2] = {BLK_E, BLK_B};
+  block_sum_store<2>(v, blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize_forces(const double* __restrict__ blk,
+                                                            int nblk, double eps,
+                                                            double* __restrict__ ev,
+                                                            float* __restrict__ pressure_f)
+{
+  double v[2] = {0., 0.};
+  for (int r = threadIdx.x; r < nblk; r += kBlock) {
+    v[0] += blk[(size_t)r * kBlkFields + BLK_E];
+    v[1] += blk[(size_t)r * kBlkFields + BLK_B];
+  }
+  block_allsum<2>(v);
+  if (threadIdx.x == 0) {
+    if (ev) {
+      ev[0] = 2. * eps * v[0];   // E_pot = 4 eps sum_{i<j}
+      ev[1] = 12. * eps * v[1];  // W = sum_{i<j} r.f
+    }
+    if (pressure_f) *pressure_f = (float)(4. * eps * v[1]);  // MDSystem.cu:108,136
+  }
+}
+
+// ---------------------------------------------------------------------------
+// integrator kernels: the reference's arithmetic (double expressions stored
+// to float, MDSystem.cpp:447-505); FMA contraction is off so that given the
+// same forces the device reproduces the reference's bits.
+
+__device__ __forceinline__ void apply_bc_one(float4& p, float4& v, int bc, double L)
+{
+#pragma clang fp contract(off)
+  if (bc == LJ_BC_NONE) return;
+  if (bc == LJ_BC_PERIODIC) {  // MDSystem.cpp:414-424
+    if (p.x < 0.) p.x = (float)(p.x + L);
+    if (p.x > L) p.x = (float)(p.x - L);
+    if (p.y < 0.) p.y = (float)(p.y + L);
+    if (p.y > L) p.y = (float)(p.y - L);
+    if (p.z < 0.) p.z = (float)(p.z + L);
+    if (p.z > L) p.z = (float)(p.z - L);
+  } else {  // MDSystem.cpp:425-435
+    if (p.x < 0. && v.x < 0) v.x = -v.x;
+    if (p.x > L && v.x > 0) v.x = -v.x;
+    if (p.y < 0. && v.y < 0) v.y = -v.y;
+    if (p.y > L && v.y > 0) v.y = -v.y;
+    if (p.z < 0. && v.z < 0) v.z = -v.z;
+    if (p.z > L && v.z > 0) v.z = -v.z;
+  }
+}
+
+__device__ __forceinline__ float kin2(const float4& v)
+{
+#pragma clang fp contract(off)
+  float s = v.x * v.x + v.y * v.y + v.z * v.z;
+  return s;
+}
+
+template <bool MAKEQ, int CANON>
+__global__ __launch_bounds__(kBlock) void k_drift(StepArgs a)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.N) return;
+  float4 p = a.pos[i];
+  float4 v = a.vel[i];
+  const float4 f = a.force[i];
+  const double dt = a.dt;
+  p.x = (float)(p.x + (dt * v.x + dt * dt * f.x / 2.));  // MDSystem.cpp:447-449
+  p.y = (float)(p.y + (dt * v.y + dt * dt * f.y / 2.));
+  p.z = (float)(p.z + (dt * v.z + dt * dt * f.z / 2.));
+  if constexpr (CANON == LJ_EVN) {
+    v.x = (float)(v.x + dt * f.x / 2.);  // MDSystem.cpp:451-453
+    v.y = (float)(v.y + dt * f.y / 2.);
+    v.z = (float)(v.z + dt * f.z / 2.);
+    a.vel[i] = v;
+  } else {
+    a.tF[i] = make_float4(0.5f * f.x, 0.5f * f.y, 0.5f * f.z, 0.f);  // MDSystem.cpp:477-479
+  }
+  a.pos[i] = p;
+  if constexpr (MAKEQ) a.q[i] = make_uint4(fixq(p.x, a.qscale), fixq(p.y, a.qscale), fixq(p.z, a.qscale), 0u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_kick_evn(StepArgs a)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v3[3] = {0., 0., 0.};
+  if (i < a.N) {
+    const Partial pp = sum_slices(a.fpart, a.apart, i, a.S, a.Npad);
+    const double e = 0.5 * (pp.b - pp.a);
+    const float fx = (float)(a.FS * pp.fx), fy = (float)(a.FS * pp.fy), fz = (float)(a.FS * pp.fz);
+    a.force[i] = make_float4(fx, fy, fz, (float)e);
+    float4 v = a.vel[i];
+    const double dt = a.dt;
+    v.x = (float)(v.x + dt * fx / 2.);  // MDSystem.cpp:460-462
+    v.y = (float)(v.y + dt * fy / 2.);
+    v.z = (float)(v.z + dt * fz / 2.);
+    if (a.bc != LJ_BC_NONE) {
+      float4 p = a.pos[i];
+      apply_bc_one(p, v, a.bc, a.L);
+      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
+    }
+    a.vel[i] = v;
+    v3[0] = kin2(v);
+    v3[1] = e;
+    v3[2] = pp.b;
+  }
+  const int fld[3] = {BLK_K2, BLK_E, BLK_B};
+  block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tvn_a(StepArgs a)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v3[3] = {0., 0., 0.};
+  if (i < a.N) {
+    const Partial pp = sum_slices(a.fpart, a.apart, i, a.S, a.Npad);
+    const double e = 0.5 * (pp.b - pp.a);
+    const float fx = (float)(a.FS * pp.fx), fy = (float)(a.FS * pp.fy), fz = (float)(a.FS * pp.fz);
+    a.force[i] = make_float4(fx, fy, fz, (float)e);
+    float4 tf = a.tF[i];
+    tf.x += 0.5f * fx;  // MDSystem.cpp:486-488
+    tf.y += 0.5f * fy;
+    tf.z += 0.5f * fz;
+    a.tF[i] = tf;
+    const float4 v = a.vel[i];
+    const double dt = a.dt;
+    float4 tv;
+    tv.x = (float)(v.x + dt * tf.x / 2.);  // MDSystem.cpp:493-495
+    tv.y = (float)(v.y + dt * tf.y / 2.);
+    tv.z = (float)(v.z + dt * tf.z / 2.);
+    tv.w = 0.f;
+    v3[0] = kin2(tv);
+    v3[1] = e;
+    v3[2] = pp.b;
+  }
+  const int fld[3] = {BLK_TV2, BLK_E, BLK_B};
+  block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a, int nblk)
+{
+#pragma clang fp contract(off)
+  const double tv2 = blk_total(a.blk, nblk, BLK_TV2);
+  const double Tkin = tv2 * (1. / 3. / a.N);  // MDSystem.cpp:370
+  const double chi = sqrt(a.T0 / Tkin);      // MDSystem.cpp:499
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v1[1] = {0.};
+  if (i < a.N) {
+    float4 v = a.vel[i];
+    const float4 tf = a.tF[i];
+    const double dt = a.dt;
+    v.x = (float)((2. * chi - 1.) * v.x + chi * dt * tf.x);  // MDSystem.cpp:503-505
+    v.y = (float)((2. * chi - 1.) * v.y + chi * dt * tf.y);
+    v.z = (float)((2. * chi - 1.) * v.z + chi * dt * tf.z);
+    if (a.bc != LJ_BC_NONE) {
+      float4 p = a.pos[i];
+      apply_bc_one(p, v, a.bc, a.L);
+      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
+    }
+    a.vel[i] = v;
+    v1[0] = kin2(v);
+  }
+  __syncthreads();
+  const int fld[1] = {BLK_K2};
+  block_sum_store<1>(v1, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+// observables from block partials: CalculateParameters (MDSystem.cpp:325-359)
+__device__ __forceinline__ void observables(const StepArgs& a, int nblk, bool advance,
+                                            bool force_terms)
+{
+  double v[3] = {0., 0., 0.};
+  for (int r = threadIdx.x; r < nblk; r += kBlock) {
+    v[0] += a.blk[(size_t)r * kBlkFields + BLK_K2];
+    v[1] += a.blk[(size_t)r * kBlkFields + BLK_E];
+    v[2] += a.blk[(size_t)r * kBlkFields + BLK_B];
+  }
+  block_allsum<3>(v);
+  if (threadIdx.x != 0) return;
+  lj_obs* o = a.obs;
+  const int N = a.N;
+  const double K = v[0] / 2.;
+  const double T = 2. * K / 3. / N;
+  if (force_terms) {
+    o->V = 2. * a.eps * v[1];
+    o->W = 12. * a.eps * v[2];
+  }
+  const double Pvir = o->W / 3.;
+  const double P = (Pvir + N * T) / (N / a.rho);
+  o->K = K;
+  o->T = T;
+  o->P = P;
+  o->U = K + o->V;
+  o->Pshear = 0.;
+  o->av_iters += 1;
+  o->av_U_tot += o->U;
+  o->av_p_tot += P;
+  o->av_T_tot += T;
+  if (advance) {
+    o->t += a.dt;
+    const int64_t k = o->step - *a.hist_base;
+    o->step += 1;
+    if (a.hist && k >= 0 && k < a.hist_cap) {
+      double* h = a.hist + 5 * k;
+      h[0] = o->U;
+      h[1] = T;
+      h[2] = K;
+      h[3] = o->V;
+      h[4] = P;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize_step(StepArgs a, int nblk)
+{
+  observables(a, nblk, true, true);
+}
+
+__global__ __launch_bounds__(kBlock) void k_params(StepArgs a, int nblk, int force_terms)
+{
+  observables(a, nblk, false, force_terms != 0);
+}
+
+__global__ void k_set_hist_base(lj_obs* obs, int64_t* base) { *base = obs->step; }
+
+__global__ __launch_bounds__(kBlock) void k_kinetic_partials(const float4* __restrict__ vel, int N,
+                                                             double* __restrict__ blk)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v1[1] = {0.};
+  if (i < N) v1[0] = kin2(vel[i]);
+  const int fld[1] = {BLK_K2};
+  block_sum_store<1>(v1, blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+// RenormalizeVelocities (MDSystem.cpp:375-389): factor = sqrt(T0/Tkin) with
+// Tkin from the K2 partials, or the given factor when factor > 0
+__global__ __launch_bounds__(kBlock) void k_scale_velocities(float4* __restrict__ vel, int N,
+                                                             const double* __restrict__ blk,
+                                                             int nblk, double T0, double factor,
+                                                             lj_obs* obs)
+{
+#pragma clang fp contract(off)
+  double Tkin = 0.;
+  if (!(factor > 0.)) {
+    Tkin = blk_total(blk, nblk, BLK_K2) * (1. / 3. / N);
+    factor = sqrt(T0 / Tkin);
+  }
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < N) {
+    float4 v = vel[i];
+    v.x = (float)(v.x * factor);
+    v.y = (float)(v.y * factor);
+    v.z = (float)(v.z * factor);
+    vel[i] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && obs) {
+    if (Tkin > 0.) {
+      obs->K *= T0 / Tkin;  // MDSystem.cpp:386-388
+      obs->T = T0;
+    } else {
+      obs->K *= factor * factor;
+      obs->T *= factor * factor;
+    }
+    obs->U = obs->K + obs->V;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_apply_bc(float4* __restrict__ pos,
+                                                     float4* __restrict__ vel, int N, int bc,
+                                                     double L)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  float4 p = pos[i], v = vel[i];
+  apply_bc_one(p, v, bc, L);
+  pos[i] = p;
+  vel[i] = v;
+}
+
+__global__ void k_store_vw(const double* ev, lj_obs* obs)
+{
+  obs->V = ev[0];
+  obs->W = ev[1];
+}
+
+__global__ void k_reset_clock(lj_obs* obs)
+{
+  obs->t = 0.;
+  obs->step = 0;
+}
+
+__global__ void k_reset_averaging(lj_obs* obs)
+{
+  obs->av_U_tot = obs->av_T_tot = obs->av_p_tot = 0.;  // MDSystem.cpp:701-705
+  obs->av_iters = 0;
+}
+
+// ---------------------------------------------------------------------------
+// RDF
+
+__device__ __forceinline__ float mi_cpu(float r, double L, const float (&t)[4])
+{
+#pragma clang fp contract(off)
+  // k = fast_round((float)(r / L)) (MDSystem.cpp:274-276, :732-739) as a step
+  // function of r with host-computed thresholds t[m-1] = min{x : k(x) >= m}
+  int k = (r >= t[0]) + (r >= t[1]) + (r >= t[2]) + (r >= t[3]);
+  k -= (r <= -t[0]) + (r <= -t[1]) + (r <= -t[2]) + (r <= -t[3]);
+  return (float)((double)r - (double)k * L);
+}
+
+__device__ __forceinline__ float mi_gpu(float d, float Lf)
+{
+#pragma clang fp contract(off)
+  // MDSystem.cu:31-33: r += -L * lround(r / L) with a correctly rounded division
+  const float k = __builtin_roundf(d / Lf);
+  return d - Lf * k;
+}
+
+template <int RULE>
+__global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, int N,
+                                                int jchunk, int periodic, double L, float dr2,
+                                                int nbins, const float* __restrict__ edges2,
+                                                float rbin_inv, float t0, float t1, float t2,
+                                                float t3, unsigned long long* __restrict__ hist)
+{
+#pragma clang fp contract(off)
+  extern __shared__ unsigned int sh_rdf[];
+  unsigned int* h = sh_rdf;
+  float* e2 = reinterpret_cast<float*>(sh_rdf + nbins);
+  for (int b = threadIdx.x; b < nbins; b += kBlock) h[b] = 0u;
+  if constexpr (RULE == LJ_RDF_RBINS)
+    for (int b = threadIdx.x; b <= nbins; b += kBlock) e2[b] = edges2[b];
+  __syncthreads();
+  const float Lf = (float)L;
+  const float thr[4] = {t0, t1, t2, t3};
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = i < N;
+  const float4 pi = pos[min(i, N - 1)];
+  const int j0 = blockIdx.y * jchunk;
+  const int j1 = min(N, j0 + jchunk);
+  for (int j = j0; j < j1; ++j) {
+    const float4 pj = pos[j];
+    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    if constexpr (RULE == LJ_RDF_REF_GPU) {
+      if (periodic) {
+        rx = mi_gpu(rx, Lf);
+        ry = mi_gpu(ry, Lf);
+        rz = mi_gpu(rz, Lf);
+      }
+      const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+      if (valid && r2 > 1e-6f) {
+        int b = (int)__builtin_floorf(r2 / dr2);
+        b = min(b, nbins - 1);
+        atomicAdd(&h[b], 1u);
+      }
+    } else {
+      if (periodic) {
+        rx = mi_cpu(rx, L, thr);
+        ry = mi_cpu(ry, L, thr);
+        rz = mi_cpu(rz, L, thr);
+      }
+      const float r2 = rx * rx + ry * ry + rz * rz;  // float, no contraction (MDSystem.cpp:279)
+      if (!valid || j == i) continue;
+      if constexpr (RULE == LJ_RDF_REF_CPU) {
+        const double qq = __builtin_floor((double)r2 / (double)dr2);  // MDSystem.cpp:282
+        if (qq < (double)nbins) atomicAdd(&h[(int)qq], 1u);
+      } else {
+        if (!(r2 < e2[nbins])) continue;
+        int k = (int)(__builtin_sqrtf(r2) * rbin_inv);
+        k = min(max(k, 0), nbins - 1);
+        while (k > 0 && e2[k] > r2) --k;
+        while (k + 1 < nbins && e2[k + 1] <= r2) ++k;
+        atomicAdd(&h[k], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += kBlock)
+    if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+template <bool PERIODIC, bool GUARDALL, int IPL, bool COMP>
+hipError_t launch_force_t(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                          const uint4* q, const float4* posf, float4* fpart, float* apart)
+{
+  dim3 grid(pl.gx, pl.S);
+  hipLaunchKernelGGL((k_force<PERIODIC, GUARDALL, IPL, COMP>), grid, dim3(kBlock), 0, st, q,
+                     posf, fpart, apart, pl.N, pl.Npad, pl.jchunk, ps.C1, ps.thr);
+  return hipGetLastError();
+}
+
+template <bool PERIODIC, bool GUARDALL>
+hipError_t launch_force_c(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                          const uint4* q, const float4* posf, float4* fpart, float* apart,
+                          bool comp)
+{
+  if (pl.ipl == 2)
+    return comp ? launch_force_t<PERIODIC, GUARDALL, 2, true>(st, pl, ps, q, posf, fpart, apart)
+                : launch_force_t<PERIODIC, GUARDALL, 2, false>(st, pl, ps, q, posf, fpart, apart);
+  return comp ? launch_force_t<PERIODIC, GUARDALL, 1, true>(st, pl, ps, q, posf, fpart, apart)
+              : launch_force_t<PERIODIC, GUARDALL, 1, false>(st, pl, ps, q, posf, fpart, apart);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+
+ForcePlan make_force_plan(int N, int jslices_override, int ipl)
+{
+  ForcePlan pl;
+  pl.N = N;
+  pl.ipl = (ipl == 2) ? 2 : 1;
+  const int waves_i = (N + 64 * pl.ipl - 1) / (64 * pl.ipl);
+  pl.gx = (waves_i + kWaves - 1) / kWaves;
+  int S;
+  if (jslices_override > 0) {
+    S = jslices_override;
+  } else {
+    // about 8 resident waves per SIMD on 256 CUs x 4 SIMDs
+    const int target = 8192;
+    S = (target + pl.gx * kWaves - 1) / (pl.gx * kWaves);
+    const int smax = (N + 63) / 64;  // at least 64 j per slice
+    S = max(1, min(S, smax));
+  }
+  S = max(1, min(S, N));
+  int jc = (N + S - 1) / S;
+  jc = (jc + 7) & ~7;
+  pl.jchunk = jc;
+  pl.S = (N + jc - 1) / jc;
+  pl.Npad = (N + 63) & ~63;
+  return pl;
+}
+
+PairScalars make_pair_scalars(double L, int periodic, double eps, double sigma, bool guard_all)
+{
+  PairScalars ps;
+  ps.periodic = periodic;
+  ps.L = L;
+  ps.eps = eps;
+  ps.sigma = sigma;
+  const double s2 = sigma * sigma;
+  if (periodic) {
+    const double u = L / 4294967296.0;  // length of one fixed-point unit
+    ps.qscale = 4294967296.0 / L;
+    ps.C1 = (float)(s2 / (u * u));
+    ps.FS = 24. * eps * u / s2;
+    ps.thr = guard_all ? (float)(1e-6 / (u * u)) : 0.f;
+  } else {
+    ps.qscale = 0.;
+    ps.C1 = (float)s2;
+    ps.FS = 24. * eps / s2;
+    ps.thr = guard_all ? 1e-6f : 0.f;
+  }
+  return ps;
+}
+
+hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, double qscale)
+{
+  hipLaunchKernelGGL(k_prep_q, dim3(nblocks(N)), dim3(kBlock), 0, st, pos, q, N, qscale);
+  return hipGetLastError();
+}
+
+hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                        const uint4* q, const float4* posf, float4* fpart, float* apart,
+                        bool compensated)
+{
+  const bool ga = ps.thr > 0.f;
+  if (ps.periodic)
+    return ga ? launch_force_c<true, true>(st, pl, ps, q, posf, fpart, apart, compensated)
+              : launch_force_c<true, false>(st, pl, ps, q, posf, fpart, apart, compensated);
+  return ga ? launch_force_c<false, true>(st, pl, ps, q, posf, fpart, apart, compensated)
+            : launch_force_c<false, false>(st, pl, ps, q, posf, fpart, apart, compensated);
+}
+
+hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                                const float4* fpart, const float* apart, float4* force,
+                                double* blk)
+{
+  hipLaunchKernelGGL(k_reduce_forces, dim3(nblocks(pl.N)), dim3(kBlock), 0, st, fpart, apart,
+                     force, blk, pl.N, pl.S, pl.Npad, ps.FS);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_forces(hipStream_t st, const double* blk, int nblk, double eps,
+                                  double* ev, float* pressure_f)
+{
+  hipLaunchKernelGGL(k_finalize_forces, dim3(1), dim3(kBlock), 0, st, blk, nblk, eps, ev,
+                     pressure_f);
+  return hipGetLastError();
+}
+
+hipError_t launch_drift(hipStream_t st, const StepArgs& a)
+{
+  const bool mq = a.bc == LJ_BC_PERIODIC;
+  dim3 g(nblocks(a.N)), b(kBlock);
+  if (a.canonical == LJ_EVN) {
+    if (mq) hipLaunchKernelGGL((k_drift<true, LJ_EVN>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_drift<false, LJ_EVN>), g, b, 0, st, a);
+  } else {
+    if (mq) hipLaunchKernelGGL((k_drift<true, LJ_TVN>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_drift<false, LJ_TVN>), g, b, 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a)
+{
+  hipLaunchKernelGGL(k_kick_evn, dim3(nblocks(a.N)), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a)
+{
+  hipLaunchKernelGGL(k_tvn_a, dim3(nblocks(a.N)), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tvn_b(hipStream_t st, const StepArgs& a)
+{
+  hipLaunchKernelGGL(k_tvn_b, dim3(nblocks(a.N)), dim3(kBlock), 0, st, a, nblocks(a.N));
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_step(hipStream_t st, const StepArgs& a)
+{
+  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(kBlock), 0, st, a, nblocks(a.N));
+  return hipGetLastError();
+}
+
+hipError_t launch_params(hipStream_t st, const StepArgs& a, int include_force_terms)
+{
+  hipLaunchKernelGGL(k_params, dim3(1), dim3(kBlock), 0, st, a, nblocks(a.N),
+                     include_force_terms);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_hist_base(hipStream_t st, lj_obs* obs, int64_t* base)
+{
+  hipLaunchKernelGGL(k_set_hist_base, dim3(1), dim3(1), 0, st, obs, base);
+  return hipGetLastError();
+}
+
+hipError_t launch_kinetic_partials(hipStream_t st, const float4* vel, int N, double* blk)
+{
+  hipLaunchKernelGGL(k_kinetic_partials, dim3(nblocks(N)), dim3(kBlock), 0, st, vel, N, blk);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_velocities(hipStream_t st, float4* vel, int N, const double* blk,
+                                   int nblk, double T0, double factor, lj_obs* obs)
+{
+  hipLaunchKernelGGL(k_scale_velocities, dim3(nblocks(N)), dim3(kBlock), 0, st, vel, N, blk, nblk,
+                     T0, factor, obs);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_bc(hipStream_t st, float4* pos, float4* vel, int N, int bc, double L)
+{
+  hipLaunchKernelGGL(k_apply_bc, dim3(nblocks(N)), dim3(kBlock), 0, st, pos, vel, N, bc, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_averaging(hipStream_t st, lj_obs* obs)
+{
+  hipLaunchKernelGGL(k_reset_averaging, dim3(1), dim3(1), 0, st, obs);
+  return hipGetLastError();
+}
+
+hipError_t launch_store_vw(hipStream_t st, const double* ev, lj_obs* obs)
+{
+  hipLaunchKernelGGL(k_store_vw, dim3(1), dim3(1), 0, st, ev, obs);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_clock(hipStream_t st, lj_obs* obs)
+{
+  hipLaunchKernelGGL(k_reset_clock, dim3(1), dim3(1), 0, st, obs);
+  return hipGetLastError();
+}
+
+hipError_t launch_rdf(hipStream_t st, const RdfArgs& a)
+{
+  const int gx = nblocks(a.N);
+  int S = max(1, min(2048 / max(gx, 1), (a.N + 255) / 256));
+  int jc = (a.N + S - 1) / S;
+  S = (a.N + jc - 1) / jc;
+  const size_t lds = sizeof(unsigned) * a.nbins + (a.rule == LJ_RDF_RBINS ? sizeof(float) * (a.nbins + 1) : 0);
+  dim3 grid(gx, S), blk(kBlock);
+#define LJ_RDF_ARGS a.pos, a.N, jc, a.periodic, a.L, a.dr2, a.nbins, a.edges2, a.rbin_inv, \
+                    a.thr[0], a.thr[1], a.thr[2], a.thr[3], a.hist
+  if (a.rule == LJ_RDF_REF_CPU)
+    hipLaunchKernelGGL(k_rdf<LJ_RDF_REF_CPU>, grid, blk, lds, st, LJ_RDF_ARGS);
+  else if (a.rule == LJ_RDF_REF_GPU)
+    hipLaunchKernelGGL(k_rdf<LJ_RDF_REF_GPU>, grid, blk, lds, st, LJ_RDF_ARGS);
+  else
+    hipLaunchKernelGGL(k_rdf<LJ_RDF_RBINS>, grid, blk, lds, st, LJ_RDF_ARGS);
+#undef LJ_RDF_ARGS
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n)
+{
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblocks(n)), dim3(kBlock), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace lj

@@ -1,0 +1,15 @@
+"""ljamd -- Python access to the MI355X Lennard-Jones time step (libljhip.so).
+
+The compute path is the HIP library; this package only declares its C-ABI
+(include/lj_hip.h) for ctypes and wraps a context in a small class.
+"""
+from ._lib import (ARRAY_FORCE, ARRAY_POS, ARRAY_VEL, BC_NONE, BC_PERIODIC, BC_WALL, EVN,
+                   LIB_PATH, PREC_COMPENSATED, PREC_FP32, RDF_RBINS, RDF_REF_CPU, RDF_REF_GPU,
+                   TVN, LJConfig, LJError, LJObs, SIGNATURES, check, lib)
+from .system import CompatArrays, LJSystem
+
+__all__ = [
+    "ARRAY_FORCE", "ARRAY_POS", "ARRAY_VEL", "BC_NONE", "BC_PERIODIC", "BC_WALL", "EVN", "TVN",
+    "LIB_PATH", "PREC_COMPENSATED", "PREC_FP32", "RDF_RBINS", "RDF_REF_CPU", "RDF_REF_GPU",
+    "LJConfig", "LJError", "LJObs", "SIGNATURES", "check", "lib", "CompatArrays", "LJSystem",
+]

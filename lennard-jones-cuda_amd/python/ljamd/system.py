@@ -1,0 +1,190 @@
+"""Python view of one lj_ctx: the device-resident box of the C-ABI.
+
+Method names follow the reference class MDSystem (src/library/MDSystem.h) so
+that test code reads like the reference's own call sites.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as _L
+from ._lib import check, lib, ptr
+
+
+class LJSystem:
+    """One box on the GPU: MDSystem's state and time step behind lj_hip.h."""
+
+    def __init__(self, N, rho, T0=1.0, canonical=_L.EVN, boundary=_L.BC_PERIODIC, eps=1.0,
+                 sigma=1.0, precision=_L.PREC_FP32, jslices=0, device=0, seed=None):
+        cfg = _L.LJConfig()
+        lib().lj_config_init(C.byref(cfg))
+        cfg.N, cfg.rho, cfg.T0 = int(N), float(rho), float(T0)
+        cfg.canonical, cfg.boundary = int(canonical), int(boundary)
+        cfg.eps, cfg.sigma = float(eps), float(sigma)
+        cfg.precision, cfg.jslices, cfg.device = int(precision), int(jslices), int(device)
+        h = C.c_void_p()
+        check(lib().lj_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.cfg = cfg
+        self.N = int(N)
+        if seed is not None:
+            self.sample_initial_state(seed)
+
+    # lifetime
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().lj_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # state
+    @property
+    def L(self):
+        return lib().lj_box_length(self.h)
+
+    def sample_initial_state(self, seed):
+        check(lib().lj_sample_initial_state(self.h, int(seed)))
+
+    def set_array(self, kind, a4):
+        a4 = np.ascontiguousarray(a4, np.float32).reshape(-1)
+        if a4.size != 4 * self.N:
+            raise ValueError("expected float32[4N]")
+        check(lib().lj_set_array(self.h, int(kind), ptr(a4)))
+
+    def get_array(self, kind):
+        out = np.empty(4 * self.N, np.float32)
+        check(lib().lj_get_array(self.h, int(kind), ptr(out)))
+        return out
+
+    def set_state(self, pos4, vel4=None, force4=None):
+        self.set_array(_L.ARRAY_POS, pos4)
+        if vel4 is not None:
+            self.set_array(_L.ARRAY_VEL, vel4)
+        if force4 is not None:
+            self.set_array(_L.ARRAY_FORCE, force4)
+
+    @property
+    def pos(self):
+        return self.get_array(_L.ARRAY_POS)
+
+    @property
+    def vel(self):
+        return self.get_array(_L.ARRAY_VEL)
+
+    @property
+    def force(self):
+        return self.get_array(_L.ARRAY_FORCE)
+
+    # MDSystem methods
+    def CalculateForces(self):
+        o = _L.LJObs()
+        check(lib().lj_compute_forces(self.h, C.byref(o)))
+        return o.as_dict()
+
+    def CalculateParameters(self):
+        o = _L.LJObs()
+        check(lib().lj_compute_parameters(self.h, C.byref(o)))
+        return o.as_dict()
+
+    def observables(self):
+        o = _L.LJObs()
+        check(lib().lj_get_observables(self.h, C.byref(o)))
+        return o.as_dict()
+
+    def Integrate(self, dt, nsteps=1):
+        """nsteps of MDSystem::Integrate; returns per-step {U,T,K,V,P} [nsteps, 5]."""
+        hist = np.zeros((max(nsteps, 1), 5), np.float64)
+        check(lib().lj_integrate(self.h, float(dt), int(nsteps), ptr(hist)))
+        return hist[:nsteps]
+
+    def rdf(self, rule=_L.RDF_REF_CPU, nbins=256, dr2=0.0, rmax=0.0, accumulate=False):
+        counts = np.zeros(nbins, np.uint64)
+        check(lib().lj_rdf(self.h, int(rule), int(nbins), float(dr2), float(rmax), ptr(counts),
+                           int(bool(accumulate))))
+        return counts
+
+    def RenormalizeVelocities(self, recompute_tkin=False):
+        check(lib().lj_renormalize_velocities(self.h, int(bool(recompute_tkin))))
+
+    def RenormalizeVelocitiesToEnergy(self, ust):
+        check(lib().lj_renormalize_to_energy(self.h, float(ust)))
+
+    def resetAveraging(self):
+        check(lib().lj_reset_averaging(self.h))
+
+    def ApplyBoundaryConditions(self):
+        check(lib().lj_apply_boundary_conditions(self.h))
+
+    def set_mode(self, canonical, boundary):
+        check(lib().lj_set_mode(self.h, int(canonical), int(boundary)))
+
+    def synchronize(self):
+        check(lib().lj_synchronize(self.h))
+
+    def device_arrays(self):
+        p, v, f = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().lj_device_arrays(self.h, C.byref(p), C.byref(v), C.byref(f)))
+        return p.value, v.value, f.value
+
+    def time_force_kernel(self, reps=20):
+        us = C.c_double()
+        check(lib().lj_time_force_kernel(self.h, int(reps), C.byref(us)))
+        return us.value
+
+    def forces_device(self, d_pos, d_force, d_ev=None, stream=None):
+        check(lib().lj_forces_device(self.h, C.c_void_p(d_pos), C.c_void_p(d_force),
+                                     C.c_void_p(d_ev) if d_ev else None,
+                                     C.c_void_p(stream) if stream else None))
+
+
+# ---- the compatibility set (reference MDSystem.cu:155-299) ----
+
+class CompatArrays:
+    """Device arrays obtained through allocateArray / copyArrayToDevice."""
+
+    def __init__(self, N):
+        self.N = N
+        self.d_pos = C.c_void_p()
+        self.d_force = C.c_void_p()
+        lib().allocateArray(C.byref(self.d_pos), N)
+        lib().allocateArray(C.byref(self.d_force), N)
+        if not self.d_pos.value or not self.d_force.value:
+            raise _L.LJError(_L.LJ_ERR_ALLOC, lib().lj_last_error().decode())
+
+    def close(self):
+        if self.d_pos.value:
+            lib().deleteArray(self.d_pos)
+        if self.d_force.value:
+            lib().deleteArray(self.d_force)
+        self.d_pos = C.c_void_p()
+        self.d_force = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def calculate(self, pos4, L, periodic, dr2, block=256):
+        """copyArrayToDevice + calculateNForces + copyArrayFromDevice, as
+        MDSystem::CalculateForces' GPU branch (MDSystem.cpp:242-250)."""
+        pos4 = np.ascontiguousarray(pos4, np.float32)
+        lib().copyArrayToDevice(self.d_pos, ptr(pos4), self.N)
+        pressure = C.c_float(0.0)
+        rdf = np.zeros(256, np.int32)
+        lib().calculateNForces(self.d_pos, self.d_force, C.byref(pressure), self.N, float(L),
+                               int(periodic), ptr(rdf), float(dr2), int(block), 1)
+        frc = np.zeros(4 * self.N, np.float32)
+        lib().copyArrayFromDevice(ptr(frc), self.d_force, 0, self.N)
+        return frc, pressure.value, rdf

@@ -1,0 +1,155 @@
+"""Force / energy / virial parity of the HIP force pass (context API).
+
+Reference: MDSystem.cpp:237-311 (CPU path), MDSystem.cu:21-138 (GPU kernel).
+Checks, on identical seeded states:
+  * against the FP64 oracle: per-particle forces, E_pot, virial W;
+  * against the reference-faithful CPU path (oracle restatement, bit-exact to
+    the compiled reference): forces, V and the virial pressure term;
+at N = 400 (config 1), 2048, 16384 (config 2) with periodic and hard-wall
+boxes, plus the exact config lattice states.
+
+Tolerances (FP32 pair arithmetic, FP64 cross-slice sums):
+  forces vs FP64:  global ||dF||/||F|| <= 2e-6,
+                   per particle |dF_i| / max(|F_i|, F_rms) <= 1e-5;
+  E_pot, W vs FP64: relative <= 1e-6;
+  forces vs reference CPU path: per particle <= 1e-5 + the reference's own
+                   deviation from FP64 at that particle (it accumulates in float).
+"""
+import numpy as np
+import pytest
+
+import ljamd
+from helpers import force_errors, perturbed_state, rel, xyz
+from oracle_lib import ORC, forces_fp64, forces_ref, lattice_state
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    (400, 0.05, 1.4, ljamd.BC_PERIODIC),
+    (400, 0.05, 1.4, ljamd.BC_WALL),
+    (2048, 0.85, 1.0, ljamd.BC_PERIODIC),
+    (2048, 0.85, 1.0, ljamd.BC_WALL),
+    (2048, 0.3, 1.0, ljamd.BC_NONE),
+    (5000, 0.6, 1.0, ljamd.BC_PERIODIC),
+    (16384, 0.85, 1.0, ljamd.BC_PERIODIC),
+    (16384, 0.85, 1.0, ljamd.BC_WALL),
+]
+
+
+def _gpu_forces(N, rho, bc, pos, precision=ljamd.PREC_FP32, jslices=0):
+    with ljamd.LJSystem(N, rho, 1.0, boundary=bc, precision=precision, jslices=jslices) as s:
+        s.set_state(pos, np.zeros_like(pos))
+        obs = s.CalculateForces()
+        return s.force, obs
+
+
+@pytest.mark.parametrize("N,rho,T0,bc", CASES)
+def test_forces_vs_fp64(N, rho, T0, bc):
+    L, pos, _ = perturbed_state(N, rho, T0, seed=11)
+    print(f"\n[case] forces N={N} rho={rho} bc={bc}", flush=True)
+    f4, obs = _gpu_forces(N, rho, bc, pos)
+    F64, ew, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
+    err = force_errors(xyz(f4), F64)
+    eE, eW = rel(obs["V"], E), rel(obs["W"], W)
+    ew_err = np.max(np.abs(f4.reshape(N, 4)[:, 3] - ew[:, 0]) / np.maximum(np.abs(ew[:, 0]), 1.0))
+    print(f"  vs FP64: {err}  E rel {eE:.2e}  W rel {eW:.2e}  Force.w max {ew_err:.2e}")
+    assert err["global_rel"] <= 2e-6
+    assert err["max_rel_floor"] <= 1e-5
+    assert eE <= 1e-6 and eW <= 1e-6
+    assert ew_err <= 1e-5
+
+
+@pytest.mark.parametrize("N,rho,T0,bc", [c for c in CASES if c[0] <= 5000])
+def test_forces_vs_reference_cpu(N, rho, T0, bc):
+    L, pos, _ = perturbed_state(N, rho, T0, seed=23)
+    f4, obs = _gpu_forces(N, rho, bc, pos)
+    fr, V, P, Ps, _ = forces_ref(pos, L, bc, ORC.orc_rdf_dr2(N), nthreads=8)
+    F64, _, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
+    Fg, Fr = xyz(f4), xyz(fr)
+    n = np.linalg.norm(F64, axis=1)
+    floor = np.sqrt(np.mean(n ** 2))
+    own = np.linalg.norm(Fr - F64, axis=1)  # the reference's own deviation
+    d = np.linalg.norm(Fg - Fr, axis=1)
+    excess = np.max((d - own) / np.maximum(n, floor))
+    print(f"\n  vs ref CPU: global {force_errors(Fg, Fr)['global_rel']:.2e} excess {excess:.2e} "
+          f"ref-own {np.max(own / np.maximum(n, floor)):.2e}  V {rel(obs['V'], V):.2e} "
+          f"Pvir {rel(obs['W'] / 3, P):.2e}")
+    assert excess <= 1e-5
+    assert rel(obs["V"], V) <= 1e-5
+    assert rel(obs["W"] / 3.0, P) <= 1e-5  # reference P before CalculateParameters = W/3
+
+
+@pytest.mark.parametrize("N,rho", [(400, 0.05), (16384, 0.85), (2048, 0.85)])
+def test_config_lattice_state(N, rho):
+    """The configs' own initial state (lattice, no displacement)."""
+    L, pos, vel = lattice_state(N, rho, 1.0, seed=1)
+    f4, obs = _gpu_forces(N, rho, ljamd.BC_PERIODIC, pos)
+    F64, _, E, W = forces_fp64(pos, L, True)
+    Fg = xyz(f4)
+    scale = np.sqrt(np.mean(np.sum(F64 ** 2, axis=1))) + 1.0
+    dmax = np.max(np.linalg.norm(Fg - F64, axis=1)) / scale
+    print(f"\n  lattice N={N}: max|dF|/(F_rms+1) {dmax:.2e} E {rel(obs['V'], E):.2e}")
+    assert dmax <= 1e-5
+    assert rel(obs["V"], E) <= 1e-6 and rel(obs["W"], W) <= 2e-6
+
+
+@pytest.mark.parametrize("prec", [ljamd.PREC_FP32, ljamd.PREC_COMPENSATED])
+@pytest.mark.parametrize("jslices", [1, 3, 64])
+def test_slicing_and_precision_modes(prec, jslices):
+    N, rho = 3000, 0.85
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=5)
+    f4, obs = _gpu_forces(N, rho, ljamd.BC_PERIODIC, pos, precision=prec, jslices=jslices)
+    F64, _, E, W = forces_fp64(pos, L, True)
+    err = force_errors(xyz(f4), F64)
+    print(f"\n  prec={prec} S={jslices}: {err}")
+    assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5
+    assert rel(obs["V"], E) <= 1e-6
+
+
+def test_eps_sigma():
+    N, rho = 1500, 0.5
+    eps, sigma = 1.7, 1.15
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=3)
+    pos = pos.reshape(N, 4)
+    pos[:, :3] *= sigma  # same reduced configuration, scaled box
+    pos = pos.reshape(-1).copy()
+    with ljamd.LJSystem(N, rho / sigma ** 3, 1.0, eps=eps, sigma=sigma) as s:
+        s.set_state(pos, np.zeros_like(pos))
+        obs = s.CalculateForces()
+        f4 = s.force
+        Ls = s.L
+    F64, _, E, W = forces_fp64(pos, Ls, True, eps=eps, sigma=sigma)
+    err = force_errors(xyz(f4), F64)
+    assert err["global_rel"] <= 2e-6, err
+    assert rel(obs["V"], E) <= 1e-6 and rel(obs["W"], W) <= 1e-6
+
+
+def test_deterministic_repeat():
+    N, rho = 4096, 0.85
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=9)
+    a, oa = _gpu_forces(N, rho, ljamd.BC_PERIODIC, pos)
+    b, ob = _gpu_forces(N, rho, ljamd.BC_PERIODIC, pos)
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    assert oa["V"] == ob["V"] and oa["W"] == ob["W"]
+
+
+def test_translation_invariance_periodic():
+    """Shifting every particle by a vector (mod L) leaves forces unchanged."""
+    N, rho = 2000, 0.7
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=4)
+    p = pos.reshape(N, 4).copy()
+    p2 = p.copy()
+    p2[:, :3] = np.mod(p2[:, :3] + np.float32(0.37 * L), np.float32(L)).astype(np.float32)
+    a, oa = _gpu_forces(N, rho, ljamd.BC_PERIODIC, p.reshape(-1))
+    b, ob = _gpu_forces(N, rho, ljamd.BC_PERIODIC, p2.reshape(-1))
+    err = force_errors(xyz(b), xyz(a))
+    assert err["global_rel"] <= 1e-5 and rel(ob["V"], oa["V"]) <= 1e-6
+
+
+def test_newton_third_law():
+    N, rho = 3000, 0.85
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=8)
+    f4, _ = _gpu_forces(N, rho, ljamd.BC_PERIODIC, pos)
+    F = xyz(f4)
+    tot = np.abs(F.sum(axis=0)).max()
+    assert tot <= 1e-5 * np.abs(F).sum() / N * np.sqrt(N)

@@ -1,0 +1,131 @@
+"""Device-resident time step (MDSystem::Integrate, MDSystem.cpp:438-583) against
+the reference CPU path restated in the oracle (bit-exact to the compiled
+reference).  Forces differ from the reference's at the rounding level, so
+trajectories are compared step by step with the divergence reported; the
+integrator arithmetic itself follows the reference's double/float pattern."""
+import numpy as np
+import pytest
+
+import ljamd
+from helpers import perturbed_state, xyz
+from oracle_lib import ORC, OracleSystem, lattice_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(N, rho, T0, canon, bc, pos, vel):
+    g = ljamd.LJSystem(N, rho, T0, canonical=canon, boundary=bc)
+    g.set_state(pos, vel)
+    g.CalculateForces()
+    g.CalculateParameters()
+    o = OracleSystem(N, T0, rho, canon, bc, nthreads=8)
+    o.set_state(pos, vel)
+    o.forces()
+    o.params()
+    return g, o
+
+
+def test_initial_state_matches_oracle():
+    for N, rho, T0 in [(400, 0.05, 1.4), (16384, 0.85, 1.0), (2048, 0.85, 1.0)]:
+        L, pos, vel = lattice_state(N, rho, T0, seed=2024)
+        with ljamd.LJSystem(N, rho, T0, seed=2024) as s:
+            assert abs(s.L - L) == 0.0
+            assert np.array_equal(s.pos, pos)
+            v = s.vel.reshape(N, 4)
+            assert np.array_equal(v[:, :3], vel.reshape(N, 4)[:, :3])
+            obs = s.observables()
+            assert abs(obs["T"] - T0) <= 1e-6 * T0  # rescaled to exactly T0 (float state)
+            assert obs["t"] == 0.0 and obs["av_iters"] == 0
+        vv = vel.reshape(N, 4)[:, :3].astype(np.float64)
+        assert np.abs(vv.sum(axis=0)).max() <= 1e-4 * np.sqrt(N)
+
+
+@pytest.mark.parametrize("N,rho,T0,canon,bc,nsteps,tolU", [
+    (400, 0.05, 1.4, ljamd.EVN, ljamd.BC_PERIODIC, 400, 2e-5),   # config 1 slice
+    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_PERIODIC, 60, 2e-4),   # config 2 (TVN)
+    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_WALL, 60, 2e-4),       # config 2 (hard wall)
+    (1000, 0.3, 1.0, ljamd.EVN, ljamd.BC_WALL, 100, 1e-4),
+    (600, 0.2, 1.0, ljamd.EVN, ljamd.BC_NONE, 60, 1e-4),
+])
+def test_trajectory_vs_reference(N, rho, T0, canon, bc, nsteps, tolU):
+    L, pos, vel = lattice_state(N, rho, T0, seed=77)
+    print(f"\n[case] traj N={N} canon={canon} bc={bc}", flush=True)
+    g, o = _pair(N, rho, T0, canon, bc, pos, vel)
+    try:
+        hg = g.Integrate(0.005, nsteps)
+        ho = o.integrate(0.005, nsteps)
+        dU = np.abs(hg[:, 0] - ho[:, 0]) / N
+        dT = np.abs(hg[:, 1] - ho[:, 1])
+        dx = np.max(np.abs(xyz(g.pos) - xyz(o.pos)))
+        k = [0, nsteps // 4, nsteps // 2, nsteps - 1]
+        print(f"  |dU|/N at steps {k}: {dU[k]}  |dT|: {dT[k]}  final max|dx| {dx:.2e}")
+        assert dU.max() <= tolU
+        assert dT.max() <= 10 * tolU
+        obs = g.observables()
+        assert obs["step"] == nsteps and abs(obs["t"] - 0.005 * nsteps) < 1e-12
+    finally:
+        g.close()
+
+
+def test_evn_energy_conservation():
+    N, rho, T0 = 2048, 0.85, 1.0
+    L, pos, vel = lattice_state(N, rho, T0, seed=5)
+    with ljamd.LJSystem(N, rho, T0) as s:
+        s.set_state(pos, vel)
+        s.CalculateForces()
+        s.CalculateParameters()
+        h = s.Integrate(0.002, 1000)
+    U = h[:, 0] / N
+    drift = abs(U[-1] - U[200])
+    fluct = np.std(U[200:])
+    print(f"\n  EVN dt=0.002: u drift {drift:.2e} fluct {fluct:.2e}")
+    assert drift < 5e-3 and fluct < 5e-3
+
+
+def test_time_reversibility():
+    N, rho, T0 = 1000, 0.5, 1.0
+    L, pos, vel = perturbed_state(N, rho, T0, seed=6, amp=0.02)
+    with ljamd.LJSystem(N, rho, T0, boundary=ljamd.BC_NONE) as s:
+        s.set_state(pos, vel)
+        s.CalculateForces()
+        s.Integrate(0.002, 40)
+        v = s.vel
+        v.reshape(N, 4)[:, :3] *= -1
+        s.set_array(ljamd.ARRAY_VEL, v)
+        s.Integrate(0.002, 40)
+        back = s.pos
+    err = np.max(np.abs(xyz(back) - xyz(pos)))
+    assert err < 1e-3, err
+
+
+def test_graph_replay_deterministic():
+    N, rho, T0 = 3000, 0.85, 1.0
+    L, pos, vel = lattice_state(N, rho, T0, seed=9)
+    outs = []
+    for chunks in ([20], [7, 13]):
+        with ljamd.LJSystem(N, rho, T0, canonical=ljamd.TVN) as s:
+            s.set_state(pos, vel)
+            s.CalculateForces()
+            hs = [s.Integrate(0.005, c) for c in chunks]
+            outs.append((s.pos, s.vel, np.concatenate(hs)))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][2], outs[1][2])
+
+
+def test_renormalize_and_modes():
+    N, rho, T0 = 500, 0.3, 1.3
+    L, pos, vel = lattice_state(N, rho, T0, seed=4)
+    with ljamd.LJSystem(N, rho, T0) as s:
+        s.set_state(pos, 2.0 * vel)
+        s.CalculateForces()
+        s.CalculateParameters()
+        s.RenormalizeVelocities(True)
+        obs = s.CalculateParameters()
+        assert abs(obs["T"] - T0) <= 1e-6
+        s.RenormalizeVelocitiesToEnergy(1.0)
+        obs = s.CalculateParameters()
+        assert abs(obs["U"] / N - 1.0) <= 1e-5
+        s.set_mode(ljamd.TVN, ljamd.BC_WALL)
+        s.Integrate(0.005, 5)
+        with pytest.raises(ljamd.LJError):
+            s.set_mode(3, 0)
