@@ -55,6 +55,52 @@ def gpu_leg(steps, warmup):
     return {"t": t, "hist": hist, "obs": obs, "force_us": us, "reps": reps}
 
 
+def batch_leg(steps, plain_steps=10, nboxes=512):
+    """512 independent periodic EVN boxes (N ~ U{200..4000}, rho* ~ U[0.05,0.9],
+    T* ~ U[0.8,2.0], seeded) stepped by one captured batched graph, against a
+    loop that launches each box's own single-box step graph."""
+    import numpy as np
+
+    import ljamd
+
+    N, rho, T0 = ljamd.ensemble_boxes(nboxes, seed=7)
+    pairs = float(np.sum(N.astype(np.float64) ** 2))
+    b = ljamd.LJBatch(N, rho, T0, canonical=ljamd.EVN, seed=SEED)
+    log("[bench] batch autotune")
+    tuning, tune_us = b.autotune(reps=3)
+    b.integrate(DT, 3)
+    log(f"[bench] batch timing {steps} steps (tuning {tuning})")
+    t0 = time.perf_counter()
+    b.integrate(DT, steps)
+    tb = time.perf_counter() - t0
+    force_us = b.time_force(5)
+    b.close()
+    log(f"[bench] plain loop over {nboxes} single-box contexts, {plain_steps} steps")
+    ctxs = [ljamd.LJSystem(int(N[k]), float(rho[k]), float(T0[k]), canonical=ljamd.EVN,
+                           seed=SEED + k) for k in range(nboxes)]
+    for c in ctxs:  # builds each box's step graph
+        c.integrate_async(DT, 1)
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for c in ctxs:
+            c.integrate_async(DT, 1)
+    for c in ctxs:
+        c.synchronize()
+    tp = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    bsps = nboxes * steps / tb
+    psps = nboxes * plain_steps / tp
+    return {"boxes": nboxes, "steps": steps, "box_steps_per_s": bsps,
+            "plain_loop_steps": plain_steps, "plain_loop_box_steps_per_s": psps,
+            "speedup_vs_plain_loop": bsps / psps, "pairs_per_step": pairs,
+            "pair_interactions_per_s": pairs * steps / tb,
+            "tuning": {"ipl": tuning[0], "jchunk": tuning[1], "blocks_per_cu": tuning[2]},
+            "force_pass_us": force_us, "ms_per_step": 1000.0 * tb / steps}
+
+
 def cpu_leg(budget_s=15.0):
     """Reference CPU path (oracle/_ref/libref_cpu.so) on the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -92,6 +138,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--batch-steps", type=int, default=500)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
@@ -103,6 +151,7 @@ def main():
     value = pairs * a.steps / g["t"]
     ms_per_step = 1000.0 * g["t"] / a.steps
     achieved = FLOP_PER_PAIR * pairs / (g["force_us"] * 1e-6) / 1e12
+    batched = None if a.no_batch else batch_leg(a.batch_steps)
     cpu = None if a.no_cpu else cpu_leg()
     line = {
         "metric": METRIC,
@@ -128,6 +177,7 @@ def main():
                      "kernel": "k_force (periodic, fixed-point minimum image)",
                      "kernel_us": g["force_us"], "traffic": None},
         "cpu_baseline": cpu,
+        "batched": batched,
         "final_T": g["obs"]["T"],
         "final_u": g["obs"]["U"] / N,
     }

@@ -180,6 +180,10 @@ int lj_get_observables(lj_ctx* ctx, lj_obs* out);
  * per step {U, T, K, V, P}. */
 int lj_integrate(lj_ctx* ctx, double dt, int nsteps, double* hist);
 
+/* Enqueue nsteps of the captured step graph and return without waiting (no
+ * history copy).  lj_synchronize() waits; observables are read afterwards. */
+int lj_integrate_async(lj_ctx* ctx, double dt, int nsteps);
+
 /* Histogram of ordered-pair distances of the current positions.
  * counts: host uint64[nbins]; accumulate != 0 adds to the device histogram
  * kept from earlier calls with the same rule/nbins, 0 clears it first.
@@ -211,6 +215,62 @@ int lj_time_force_kernel(lj_ctx* ctx, int reps, double* us_per_launch);
  * the context's N.  Enqueued on `stream` (NULL = context stream), async. */
 int lj_forces_device(lj_ctx* ctx, const float* d_pos4, float* d_force4, double* d_ev,
                      void* stream);
+
+/* ------------------------------------------------------------------ */
+/* (4) batched ensemble: many independent boxes in one launch          */
+/* ------------------------------------------------------------------ */
+/* Every box runs the single-box time step above (reference
+ * MDSystem::Integrate per box) with its own N, density and temperature, in
+ * one set of launches per step: the force pass is one persistent launch that
+ * drains a work queue of (box, i-tile, j-slice) items; the per-particle
+ * kernels use workgroups that never straddle boxes; the whole step is
+ * captured once into a HIP graph.  Per-box energy and sub-volume particle
+ * counts are accumulated on the device (the fluctuation observables of
+ * src/tasks/run-fluctuations/include/run-fluctuations-aux.h:128-241 and
+ * MDSystem::Fluctuations, MDSystem.cpp:708-730). */
+
+typedef struct lj_batch lj_batch;
+
+/* Tiling of the batched force pass (picked by lj_batch_autotune):
+ *   ipl            i particles per lane (1 or 2): i-tile = 64*ipl particles
+ *   jchunk         j particles per work item (waves per box = tiles x slices)
+ *   blocks_per_cu  persistent grid of blocks_per_cu workgroups per CU pulling
+ *                  items from a queue; 0 = one wave per item (no queue) */
+typedef struct lj_batch_tuning {
+  int ipl;
+  int jchunk;
+  int blocks_per_cu;
+} lj_batch_tuning;
+
+/* per-box statistics, LJ_BATCH_NSTAT doubles per box, accumulated over steps:
+ * samples, sum U, sum U^2, sum V, sum V^2, sum K, sum K^2, sum P, sum P^2,
+ * sum n, sum n^2, sum n^3, sum n^4 (n = particles in the centred sub-cube
+ * of volume fraction f, MDSystem.cpp:711-723) */
+#define LJ_BATCH_NSTAT 13
+
+int lj_batch_create(int nboxes, const int* N, const double* rho, const double* T0, int canonical,
+                    int boundary, int device, lj_batch** out);
+int lj_batch_destroy(lj_batch* b);
+int lj_batch_num_particles(const lj_batch* b);
+/* lattice + seeded Gaussian state per box (box k uses seed + k), then forces */
+int lj_batch_sample_initial_state(lj_batch* b, uint64_t seed);
+/* box = -1: all boxes (flat float4[sum N]); else float4[N_box] */
+int lj_batch_set_array(lj_batch* b, int box, int type, const float* host4N);
+int lj_batch_get_array(lj_batch* b, int box, int type, float* host4N);
+int lj_batch_compute_forces(lj_batch* b);
+/* nsteps of the captured batched step */
+int lj_batch_integrate(lj_batch* b, double dt, int nsteps);
+int lj_batch_get_observables(lj_batch* b, lj_obs* out /* nboxes */);
+int lj_batch_set_subvolume(lj_batch* b, double fraction);
+int lj_batch_get_stats(lj_batch* b, double* out /* nboxes * LJ_BATCH_NSTAT */);
+int lj_batch_reset_stats(lj_batch* b);
+int lj_batch_set_tuning(lj_batch* b, const lj_batch_tuning* t);
+int lj_batch_get_tuning(lj_batch* b, lj_batch_tuning* t);
+/* average time of one batched force pass (HIP events), microseconds */
+int lj_batch_time_force(lj_batch* b, int reps, double* us);
+/* Times every candidate tiling on this batch (reps force passes each), keeps
+ * the fastest; cached in the file named by LJ_AUTOTUNE_CACHE when set. */
+int lj_batch_autotune(lj_batch* b, int reps, lj_batch_tuning* best, double* best_us);
 
 #ifdef __cplusplus
 }

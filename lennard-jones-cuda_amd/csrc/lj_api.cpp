@@ -11,13 +11,15 @@
 #include <vector>
 
 #include "lj_internal.h"
+#include "lj_util.h"
 
 using namespace lj;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
+namespace lj {
 int fail(int code, const char* fmt, ...)
 {
   char buf[512];
@@ -28,50 +30,8 @@ int fail(int code, const char* fmt, ...)
   g_err = buf;
   return code;
 }
-
-#define LJ_HIP(expr)                                                                       \
-  do {                                                                                     \
-    hipError_t e_ = (expr);                                                                \
-    if (e_ != hipSuccess)                                                                  \
-      return fail(LJ_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),     \
-                  __FILE__, __LINE__);                                                     \
-  } while (0)
-
-#define LJ_TRY(expr)        \
-  do {                      \
-    int rc_ = (expr);       \
-    if (rc_ != LJ_OK) return rc_; \
-  } while (0)
-
-template <class T>
-int dev_alloc(T** p, size_t count)
-{
-  *p = nullptr;
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(LJ_ERR_ALLOC, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
-  }
-  return LJ_OK;
-}
-
-template <class T>
-void dev_free(T*& p)
-{
-  if (p) (void)hipFree(p);
-  p = nullptr;
-}
-
-int check_device()
-{
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
-    return fail(LJ_ERR_DEVICE, "no HIP device available (the HIP path has no CPU fallback)");
-  return LJ_OK;
-}
-
-}  // namespace
+const std::string& last_error() { return g_err; }
+}  // namespace lj
 
 // ---------------------------------------------------------------------------
 // context
@@ -200,6 +160,31 @@ int ensure_hist(lj_ctx* c, int n)
   c->hist = h;
   c->hist_cap = cap;
   drop_graph(c);  // the graph holds the old pointer
+  return LJ_OK;
+}
+
+// the step captured once per (dt, ensemble, boundary) and replayed
+int ensure_graph(lj_ctx* c, double dt)
+{
+  if (c->gexec && c->g_dt == dt && c->g_canon == c->cfg.canonical && c->g_bc == c->cfg.boundary)
+    return LJ_OK;
+  hipStream_t st = c->stream;
+  drop_graph(c);
+  LJ_HIP(hipStreamSynchronize(st));
+  LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(c, st, dt);
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(st, &g);
+  if (rc != LJ_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  LJ_HIP(ee);
+  c->graph = g;
+  LJ_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  c->g_dt = dt;
+  c->g_canon = c->cfg.canonical;
+  c->g_bc = c->cfg.boundary;
   return LJ_OK;
 }
 
@@ -426,30 +411,25 @@ int lj_integrate(lj_ctx* c, double dt, int nsteps, double* hist)
   if (!std::isfinite(dt)) return fail(LJ_ERR_ARG, "lj_integrate: dt not finite");
   if (nsteps == 0) return LJ_OK;
   LJ_TRY(ensure_hist(c, nsteps));
+  LJ_TRY(ensure_graph(c, dt));
   hipStream_t st = c->stream;
   LJ_HIP(launch_set_hist_base(st, c->obs, c->hist_base));
-  if (!c->gexec || c->g_dt != dt || c->g_canon != c->cfg.canonical || c->g_bc != c->cfg.boundary) {
-    drop_graph(c);
-    LJ_HIP(hipStreamSynchronize(st));
-    LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(c, st, dt);
-    hipGraph_t g = nullptr;
-    hipError_t ee = hipStreamEndCapture(st, &g);
-    if (rc != LJ_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    LJ_HIP(ee);
-    c->graph = g;
-    LJ_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
-    c->g_dt = dt;
-    c->g_canon = c->cfg.canonical;
-    c->g_bc = c->cfg.boundary;
-  }
   for (int k = 0; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(c->gexec, st));
   if (hist)
     LJ_HIP(hipMemcpyAsync(hist, c->hist, sizeof(double) * 5 * nsteps, hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_integrate_async(lj_ctx* c, double dt, int nsteps)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (nsteps < 0) return fail(LJ_ERR_ARG, "lj_integrate_async: nsteps < 0");
+  if (!std::isfinite(dt)) return fail(LJ_ERR_ARG, "lj_integrate_async: dt not finite");
+  if (nsteps == 0) return LJ_OK;
+  LJ_TRY(ensure_graph(c, dt));
+  LJ_HIP(launch_set_hist_base(c->stream, c->obs, c->hist_base));
+  for (int k = 0; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(c->gexec, c->stream));
   return LJ_OK;
 }
 

@@ -1,0 +1,429 @@
+// lj_batch.hip -- many independent boxes of uneven N, density and temperature
+// advanced together: one persistent force launch drains a work queue of
+// (box, i-tile, j-slice) items over all boxes; the per-particle kernels run
+// over workgroups that never straddle two boxes, so every per-box reduction
+// stays a fixed-order FP64 sum.  Physics per box is the single-box step of
+// lj_kernels.hip (reference MDSystem.cpp:438-583, MDSystem.cu:21-138).
+#include "lj_device.h"
+
+namespace lj {
+
+namespace {
+
+using namespace dev;
+
+template <bool PERIODIC, int IPL>
+__device__ __forceinline__ void batch_item(const uint4* __restrict__ q, const float4* __restrict__ pf,
+                                           float4* __restrict__ fpart, float* __restrict__ apart,
+                                           const BoxParam& bp, int i0, int s, int lane)
+{
+  const int N = bp.N;
+  const int j0 = s * bp.jchunk;
+  const int j1 = min(N, j0 + bp.jchunk);
+  const uint4* qb = q + bp.off;
+  const float4* pb = pf + bp.off;
+  PosT<PERIODIC> pi[IPL];
+  int ii[IPL];
+  Acc hi[IPL], lo[IPL];
+#pragma unroll
+  for (int k = 0; k < IPL; ++k) {
+    ii[k] = i0 + k * 64 + lane;
+    pi[k] = load_pos<PERIODIC>(qb, pb, min(ii[k], N - 1));
+    acc_zero(hi[k]);
+    acc_zero(lo[k]);
+  }
+  const int d0 = max(j0, i0), d1 = min(j1, i0 + 64 * IPL);
+  if (d0 >= d1) {
+    j_range<PERIODIC, 0, IPL, false>(qb, pb, j0, j1, pi, ii, bp.C1, 0.f, hi, lo);
+  } else {
+    j_range<PERIODIC, 0, IPL, false>(qb, pb, j0, d0, pi, ii, bp.C1, 0.f, hi, lo);
+    j_range<PERIODIC, 1, IPL, false>(qb, pb, d0, d1, pi, ii, bp.C1, 0.f, hi, lo);
+    j_range<PERIODIC, 0, IPL, false>(qb, pb, d1, j1, pi, ii, bp.C1, 0.f, hi, lo);
+  }
+#pragma unroll
+  for (int k = 0; k < IPL; ++k) {
+    if (ii[k] < N) {
+      const size_t o = (size_t)bp.pbase + (size_t)s * bp.Npad + ii[k];
+      fpart[o] = make_float4(hi[k].fx, hi[k].fy, hi[k].fz, hi[k].b);
+      apart[o] = hi[k].a;
+    }
+  }
+}
+
+// PERSIST: waves pull items from the queue head until it passes nitems (every
+// wave reaches that exit); otherwise wave w of the grid takes item w.
+template <bool PERIODIC, int IPL, bool PERSIST>
+__global__ __launch_bounds__(kBlock) void k_batch_force(const uint4* __restrict__ q,
+                                                        const float4* __restrict__ pf,
+                                                        float4* __restrict__ fpart,
+                                                        float* __restrict__ apart,
+                                                        const BoxParam* __restrict__ box,
+                                                        const BatchItem* __restrict__ items,
+                                                        unsigned* __restrict__ counter, int nitems)
+{
+  const int lane = threadIdx.x & 63;
+  if constexpr (PERSIST) {
+    for (;;) {
+      unsigned it = 0;
+      if (lane == 0) it = atomicAdd(counter, 1u);
+      it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
+      if (it >= (unsigned)nitems) break;
+      const BatchItem w = items[it];
+      const BoxParam bp = box[w.box];
+      batch_item<PERIODIC, IPL>(q, pf, fpart, apart, bp, w.i0, w.s, lane);
+    }
+  } else {
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int it = blockIdx.x * kWaves + wv;
+    if (it >= nitems) return;
+    const BatchItem w = items[it];
+    const BoxParam bp = box[w.box];
+    batch_item<PERIODIC, IPL>(q, pf, fpart, apart, bp, w.i0, w.s, lane);
+  }
+}
+
+struct PIdx {
+  int b, i, gi;
+  bool valid;
+};
+
+__device__ __forceinline__ PIdx pidx(const BatchArgs& a)
+{
+  const int2 pb = a.pblock[blockIdx.x];
+  PIdx r;
+  r.b = pb.x;
+  r.i = pb.y + (int)threadIdx.x;
+  const BoxParam& bp = a.box[r.b];
+  r.valid = r.i < bp.N;
+  r.gi = bp.off + r.i;
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_prep_q(BatchArgs a)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0u;
+  const PIdx p = pidx(a);
+  if (!p.valid) return;
+  const double qs = a.box[p.b].qscale;
+  const float4 x = a.pos[p.gi];
+  a.q[p.gi] = make_uint4(fixq(x.x, qs), fixq(x.y, qs), fixq(x.z, qs), 0u);
+}
+
+template <int CANON>
+__global__ __launch_bounds__(kBlock) void k_batch_drift(BatchArgs a)
+{
+#pragma clang fp contract(off)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0u;  // queue of this step's force pass
+  const PIdx pp = pidx(a);
+  if (!pp.valid) return;
+  const int i = pp.gi;
+  float4 p = a.pos[i];
+  float4 v = a.vel[i];
+  const float4 f = a.force[i];
+  const double dt = a.dt;
+  p.x = (float)(p.x + (dt * v.x + dt * dt * f.x / 2.));  // MDSystem.cpp:447-449
+  p.y = (float)(p.y + (dt * v.y + dt * dt * f.y / 2.));
+  p.z = (float)(p.z + (dt * v.z + dt * dt * f.z / 2.));
+  if constexpr (CANON == LJ_EVN) {
+    v.x = (float)(v.x + dt * f.x / 2.);  // MDSystem.cpp:451-453
+    v.y = (float)(v.y + dt * f.y / 2.);
+    v.z = (float)(v.z + dt * f.z / 2.);
+    a.vel[i] = v;
+  } else {
+    a.tF[i] = make_float4(0.5f * f.x, 0.5f * f.y, 0.5f * f.z, 0.f);  // MDSystem.cpp:477-479
+  }
+  a.pos[i] = p;
+  if (a.bc == LJ_BC_PERIODIC) {
+    const double qs = a.box[pp.b].qscale;
+    a.q[i] = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), 0u);
+  }
+}
+
+__device__ __forceinline__ Partial box_slices(const BatchArgs& a, const BoxParam& bp, int i)
+{
+  return sum_slices(a.fpart + bp.pbase, a.apart + bp.pbase, i, bp.S, bp.Npad);
+}
+
+__device__ __forceinline__ float inside(const float4& p, const BoxParam& bp)
+{
+  return (p.x >= bp.sub_lo && p.x <= bp.sub_hi && p.y >= bp.sub_lo && p.y <= bp.sub_hi &&
+          p.z >= bp.sub_lo && p.z <= bp.sub_hi)
+             ? 1.f
+             : 0.f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_kick_evn(BatchArgs a)
+{
+#pragma clang fp contract(off)
+  const PIdx pp = pidx(a);
+  const BoxParam& bp = a.box[pp.b];
+  double v4[4] = {0., 0., 0., 0.};
+  if (pp.valid) {
+    const int i = pp.gi;
+    const Partial s = box_slices(a, bp, pp.i);
+    const double e = 0.5 * (s.b - s.a);
+    const float fx = (float)(bp.FS * s.fx), fy = (float)(bp.FS * s.fy), fz = (float)(bp.FS * s.fz);
+    a.force[i] = make_float4(fx, fy, fz, (float)e);
+    float4 v = a.vel[i];
+    const double dt = a.dt;
+    v.x = (float)(v.x + dt * fx / 2.);  // MDSystem.cpp:460-462
+    v.y = (float)(v.y + dt * fy / 2.);
+    v.z = (float)(v.z + dt * fz / 2.);
+    float4 p = a.pos[i];
+    if (a.bc != LJ_BC_NONE) {
+      apply_bc_one(p, v, a.bc, bp.L);
+      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
+    }
+    a.vel[i] = v;
+    v4[0] = kin2(v);
+    v4[1] = e;
+    v4[2] = s.b;
+    v4[3] = inside(p, bp);
+  }
+  const int fld[4] = {BLK_K2, BLK_E, BLK_B, BLK_NSUB};
+  block_sum_store<4>(v4, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_tvn_a(BatchArgs a)
+{
+#pragma clang fp contract(off)
+  const PIdx pp = pidx(a);
+  const BoxParam& bp = a.box[pp.b];
+  double v3[3] = {0., 0., 0.};
+  if (pp.valid) {
+    const int i = pp.gi;
+    const Partial s = box_slices(a, bp, pp.i);
+    const double e = 0.5 * (s.b - s.a);
+    const float fx = (float)(bp.FS * s.fx), fy = (float)(bp.FS * s.fy), fz = (float)(bp.FS * s.fz);
+    a.force[i] = make_float4(fx, fy, fz, (float)e);
+    float4 tf = a.tF[i];
+    tf.x += 0.5f * fx;  // MDSystem.cpp:486-488
+    tf.y += 0.5f * fy;
+    tf.z += 0.5f * fz;
+    a.tF[i] = tf;
+    const float4 v = a.vel[i];
+    const double dt = a.dt;
+    float4 tv;
+    tv.x = (float)(v.x + dt * tf.x / 2.);  // MDSystem.cpp:493-495
+    tv.y = (float)(v.y + dt * tf.y / 2.);
+    tv.z = (float)(v.z + dt * tf.z / 2.);
+    tv.w = 0.f;
+    v3[0] = kin2(tv);
+    v3[1] = e;
+    v3[2] = s.b;
+  }
+  const int fld[3] = {BLK_TV2, BLK_E, BLK_B};
+  block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_tvn_b(BatchArgs a)
+{
+#pragma clang fp contract(off)
+  const PIdx pp = pidx(a);
+  const BoxParam& bp = a.box[pp.b];
+  const double tv2 = blk_total(a.blk + (size_t)bp.blk0 * kBlkFields, bp.nblk, BLK_TV2);
+  const double Tkin = tv2 * (1. / 3. / bp.N);  // MDSystem.cpp:370
+  const double chi = sqrt(bp.T0 / Tkin);      // MDSystem.cpp:499
+  double v2[2] = {0., 0.};
+  if (pp.valid) {
+    const int i = pp.gi;
+    float4 v = a.vel[i];
+    const float4 tf = a.tF[i];
+    const double dt = a.dt;
+    v.x = (float)((2. * chi - 1.) * v.x + chi * dt * tf.x);  // MDSystem.cpp:503-505
+    v.y = (float)((2. * chi - 1.) * v.y + chi * dt * tf.y);
+    v.z = (float)((2. * chi - 1.) * v.z + chi * dt * tf.z);
+    float4 p = a.pos[i];
+    if (a.bc != LJ_BC_NONE) {
+      apply_bc_one(p, v, a.bc, bp.L);
+      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
+    }
+    a.vel[i] = v;
+    v2[0] = kin2(v);
+    v2[1] = inside(p, bp);
+  }
+  __syncthreads();
+  const int fld[2] = {BLK_K2, BLK_NSUB};
+  block_sum_store<2>(v2, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_reduce_forces(BatchArgs a)
+{
+  const PIdx pp = pidx(a);
+  const BoxParam& bp = a.box[pp.b];
+  double v2[2] = {0., 0.};
+  if (pp.valid) {
+    const Partial s = box_slices(a, bp, pp.i);
+    const double e = 0.5 * (s.b - s.a);
+    a.force[pp.gi] = make_float4((float)(bp.FS * s.fx), (float)(bp.FS * s.fy),
+                                 (float)(bp.FS * s.fz), (float)e);
+    v2[0] = e;
+    v2[1] = s.b;
+  }
+  const int fld[2] = {BLK_E, BLK_B};
+  block_sum_store<2>(v2, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_kinetic(BatchArgs a)
+{
+  const PIdx pp = pidx(a);
+  const BoxParam& bp = a.box[pp.b];
+  double v2[2] = {0., 0.};
+  if (pp.valid) {
+    v2[0] = kin2(a.vel[pp.gi]);
+    v2[1] = inside(a.pos[pp.gi], bp);
+  }
+  const int fld[2] = {BLK_K2, BLK_NSUB};
+  block_sum_store<2>(v2, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+}
+
+// one workgroup per box: observables (MDSystem.cpp:325-359) and, when
+// advancing, the accumulated fluctuation statistics of the box
+__global__ __launch_bounds__(kBlock) void k_batch_finalize(BatchArgs a, int advance)
+{
+  const int b = blockIdx.x;
+  const BoxParam& bp = a.box[b];
+  double v[4] = {0., 0., 0., 0.};
+  const double* rows = a.blk + (size_t)bp.blk0 * kBlkFields;
+  for (int r = threadIdx.x; r < bp.nblk; r += kBlock) {
+    v[0] += rows[(size_t)r * kBlkFields + BLK_K2];
+    v[1] += rows[(size_t)r * kBlkFields + BLK_E];
+    v[2] += rows[(size_t)r * kBlkFields + BLK_B];
+    v[3] += rows[(size_t)r * kBlkFields + BLK_NSUB];
+  }
+  block_allsum<4>(v);
+  if (threadIdx.x != 0) return;
+  lj_obs* o = a.obs + b;
+  const int N = bp.N;
+  const double K = v[0] / 2.;
+  const double T = 2. * K / 3. / N;
+  o->V = 2. * a.eps * v[1];
+  o->W = 12. * a.eps * v[2];
+  const double P = (o->W / 3. + N * T) / (N / bp.rho);
+  o->K = K;
+  o->T = T;
+  o->P = P;
+  o->U = K + o->V;
+  o->Pshear = 0.;
+  if (advance) {
+    o->t += a.dt;
+    o->step += 1;
+    o->av_iters += 1;
+    o->av_U_tot += o->U;
+    o->av_p_tot += P;
+    o->av_T_tot += T;
+    double* st = a.stats + (size_t)b * kNStat;
+    const double n = v[3];
+    st[ST_SAMPLES] += 1.;
+    st[ST_U] += o->U;
+    st[ST_U2] += o->U * o->U;
+    st[ST_V] += o->V;
+    st[ST_V2] += o->V * o->V;
+    st[ST_K] += K;
+    st[ST_K2] += K * K;
+    st[ST_P] += P;
+    st[ST_P2] += P * P;
+    st[ST_N1] += n;
+    st[ST_N2] += n * n;
+    st[ST_N3] += n * n * n;
+    st[ST_N4] += n * n * n * n;
+  }
+}
+
+__global__ void k_batch_zero_stats(double* stats, int n, lj_obs* obs, int nboxes)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) stats[i] = 0.;
+  if (i < nboxes) {
+    obs[i].av_U_tot = obs[i].av_T_tot = obs[i].av_p_tot = 0.;
+    obs[i].av_iters = 0;
+  }
+}
+
+template <bool PERIODIC, int IPL>
+void launch_batch_force_t(hipStream_t st, const BatchArgs& a)
+{
+  if (a.persistent_grid > 0)
+    hipLaunchKernelGGL((k_batch_force<PERIODIC, IPL, true>), dim3(a.persistent_grid), dim3(kBlock),
+                       0, st, a.q, a.pos, a.fpart, a.apart, a.box, a.items, a.counter, a.nitems);
+  else
+    hipLaunchKernelGGL((k_batch_force<PERIODIC, IPL, false>),
+                       dim3((a.nitems + kWaves - 1) / kWaves), dim3(kBlock), 0, st, a.q, a.pos,
+                       a.fpart, a.apart, a.box, a.items, a.counter, a.nitems);
+}
+
+}  // namespace
+
+hipError_t launch_batch_prep_q(hipStream_t st, const BatchArgs& a)
+{
+  hipLaunchKernelGGL(k_batch_prep_q, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a)
+{
+  if (a.canonical == LJ_EVN)
+    hipLaunchKernelGGL(k_batch_drift<LJ_EVN>, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_batch_drift<LJ_TVN>, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic)
+{
+  if (periodic) {
+    if (a.ipl == 2) launch_batch_force_t<true, 2>(st, a);
+    else launch_batch_force_t<true, 1>(st, a);
+  } else {
+    if (a.ipl == 2) launch_batch_force_t<false, 2>(st, a);
+    else launch_batch_force_t<false, 1>(st, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_kick_evn(hipStream_t st, const BatchArgs& a)
+{
+  hipLaunchKernelGGL(k_batch_kick_evn, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_tvn_a(hipStream_t st, const BatchArgs& a)
+{
+  hipLaunchKernelGGL(k_batch_tvn_a, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_tvn_b(hipStream_t st, const BatchArgs& a)
+{
+  hipLaunchKernelGGL(k_batch_tvn_b, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize(hipStream_t st, const BatchArgs& a, int advance)
+{
+  hipLaunchKernelGGL(k_batch_finalize, dim3(a.nboxes), dim3(kBlock), 0, st, a, advance);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_reduce_forces(hipStream_t st, const BatchArgs& a)
+{
+  hipLaunchKernelGGL(k_batch_reduce_forces, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_kinetic(hipStream_t st, const BatchArgs& a)
+{
+  hipLaunchKernelGGL(k_batch_kinetic, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_zero_stats(hipStream_t st, const BatchArgs& a)
+{
+  const int n = a.nboxes * kNStat;
+  const int m = n > a.nboxes ? n : a.nboxes;
+  hipLaunchKernelGGL(k_batch_zero_stats, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     a.stats, n, a.obs, a.nboxes);
+  return hipGetLastError();
+}
+
+}  // namespace lj

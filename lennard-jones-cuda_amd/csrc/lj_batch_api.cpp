@@ -1,0 +1,523 @@
+// lj_batch_api.cpp -- C-ABI of the batched ensemble (many boxes, one launch
+// per kernel per step, captured in a HIP graph) and its autotuner.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "lj_internal.h"
+#include "lj_util.h"
+
+using namespace lj;
+
+struct lj_batch {
+  int device = 0;
+  int nboxes = 0;
+  int canonical = LJ_EVN;
+  int boundary = LJ_BC_PERIODIC;
+  double eps = 1.0;
+  int ncu = 256;
+  std::vector<int> N;
+  std::vector<double> rho, T0, L;
+  std::vector<int> off;
+  int Ntot = 0;
+  double sub_fraction = 0.5;
+  lj_batch_tuning tune{1, 512, 8};
+  hipStream_t stream = nullptr;
+  // device
+  float4 *pos = nullptr, *vel = nullptr, *force = nullptr, *tF = nullptr;
+  uint4* q = nullptr;
+  float4* fpart = nullptr;
+  float* apart = nullptr;
+  size_t part_cap = 0;
+  double* blk = nullptr;
+  lj_obs* obs = nullptr;
+  double* stats = nullptr;
+  BoxParam* box = nullptr;
+  int2* pblock = nullptr;
+  BatchItem* items = nullptr;
+  size_t items_cap = 0;
+  unsigned* counter = nullptr;
+  int nblocks = 0, nitems = 0;
+  std::vector<BoxParam> hbox;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0.;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+void bdrop_graph(lj_batch* b)
+{
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+int bsetdev(lj_batch* b)
+{
+  if (!b) return fail(LJ_ERR_ARG, "null batch");
+  LJ_HIP(hipSetDevice(b->device));
+  return LJ_OK;
+}
+
+BatchArgs bargs(lj_batch* b, double dt)
+{
+  BatchArgs a{};
+  a.pos = b->pos;
+  a.vel = b->vel;
+  a.force = b->force;
+  a.tF = b->tF;
+  a.q = b->q;
+  a.fpart = b->fpart;
+  a.apart = b->apart;
+  a.blk = b->blk;
+  a.obs = b->obs;
+  a.stats = b->stats;
+  a.box = b->box;
+  a.pblock = b->pblock;
+  a.items = b->items;
+  a.counter = b->counter;
+  a.nboxes = b->nboxes;
+  a.nblocks = b->nblocks;
+  a.nitems = b->nitems;
+  a.dt = dt;
+  a.eps = b->eps;
+  a.bc = b->boundary;
+  a.canonical = b->canonical;
+  a.ipl = b->tune.ipl;
+  a.persistent_grid = b->tune.blocks_per_cu > 0 ? b->tune.blocks_per_cu * b->ncu : 0;
+  return a;
+}
+
+// (re)build the per-box tables and the work items for the current tuning
+int build_tables(lj_batch* b)
+{
+  const int ipl = b->tune.ipl;
+  const int tile = 64 * ipl;
+  const int jc0 = b->tune.jchunk;
+  std::vector<BoxParam> bp(b->nboxes);
+  std::vector<int2> pblock;
+  std::vector<BatchItem> items;
+  size_t pbase = 0;
+  int blk0 = 0;
+  for (int k = 0; k < b->nboxes; ++k) {
+    const int N = b->N[k];
+    BoxParam& p = bp[k];
+    std::memset(&p, 0, sizeof(p));
+    p.N = N;
+    p.off = b->off[k];
+    int jc = std::min(jc0, (N + 7) & ~7);
+    jc = (jc + 7) & ~7;
+    p.jchunk = jc;
+    p.S = (N + jc - 1) / jc;
+    p.Npad = (N + 63) & ~63;
+    p.pbase = (int)pbase;
+    pbase += (size_t)p.S * p.Npad;
+    p.nblk = (N + kBlock - 1) / kBlock;
+    p.blk0 = blk0;
+    for (int t = 0; t < p.nblk; ++t) pblock.push_back(make_int2(k, t * kBlock));
+    blk0 += p.nblk;
+    const PairScalars ps = make_pair_scalars(b->L[k], b->boundary == LJ_BC_PERIODIC, b->eps, 1.0, false);
+    p.L = b->L[k];
+    p.qscale = ps.qscale;
+    p.FS = ps.FS;
+    p.C1 = ps.C1;
+    p.rho = b->rho[k];
+    p.T0 = b->T0[k];
+    const double side = b->L[k] * std::cbrt(b->sub_fraction);  // MDSystem.cpp:711
+    p.sub_lo = 0.5 * b->L[k] - 0.5 * side;
+    p.sub_hi = 0.5 * b->L[k] + 0.5 * side;
+    for (int t = 0; t < (N + tile - 1) / tile; ++t)
+      for (int s = 0; s < p.S; ++s) items.push_back(BatchItem{k, t * tile, s, 0});
+  }
+  if (pbase > (size_t)0x7fffffff) return fail(LJ_ERR_UNSUPPORTED, "batch partial buffer too large");
+  // longest items first (pairs per item), boxes with larger N first on ties
+  std::stable_sort(items.begin(), items.end(), [&](const BatchItem& x, const BatchItem& y) {
+    auto cost = [&](const BatchItem& w) {
+      const BoxParam& p = bp[w.box];
+      const long long ni = std::min(tile, p.N - w.i0);
+      const long long nj = std::min(p.jchunk, p.N - w.s * p.jchunk);
+      return ni * nj;
+    };
+    const long long cx = cost(x), cy = cost(y);
+    if (cx != cy) return cx > cy;
+    return bp[x.box].N > bp[y.box].N;
+  });
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  if (pbase > b->part_cap) {
+    dev_free(b->fpart);
+    dev_free(b->apart);
+    b->part_cap = 0;
+    LJ_TRY(dev_alloc(&b->fpart, pbase));
+    LJ_TRY(dev_alloc(&b->apart, pbase));
+    b->part_cap = pbase;
+  }
+  if (items.size() > b->items_cap) {
+    dev_free(b->items);
+    b->items_cap = 0;
+    LJ_TRY(dev_alloc(&b->items, items.size()));
+    b->items_cap = items.size();
+  }
+  if (!b->box) LJ_TRY(dev_alloc(&b->box, b->nboxes));
+  if (!b->pblock) LJ_TRY(dev_alloc(&b->pblock, pblock.size()));
+  if (!b->blk) LJ_TRY(dev_alloc(&b->blk, (size_t)blk0 * kBlkFields));
+  LJ_HIP(hipMemcpy(b->box, bp.data(), sizeof(BoxParam) * bp.size(), hipMemcpyHostToDevice));
+  LJ_HIP(hipMemcpy(b->pblock, pblock.data(), sizeof(int2) * pblock.size(), hipMemcpyHostToDevice));
+  LJ_HIP(hipMemcpy(b->items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice));
+  b->hbox = bp;
+  b->nblocks = (int)pblock.size();
+  b->nitems = (int)items.size();
+  bdrop_graph(b);
+  return LJ_OK;
+}
+
+int enqueue_batch_step(lj_batch* b, hipStream_t st, double dt)
+{
+  const BatchArgs a = bargs(b, dt);
+  LJ_HIP(launch_batch_drift(st, a));
+  LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC));
+  if (b->canonical == LJ_EVN) {
+    LJ_HIP(launch_batch_kick_evn(st, a));
+  } else {
+    LJ_HIP(launch_batch_tvn_a(st, a));
+    LJ_HIP(launch_batch_tvn_b(st, a));
+  }
+  LJ_HIP(launch_batch_finalize(st, a, 1));
+  return LJ_OK;
+}
+
+int enqueue_batch_forces(lj_batch* b, hipStream_t st)
+{
+  const BatchArgs a = bargs(b, 0.);
+  LJ_HIP(launch_batch_prep_q(st, a));  // also resets the queue head
+  LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC));
+  return LJ_OK;
+}
+
+std::string tune_key(lj_batch* b)
+{
+  hipDeviceProp_t prop;
+  std::string dev = "gfx";
+  if (hipGetDeviceProperties(&prop, b->device) == hipSuccess) dev = prop.gcnArchName;
+  unsigned long long h = 1469598103934665603ull;  // FNV-1a over the box sizes
+  for (int n : b->N) {
+    h ^= (unsigned long long)n;
+    h *= 1099511628211ull;
+  }
+  std::ostringstream os;
+  os << dev << ":" << b->ncu << ":" << b->nboxes << ":" << b->boundary << ":" << std::hex << h;
+  return os.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+int lj_batch_create(int nboxes, const int* N, const double* rho, const double* T0, int canonical,
+                    int boundary, int device, lj_batch** out)
+{
+  if (!out || !N || !rho || !T0) return fail(LJ_ERR_ARG, "lj_batch_create: null argument");
+  *out = nullptr;
+  if (nboxes <= 0) return fail(LJ_ERR_ARG, "lj_batch_create: nboxes must be positive");
+  if (canonical != 0 && canonical != 1) return fail(LJ_ERR_ARG, "lj_batch_create: canonical must be 0 or 1");
+  if (boundary < 0 || boundary > 2) return fail(LJ_ERR_ARG, "lj_batch_create: boundary must be 0, 1 or 2");
+  long long tot = 0;
+  for (int k = 0; k < nboxes; ++k) {
+    if (N[k] <= 0 || !(rho[k] > 0.) || !(T0[k] >= 0.))
+      return fail(LJ_ERR_ARG, "lj_batch_create: box %d has N=%d rho=%g T0=%g", k, N[k], rho[k], T0[k]);
+    tot += N[k];
+  }
+  if (tot > (1ll << 28)) return fail(LJ_ERR_ARG, "lj_batch_create: too many particles (%lld)", tot);
+  LJ_TRY(check_device());
+  LJ_HIP(hipSetDevice(device));
+  lj_batch* b = new lj_batch();
+  b->device = device;
+  b->nboxes = nboxes;
+  b->canonical = canonical;
+  b->boundary = boundary;
+  b->N.assign(N, N + nboxes);
+  b->rho.assign(rho, rho + nboxes);
+  b->T0.assign(T0, T0 + nboxes);
+  int off = 0;
+  for (int k = 0; k < nboxes; ++k) {
+    b->off.push_back(off);
+    off += N[k];
+    b->L.push_back(box_length(N[k], rho[k]));
+  }
+  b->Ntot = off;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    b->ncu = ncu;
+  int rc = LJ_OK;
+  auto A = [&](int r) { if (rc == LJ_OK) rc = r; };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    b->stream = nullptr;
+    A(fail(LJ_ERR_DEVICE, "hipStreamCreate failed"));
+  }
+  A(dev_alloc(&b->pos, off));
+  A(dev_alloc(&b->vel, off));
+  A(dev_alloc(&b->force, off));
+  A(dev_alloc(&b->tF, off));
+  A(dev_alloc(&b->q, off));
+  A(dev_alloc(&b->obs, nboxes));
+  A(dev_alloc(&b->stats, (size_t)nboxes * kNStat));
+  A(dev_alloc(&b->counter, 1));
+  if (rc == LJ_OK && (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess))
+    A(fail(LJ_ERR_DEVICE, "hipEventCreate failed"));
+  if (rc == LJ_OK) A(build_tables(b));
+  if (rc == LJ_OK) {
+    std::vector<float> z((size_t)4 * off, 0.f);
+    std::vector<lj_obs> o(nboxes);
+    std::memset(o.data(), 0, sizeof(lj_obs) * nboxes);
+    std::vector<double> zs((size_t)nboxes * kNStat, 0.);
+    const size_t bytes = sizeof(float4) * (size_t)off;
+    if (hipMemcpy(b->pos, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->vel, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->force, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->tF, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->obs, o.data(), sizeof(lj_obs) * nboxes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->stats, zs.data(), sizeof(double) * zs.size(), hipMemcpyHostToDevice) != hipSuccess)
+      A(fail(LJ_ERR_DEVICE, "initial upload failed"));
+  }
+  if (rc != LJ_OK) {
+    std::string msg = last_error();
+    lj_batch_destroy(b);
+    fail(rc, "%s", msg.c_str());
+    return rc;
+  }
+  *out = b;
+  return LJ_OK;
+}
+
+int lj_batch_destroy(lj_batch* b)
+{
+  if (!b) return LJ_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  bdrop_graph(b);
+  dev_free(b->pos);
+  dev_free(b->vel);
+  dev_free(b->force);
+  dev_free(b->tF);
+  dev_free(b->q);
+  dev_free(b->fpart);
+  dev_free(b->apart);
+  dev_free(b->blk);
+  dev_free(b->obs);
+  dev_free(b->stats);
+  dev_free(b->box);
+  dev_free(b->pblock);
+  dev_free(b->items);
+  dev_free(b->counter);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return LJ_OK;
+}
+
+int lj_batch_num_particles(const lj_batch* b) { return b ? b->Ntot : 0; }
+
+int lj_batch_set_array(lj_batch* b, int box, int type, const float* host4N)
+{
+  LJ_TRY(bsetdev(b));
+  if (box < -1 || box >= b->nboxes || !host4N) return fail(LJ_ERR_ARG, "lj_batch_set_array: bad box or pointer");
+  float4* d = type == LJ_ARRAY_POS ? b->pos : type == LJ_ARRAY_VEL ? b->vel : type == LJ_ARRAY_FORCE ? b->force : nullptr;
+  if (!d) return fail(LJ_ERR_ARG, "lj_batch_set_array: unknown array type %d", type);
+  const size_t o = box < 0 ? 0 : b->off[box];
+  const size_t n = box < 0 ? b->Ntot : b->N[box];
+  LJ_HIP(hipMemcpyAsync(d + o, host4N, sizeof(float4) * n, hipMemcpyHostToDevice, b->stream));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_get_array(lj_batch* b, int box, int type, float* host4N)
+{
+  LJ_TRY(bsetdev(b));
+  if (box < -1 || box >= b->nboxes || !host4N) return fail(LJ_ERR_ARG, "lj_batch_get_array: bad box or pointer");
+  float4* d = type == LJ_ARRAY_POS ? b->pos : type == LJ_ARRAY_VEL ? b->vel : type == LJ_ARRAY_FORCE ? b->force : nullptr;
+  if (!d) return fail(LJ_ERR_ARG, "lj_batch_get_array: unknown array type %d", type);
+  const size_t o = box < 0 ? 0 : b->off[box];
+  const size_t n = box < 0 ? b->Ntot : b->N[box];
+  LJ_HIP(hipMemcpyAsync(host4N, d + o, sizeof(float4) * n, hipMemcpyDeviceToHost, b->stream));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_compute_forces(lj_batch* b)
+{
+  LJ_TRY(bsetdev(b));
+  hipStream_t st = b->stream;
+  LJ_TRY(enqueue_batch_forces(b, st));
+  const BatchArgs a = bargs(b, 0.);
+  LJ_HIP(launch_batch_reduce_forces(st, a));
+  LJ_HIP(launch_batch_kinetic(st, a));
+  LJ_HIP(launch_batch_finalize(st, a, 0));
+  LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_batch_sample_initial_state(lj_batch* b, uint64_t seed)
+{
+  LJ_TRY(bsetdev(b));
+  std::vector<float> p((size_t)4 * b->Ntot), v((size_t)4 * b->Ntot);
+  for (int k = 0; k < b->nboxes; ++k) {
+    const size_t o = (size_t)4 * b->off[k];
+    lattice_positions(b->N[k], b->L[k], p.data() + o);
+    gaussian_velocities(b->N[k], b->T0[k], seed + (uint64_t)k, v.data() + o);
+  }
+  LJ_TRY(lj_batch_set_array(b, -1, LJ_ARRAY_POS, p.data()));
+  LJ_TRY(lj_batch_set_array(b, -1, LJ_ARRAY_VEL, v.data()));
+  LJ_TRY(lj_batch_compute_forces(b));
+  return lj_batch_reset_stats(b);
+}
+
+int lj_batch_set_subvolume(lj_batch* b, double fraction)
+{
+  LJ_TRY(bsetdev(b));
+  if (!(fraction > 0.) || fraction > 1.) return fail(LJ_ERR_ARG, "lj_batch_set_subvolume: fraction must be in (0, 1]");
+  b->sub_fraction = fraction;
+  return build_tables(b);
+}
+
+int lj_batch_reset_stats(lj_batch* b)
+{
+  LJ_TRY(bsetdev(b));
+  LJ_HIP(launch_batch_zero_stats(b->stream, bargs(b, 0.)));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_get_stats(lj_batch* b, double* out)
+{
+  LJ_TRY(bsetdev(b));
+  if (!out) return fail(LJ_ERR_ARG, "lj_batch_get_stats: null output");
+  LJ_HIP(hipMemcpyAsync(out, b->stats, sizeof(double) * kNStat * b->nboxes, hipMemcpyDeviceToHost, b->stream));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_get_observables(lj_batch* b, lj_obs* out)
+{
+  LJ_TRY(bsetdev(b));
+  if (!out) return fail(LJ_ERR_ARG, "lj_batch_get_observables: null output");
+  LJ_HIP(hipMemcpyAsync(out, b->obs, sizeof(lj_obs) * b->nboxes, hipMemcpyDeviceToHost, b->stream));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_set_tuning(lj_batch* b, const lj_batch_tuning* t)
+{
+  LJ_TRY(bsetdev(b));
+  if (!t || (t->ipl != 1 && t->ipl != 2) || t->jchunk < 8 || t->jchunk > (1 << 20) ||
+      t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
+    return fail(LJ_ERR_ARG, "lj_batch_set_tuning: ipl in {1,2}, jchunk in [8, 2^20], blocks_per_cu in [0, 64]");
+  b->tune = *t;
+  return build_tables(b);
+}
+
+int lj_batch_get_tuning(lj_batch* b, lj_batch_tuning* t)
+{
+  if (!b || !t) return fail(LJ_ERR_ARG, "lj_batch_get_tuning: null argument");
+  *t = b->tune;
+  return LJ_OK;
+}
+
+int lj_batch_time_force(lj_batch* b, int reps, double* us)
+{
+  LJ_TRY(bsetdev(b));
+  if (reps <= 0 || !us) return fail(LJ_ERR_ARG, "lj_batch_time_force: bad arguments");
+  hipStream_t st = b->stream;
+  LJ_TRY(enqueue_batch_forces(b, st));  // warm
+  LJ_HIP(hipEventRecord(b->e0, st));
+  for (int k = 0; k < reps; ++k) LJ_TRY(enqueue_batch_forces(b, st));
+  LJ_HIP(hipEventRecord(b->e1, st));
+  LJ_HIP(hipEventSynchronize(b->e1));
+  float ms = 0.f;
+  LJ_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+  *us = 1000.0 * ms / reps;
+  return LJ_OK;
+}
+
+int lj_batch_autotune(lj_batch* b, int reps, lj_batch_tuning* best_out, double* best_us)
+{
+  LJ_TRY(bsetdev(b));
+  if (reps <= 0) reps = 3;
+  const char* cache = std::getenv("LJ_AUTOTUNE_CACHE");
+  const std::string key = tune_key(b);
+  if (cache) {
+    std::ifstream in(cache);
+    std::string k;
+    lj_batch_tuning t{};
+    double us = 0.;
+    while (in >> k >> t.ipl >> t.jchunk >> t.blocks_per_cu >> us)
+      if (k == key) {
+        LJ_TRY(lj_batch_set_tuning(b, &t));
+        if (best_out) *best_out = t;
+        if (best_us) *best_us = us;
+        return LJ_OK;
+      }
+  }
+  const int ipls[] = {1, 2};
+  const int jcs[] = {256, 512, 1024, 2048};
+  const int bpcs[] = {0, 4, 8};
+  lj_batch_tuning best = b->tune;
+  double best_t = 1e300;
+  for (int ipl : ipls)
+    for (int jc : jcs)
+      for (int bpc : bpcs) {
+        lj_batch_tuning t{ipl, jc, bpc};
+        LJ_TRY(lj_batch_set_tuning(b, &t));
+        double us = 0.;
+        LJ_TRY(lj_batch_time_force(b, reps, &us));
+        if (us < best_t) {
+          best_t = us;
+          best = t;
+        }
+      }
+  LJ_TRY(lj_batch_set_tuning(b, &best));
+  if (cache) {
+    std::ofstream o(cache, std::ios::app);
+    o << key << " " << best.ipl << " " << best.jchunk << " " << best.blocks_per_cu << " " << best_t << "\n";
+  }
+  if (best_out) *best_out = best;
+  if (best_us) *best_us = best_t;
+  return LJ_OK;
+}
+
+int lj_batch_integrate(lj_batch* b, double dt, int nsteps)
+{
+  LJ_TRY(bsetdev(b));
+  if (nsteps < 0 || !std::isfinite(dt)) return fail(LJ_ERR_ARG, "lj_batch_integrate: bad arguments");
+  if (nsteps == 0) return LJ_OK;
+  hipStream_t st = b->stream;
+  if (!b->gexec || b->g_dt != dt) {
+    bdrop_graph(b);
+    LJ_HIP(hipStreamSynchronize(st));
+    LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_batch_step(b, st, dt);
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(st, &g);
+    if (rc != LJ_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    LJ_HIP(ee);
+    b->graph = g;
+    LJ_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->g_dt = dt;
+  }
+  for (int k = 0; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(b->gexec, st));
+  LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+// lj_device.h -- device building blocks shared by the single-box and the
+// batched kernels: the LJ pair body, the SGPR-streamed j loop with optional
+// TwoSum compensation, deterministic FP64 block reductions, and the
+// per-particle helpers of the integrator.
+#pragma once
+
+#include "lj_internal.h"
+
+namespace lj {
+namespace dev {
+
+constexpr int kWaves = kBlock / 64;  // waves per workgroup
+constexpr int kJB = 32;              // j block of the compensated accumulation
+
+struct Acc {
+  float fx, fy, fz, a, b;
+};
+
+__device__ __forceinline__ void acc_zero(Acc& c) { c.fx = c.fy = c.fz = c.a = c.b = 0.f; }
+
+// Knuth TwoSum: hi + lo += x without loss
+__device__ __forceinline__ void two_sum(float& hi, float& lo, float x)
+{
+  float s = hi + x;
+  float bp = s - hi;
+  float err = (hi - (s - bp)) + (x - bp);
+  hi = s;
+  lo += err;
+}
+
+template <bool PERIODIC>
+struct PosT;
+template <>
+struct PosT<true> {
+  unsigned x, y, z;
+};
+template <>
+struct PosT<false> {
+  float x, y, z;
+};
+
+template <bool PERIODIC>
+__device__ __forceinline__ PosT<PERIODIC> load_pos(const uint4* q, const float4* pf, int j)
+{
+  PosT<PERIODIC> p;
+  if constexpr (PERIODIC) {
+    uint4 v = q[j];
+    p.x = v.x;
+    p.y = v.y;
+    p.z = v.z;
+  } else {
+    float4 v = pf[j];
+    p.x = v.x;
+    p.y = v.y;
+    p.z = v.z;
+  }
+  return p;
+}
+
+// GUARD: 0 no exclusion, 1 self by index (i == j), 2 pairs with r2 <= thr
+template <bool PERIODIC, int GUARD>
+__device__ __forceinline__ void pair(const PosT<PERIODIC>& pi, const PosT<PERIODIC>& pj, int i,
+                                     int j, float C1, float thr, Acc& c)
+{
+  float dx, dy, dz;
+  if constexpr (PERIODIC) {
+    dx = (float)(int)(pi.x - pj.x);  // modular difference == minimum image
+    dy = (float)(int)(pi.y - pj.y);
+    dz = (float)(int)(pi.z - pj.z);
+  } else {
+    dx = pi.x - pj.x;
+    dy = pi.y - pj.y;
+    dz = pi.z - pj.z;
+  }
+  float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+  float ir2 = __builtin_amdgcn_rcpf(r2) * C1;  // (sigma/r)^2
+  if constexpr (GUARD == 1) ir2 = (i == j) ? 0.f : ir2;
+  if constexpr (GUARD == 2) ir2 = (r2 > thr) ? ir2 : 0.f;
+  float ir6 = ir2 * ir2 * ir2;
+  float w = ir6 * __builtin_fmaf(ir6, 2.f, -1.f);  // (s/r)^6 (2 (s/r)^6 - 1)
+  float s = w * ir2;
+  c.fx = __builtin_fmaf(dx, s, c.fx);
+  c.fy = __builtin_fmaf(dy, s, c.fy);
+  c.fz = __builtin_fmaf(dz, s, c.fz);
+  c.a += ir6;
+  c.b += w;
+}
+
+template <bool PERIODIC, int GUARD, int IPL, bool COMP>
+__device__ __forceinline__ void j_range(const uint4* __restrict__ q, const float4* __restrict__ pf,
+                                        int jb, int je, const PosT<PERIODIC> (&pi)[IPL],
+                                        const int (&ii)[IPL], float C1, float thr, Acc (&hi)[IPL],
+                                        Acc (&lo)[IPL])
+{
+  for (int b0 = jb; b0 < je; b0 += kJB) {
+    const int b1 = min(je, b0 + kJB);
+    Acc t[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) acc_zero(t[k]);
+#pragma unroll 8
+    for (int j = b0; j < b1; ++j) {
+      const PosT<PERIODIC> pj = load_pos<PERIODIC>(q, pf, j);
+#pragma unroll
+      for (int k = 0; k < IPL; ++k) pair<PERIODIC, GUARD>(pi[k], pj, ii[k], j, C1, thr, t[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+      if constexpr (COMP) {
+        two_sum(hi[k].fx, lo[k].fx, t[k].fx);
+        two_sum(hi[k].fy, lo[k].fy, t[k].fy);
+        two_sum(hi[k].fz, lo[k].fz, t[k].fz);
+        two_sum(hi[k].a, lo[k].a, t[k].a);
+        two_sum(hi[k].b, lo[k].b, t[k].b);
+      } else {
+        hi[k].fx += t[k].fx;
+        hi[k].fy += t[k].fy;
+        hi[k].fz += t[k].fz;
+        hi[k].a += t[k].a;
+        hi[k].b += t[k].b;
+      }
+    }
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// deterministic block reduction of NV doubles: butterfly inside the wave, then
+// the 4 wave results in wave order.  Row `out` receives the block totals.
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out, const int (&field)[NV])
+{
+  __shared__ double sh[kWaves][NV];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wv][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.;
+    for (int w = 0; w < kWaves; ++w) s += sh[w][threadIdx.x];
+    int f = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      if (k == (int)threadIdx.x) f = field[k];
+    out[f] = s;
+  }
+}
+
+// block-wide total of NV doubles, result broadcast to every thread
+template <int NV>
+__device__ __forceinline__ void block_allsum(double (&v)[NV])
+{
+  __shared__ double sh[kWaves][NV];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wv][k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.;
+    for (int w = 0; w < kWaves; ++w) s += sh[w][k];
+    v[k] = s;
+  }
+}
+
+// sum of one field over the rows of blk in a fixed order (every block of the
+// grid obtains the same bits)
+__device__ __forceinline__ double blk_total(const double* blk, int nblk, int field)
+{
+  double v[1] = {0.};
+  for (int r = threadIdx.x; r < nblk; r += kBlock) v[0] += blk[(size_t)r * kBlkFields + field];
+  block_allsum<1>(v);
+  return v[0];
+}
+
+struct Partial {
+  double fx, fy, fz, a, b;
+};
+
+__device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
+                                              const float* __restrict__ apart, int i, int S,
+                                              int Npad)
+{
+  Partial p{0., 0., 0., 0., 0.};
+  for (int s = 0; s < S; ++s) {
+    const float4 f = fpart[(size_t)s * Npad + i];
+    p.fx += f.x;
+    p.fy += f.y;
+    p.fz += f.z;
+    p.b += f.w;
+    p.a += apart[(size_t)s * Npad + i];
+  }
+  return p;
+}
+
+__device__ __forceinline__ unsigned fixq(float x, double qs)
+{
+  return (unsigned)(long long)__builtin_rint((double)x * qs);
+}
+
+
+__device__ __forceinline__ void apply_bc_one(float4& p, float4& v, int bc, double L)
+{
+#pragma clang fp contract(off)
+  if (bc == LJ_BC_NONE) return;
+  if (bc == LJ_BC_PERIODIC) {  // MDSystem.cpp:414-424
+    if (p.x < 0.) p.x = (float)(p.x + L);
+    if (p.x > L) p.x = (float)(p.x - L);
+    if (p.y < 0.) p.y = (float)(p.y + L);
+    if (p.y > L) p.y = (float)(p.y - L);
+    if (p.z < 0.) p.z = (float)(p.z + L);
+    if (p.z > L) p.z = (float)(p.z - L);
+  } else {  // MDSystem.cpp:425-435
+    if (p.x < 0. && v.x < 0) v.x = -v.x;
+    if (p.x > L && v.x > 0) v.x = -v.x;
+    if (p.y < 0. && v.y < 0) v.y = -v.y;
+    if (p.y > L && v.y > 0) v.y = -v.y;
+    if (p.z < 0. && v.z < 0) v.z = -v.z;
+    if (p.z > L && v.z > 0) v.z = -v.z;
+  }
+}
+
+__device__ __forceinline__ float kin2(const float4& v)
+{
+#pragma clang fp contract(off)
+  float s = v.x * v.x + v.y * v.y + v.z * v.z;
+  return s;
+}
+
+
+}  // namespace dev
+}  // namespace lj

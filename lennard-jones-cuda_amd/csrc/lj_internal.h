@@ -19,7 +19,8 @@ enum BlkField {
   BLK_E = 1,    // sum_i e_i, e_i = sum_j (s/r)^12-(s/r)^6
   BLK_B = 2,    // sum_i b_i, b_i = sum_j (s/r)^6 (2 (s/r)^6 - 1)
   BLK_TV2 = 3,  // TVN: sum_i fl32(|v'|^2)
-  kBlkFields = 4
+  BLK_NSUB = 4, // batched boxes: particles inside the sub-volume
+  kBlkFields = 5
 };
 
 constexpr int kBlock = 256;  // threads per workgroup of the per-particle kernels
@@ -114,6 +115,63 @@ struct RdfArgs {
 };
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);
+
+// ---------------------------------------------------------------------------
+// batched ensemble (lj_batch.hip)
+
+// Per-box constants of a batched ensemble (device array, one per box).
+struct BoxParam {
+  int N, off;        // particles, offset of the box in the flat arrays
+  int S, jchunk;     // j slices of the force pass
+  int Npad;          // row pitch of the box's partial buffers
+  int pbase;         // offset of the box's partial rows (elements)
+  int blk0, nblk;    // first block row and number of block rows
+  double L, qscale, FS, rho, T0;
+  float C1, pad;
+  double sub_lo, sub_hi;  // sub-volume cube [lo, hi]^3 of the fluctuation counts
+};
+
+// One work item of the batched force pass: box, i-tile start (local), slice.
+struct BatchItem {
+  int box, i0, s, pad;
+};
+
+// per-box accumulated fluctuation statistics (LJ_BATCH_NSTAT doubles per box)
+enum BatchStat {
+  ST_SAMPLES = 0, ST_U, ST_U2, ST_V, ST_V2, ST_K, ST_K2, ST_P, ST_P2,
+  ST_N1, ST_N2, ST_N3, ST_N4, kNStat
+};
+
+struct BatchArgs {
+  float4* pos;
+  float4* vel;
+  float4* force;
+  float4* tF;
+  uint4* q;
+  float4* fpart;
+  float* apart;
+  double* blk;
+  lj_obs* obs;          // per box
+  double* stats;        // per box, kNStat
+  const BoxParam* box;
+  const int2* pblock;   // per block: (box, local start)
+  const BatchItem* items;
+  unsigned* counter;    // work-queue head
+  int nboxes, nblocks, nitems;
+  double dt, eps;
+  int bc, canonical;
+  int ipl, persistent_grid;
+};
+hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a);
+hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic);
+hipError_t launch_batch_kick_evn(hipStream_t st, const BatchArgs& a);
+hipError_t launch_batch_tvn_a(hipStream_t st, const BatchArgs& a);
+hipError_t launch_batch_tvn_b(hipStream_t st, const BatchArgs& a);
+hipError_t launch_batch_finalize(hipStream_t st, const BatchArgs& a, int advance);
+hipError_t launch_batch_prep_q(hipStream_t st, const BatchArgs& a);
+hipError_t launch_batch_reduce_forces(hipStream_t st, const BatchArgs& a);
+hipError_t launch_batch_kinetic(hipStream_t st, const BatchArgs& a);
+hipError_t launch_batch_zero_stats(hipStream_t st, const BatchArgs& a);
 
 // host helpers (lj_host.cpp)
 void lattice_positions(int N, double L, float* pos4);

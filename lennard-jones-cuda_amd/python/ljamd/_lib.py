@@ -35,6 +35,14 @@ class LJObs(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class LJBatchTuning(C.Structure):
+    _fields_ = [("ipl", C.c_int), ("jchunk", C.c_int), ("blocks_per_cu", C.c_int)]
+
+
+BATCH_NSTAT = 13
+BATCH_STAT_NAMES = ("samples", "U", "U2", "V", "V2", "K", "K2", "P", "P2", "n1", "n2", "n3", "n4")
+
+
 class LJError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"ljhip error {code}: {msg}")
@@ -76,6 +84,7 @@ SIGNATURES = [
     ("lj_compute_parameters", _i, [_vp, C.POINTER(LJObs)]),
     ("lj_get_observables", _i, [_vp, C.POINTER(LJObs)]),
     ("lj_integrate", _i, [_vp, _d, _i, _vp]),
+    ("lj_integrate_async", _i, [_vp, _d, _i]),
     ("lj_rdf", _i, [_vp, _i, _i, _f, _d, _vp, _i]),
     ("lj_renormalize_velocities", _i, [_vp, _i]),
     ("lj_renormalize_to_energy", _i, [_vp, _d]),
@@ -86,6 +95,22 @@ SIGNATURES = [
     ("lj_device_arrays", _i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     ("lj_time_force_kernel", _i, [_vp, _i, C.POINTER(_d)]),
     ("lj_forces_device", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("lj_batch_create", _i, [_i, _vp, _vp, _vp, _i, _i, _i, C.POINTER(_vp)]),
+    ("lj_batch_destroy", _i, [_vp]),
+    ("lj_batch_num_particles", _i, [_vp]),
+    ("lj_batch_sample_initial_state", _i, [_vp, C.c_uint64]),
+    ("lj_batch_set_array", _i, [_vp, _i, _i, _vp]),
+    ("lj_batch_get_array", _i, [_vp, _i, _i, _vp]),
+    ("lj_batch_compute_forces", _i, [_vp]),
+    ("lj_batch_integrate", _i, [_vp, _d, _i]),
+    ("lj_batch_get_observables", _i, [_vp, _vp]),
+    ("lj_batch_set_subvolume", _i, [_vp, _d]),
+    ("lj_batch_get_stats", _i, [_vp, _vp]),
+    ("lj_batch_reset_stats", _i, [_vp]),
+    ("lj_batch_set_tuning", _i, [_vp, C.POINTER(LJBatchTuning)]),
+    ("lj_batch_get_tuning", _i, [_vp, C.POINTER(LJBatchTuning)]),
+    ("lj_batch_time_force", _i, [_vp, _i, C.POINTER(_d)]),
+    ("lj_batch_autotune", _i, [_vp, _i, C.POINTER(LJBatchTuning), C.POINTER(_d)]),
 ]
 
 _lib = None

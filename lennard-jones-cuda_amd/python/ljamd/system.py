@@ -108,6 +108,9 @@ class LJSystem:
         check(lib().lj_integrate(self.h, float(dt), int(nsteps), ptr(hist)))
         return hist[:nsteps]
 
+    def integrate_async(self, dt, nsteps=1):
+        check(lib().lj_integrate_async(self.h, float(dt), int(nsteps)))
+
     def rdf(self, rule=_L.RDF_REF_CPU, nbins=256, dr2=0.0, rmax=0.0, accumulate=False):
         counts = np.zeros(nbins, np.uint64)
         check(lib().lj_rdf(self.h, int(rule), int(nbins), float(dr2), float(rmax), ptr(counts),

@@ -239,10 +239,10 @@ static void fp64_forces_core(int N, const double* x, double L, int periodic,
       double dx = x[3 * i] - x[3 * j];
       double dy = x[3 * i + 1] - x[3 * j + 1];
       double dz = x[3 * i + 2] - x[3 * j + 2];
-      if (periodic) {
-        dx -= L * nearbyint(dx / L);
-        dy -= L * nearbyint(dy / L);
-        dz -= L * nearbyint(dz / L);
+      if (periodic) { /* half away from zero, as fast_round / lround (MDSystem.cpp:732, .cu:31) */
+        dx -= L * round(dx / L);
+        dy -= L * round(dy / L);
+        dz -= L * round(dz / L);
       }
       double r2 = dx * dx + dy * dy + dz * dz;
       if (skip_r2 > 0. && !(r2 > skip_r2)) continue;

@@ -78,7 +78,8 @@ void orc_forces_ref_omp(int N, const float* pos4, double L, int bc, float dr2,
  *                 MDSystem.cu:52, unscaled)
  *   ew[2i+1]   = sum_j r_ij . f_ij                   (per-i virial)
  *   *E = 4 eps sum_{i<j} ((s/r)^12-(s/r)^6),  *W = sum_{i<j} r_ij . f_ij.
- * periodic != 0 -> minimum image r -= L rint(r/L).
+ * periodic != 0 -> minimum image r -= L round(r/L) (half away from zero, the
+ * reference's tie rule MDSystem.cpp:732-739).
  * skip_r2 <= 0: exclude the self pair by index (MDSystem.cpp:267);
  * skip_r2 > 0: exclude every pair with r^2 <= skip_r2 (MDSystem.cu:37).
  * Any output pointer may be NULL.  nthreads <= 0 uses all cores. */

@@ -1,0 +1,126 @@
+"""Batched ensemble: every box of one persistent launch against the single-box
+path and the FP64 oracle on identical states; tilings agree; per-box
+fluctuation statistics (MDSystem.cpp:708-730 sub-volume counts) match a host
+recount."""
+import numpy as np
+import pytest
+
+import ljamd
+from helpers import force_errors, rel, xyz
+from oracle_lib import ORC, forces_fp64, lattice_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_batch(nb=10, seed=3):
+    N, rho, T0 = ljamd.ensemble_boxes(nb, seed=seed, n_range=(200, 1600))
+    return N, rho, T0
+
+
+def _perturbed(N, rho, T0, seed):
+    L, pos, vel = lattice_state(int(N), float(rho), float(T0), seed)
+    rng = np.random.default_rng(seed)
+    n = ORC.orc_lattice_n(int(N))
+    p = pos.reshape(-1, 4)
+    p[:, :3] += rng.normal(0, 0.05 * L / n, (int(N), 3)).astype(np.float32)
+    return L, p.reshape(-1).copy(), vel
+
+
+@pytest.mark.parametrize("tuning", [(1, 512, 8), (2, 256, 0), (1, 2048, 4), (2, 1024, 8)])
+def test_batch_forces_vs_oracle(tuning):
+    N, rho, T0 = _small_batch()
+    print(f"\n[case] batch forces tuning={tuning}", flush=True)
+    states = [_perturbed(N[k], rho[k], T0[k], 100 + k) for k in range(len(N))]
+    with ljamd.LJBatch(N, rho, T0) as b:
+        b.set_tuning(*tuning)
+        b.set_array(ljamd.ARRAY_POS, np.concatenate([s[1] for s in states]))
+        b.set_array(ljamd.ARRAY_VEL, np.concatenate([s[2] for s in states]))
+        b.compute_forces()
+        obs = b.observables()
+        for k in range(len(N)):
+            L, pos, _ = states[k]
+            f4 = b.get_array(ljamd.ARRAY_FORCE, box=k)
+            F64, _, E, W = forces_fp64(pos, L, True)
+            err = force_errors(xyz(f4), F64)
+            assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5, (k, err)
+            assert rel(obs[k]["V"], E) <= 1e-6 and rel(obs[k]["W"], W) <= 1e-5
+
+
+def test_batch_tilings_agree_and_persistent_is_bitwise():
+    N, rho, T0 = _small_batch(seed=5)
+    outs = {}
+    for tuning in [(1, 512, 0), (1, 512, 8), (2, 1024, 4)]:
+        with ljamd.LJBatch(N, rho, T0, seed=11) as b:
+            b.set_tuning(*tuning)
+            b.compute_forces()
+            outs[tuning] = b.get_array(ljamd.ARRAY_FORCE)
+    assert np.array_equal(outs[(1, 512, 0)], outs[(1, 512, 8)])  # same items, any wave order
+    F1, F2 = xyz(outs[(1, 512, 0)]), xyz(outs[(2, 1024, 4)])
+    assert force_errors(F2, F1)["global_rel"] <= 2e-6
+
+
+@pytest.mark.parametrize("canon", [ljamd.EVN, ljamd.TVN])
+def test_batch_trajectory_matches_single_box(canon):
+    N, rho, T0 = _small_batch(nb=6, seed=9)
+    nsteps = 40
+    with ljamd.LJBatch(N, rho, T0, canonical=canon, seed=21) as b:
+        b.autotune(reps=2)
+        b.integrate(0.005, nsteps)
+        obs_b = b.observables()
+        pos_b = [b.get_array(ljamd.ARRAY_POS, box=k) for k in range(len(N))]
+    for k in range(len(N)):
+        with ljamd.LJSystem(int(N[k]), float(rho[k]), float(T0[k]), canonical=canon) as s:
+            s.sample_initial_state(21 + k)
+            h = s.Integrate(0.005, nsteps)
+            o = s.observables()
+            p = s.pos
+        du = abs(obs_b[k]["U"] - o["U"]) / N[k]
+        dx = np.max(np.abs(xyz(pos_b[k]) - xyz(p)))
+        assert obs_b[k]["step"] == nsteps
+        assert du <= 1e-4 and dx <= 1e-3, (k, du, dx)
+
+
+def test_batch_fluctuation_stats():
+    N, rho, T0 = _small_batch(nb=8, seed=13)
+    frac = 0.3
+    with ljamd.LJBatch(N, rho, T0, seed=4) as b:
+        b.set_subvolume(frac)
+        b.integrate(0.005, 10)
+        b.reset_stats()
+        b.integrate(0.005, 1)
+        st = b.stats()
+        obs = b.observables()
+        for k in range(len(N)):
+            L = ORC.orc_box_length(int(N[k]), float(rho[k]))
+            p = xyz(b.get_array(ljamd.ARRAY_POS, box=k))
+            side = L * frac ** (1.0 / 3.0)
+            lo, hi = 0.5 * L - 0.5 * side, 0.5 * L + 0.5 * side
+            n = np.sum(np.all((p >= lo) & (p <= hi), axis=1))
+            assert st[k, 0] == 1.0
+            assert st[k, 9] == n and st[k, 10] == n * n
+            assert st[k, 1] == obs[k]["U"] and st[k, 2] == obs[k]["U"] ** 2
+        b.integrate(0.005, 5)
+        st = b.stats()
+        assert np.all(st[:, 0] == 6.0)
+        var = st[:, 2] / 6 - (st[:, 1] / 6) ** 2
+        assert np.all(var >= -1e-9 * np.abs(st[:, 2] / 6))
+
+
+def test_batch_config_workload_builds():
+    """The 512-box workload (N ~ U{200..4000}) builds, steps and stays finite."""
+    N, rho, T0 = ljamd.ensemble_boxes(512, seed=7)
+    with ljamd.LJBatch(N, rho, T0, seed=1) as b:
+        b.integrate(0.005, 3)
+        obs = b.observables()
+    U = np.array([o["U"] for o in obs])
+    assert np.all(np.isfinite(U)) and all(o["step"] == 3 for o in obs)
+
+
+def test_batch_bad_args():
+    with pytest.raises(ljamd.LJError):
+        ljamd.LJBatch([100, 0], [0.5, 0.5], [1.0, 1.0])
+    with ljamd.LJBatch([100, 200], [0.5, 0.5], [1.0, 1.0]) as b:
+        with pytest.raises(ljamd.LJError):
+            b.set_tuning(3, 512, 0)
+        with pytest.raises(ljamd.LJError):
+            b.set_subvolume(0.0)

@@ -11,7 +11,7 @@ boxes, plus the exact config lattice states.
 Tolerances (FP32 pair arithmetic, FP64 cross-slice sums):
   forces vs FP64:  global ||dF||/||F|| <= 2e-6,
                    per particle |dF_i| / max(|F_i|, F_rms) <= 1e-5;
-  E_pot, W vs FP64: relative <= 1e-6;
+  E_pot vs FP64: relative <= 1e-6;  virial W: relative <= 1e-5;
   forces vs reference CPU path: per particle <= 1e-5 + the reference's own
                    deviation from FP64 at that particle (it accumulates in float).
 """
@@ -55,7 +55,7 @@ def test_forces_vs_fp64(N, rho, T0, bc):
     print(f"  vs FP64: {err}  E rel {eE:.2e}  W rel {eW:.2e}  Force.w max {ew_err:.2e}")
     assert err["global_rel"] <= 2e-6
     assert err["max_rel_floor"] <= 1e-5
-    assert eE <= 1e-6 and eW <= 1e-6
+    assert eE <= 1e-6 and eW <= 1e-5  # W has cancellation between repulsive/attractive parts
     assert ew_err <= 1e-5
 
 
@@ -85,11 +85,11 @@ def test_config_lattice_state(N, rho):
     L, pos, vel = lattice_state(N, rho, 1.0, seed=1)
     f4, obs = _gpu_forces(N, rho, ljamd.BC_PERIODIC, pos)
     F64, _, E, W = forces_fp64(pos, L, True)
-    Fg = xyz(f4)
-    scale = np.sqrt(np.mean(np.sum(F64 ** 2, axis=1))) + 1.0
-    dmax = np.max(np.linalg.norm(Fg - F64, axis=1)) / scale
-    print(f"\n  lattice N={N}: max|dF|/(F_rms+1) {dmax:.2e} E {rel(obs['V'], E):.2e}")
-    assert dmax <= 1e-5
+    # partially filled lattices: interior forces vanish, surface ones are large;
+    # the per-particle metric uses the rms force as floor
+    err = force_errors(xyz(f4), F64)
+    print(f"\n  lattice N={N}: {err} E {rel(obs['V'], E):.2e}")
+    assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5
     assert rel(obs["V"], E) <= 1e-6 and rel(obs["W"], W) <= 2e-6
 
 
