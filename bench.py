@@ -36,10 +36,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_leg(steps, warmup):
+def profiled_traffic():
+    """HBM bytes per force launch from the committed rocprofv3 summary."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "k_force_n16384_periodic.md")
+    try:
+        m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def gpu_leg(steps, warmup, boundary=None):
     import ljamd
 
-    s = ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, boundary=ljamd.BC_PERIODIC, seed=SEED)
+    bc = ljamd.BC_PERIODIC if boundary is None else boundary
+    s = ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, boundary=bc, seed=SEED)
     log(f"[bench] warmup {warmup} steps")
     s.Integrate(DT, max(warmup, 1))
     s.synchronize()
@@ -132,6 +145,31 @@ def cpu_leg(budget_s=15.0):
             "steps_per_s": steps / tt}
 
 
+def profile_force(reps=20, copy_mib=256):
+    """Launch pattern for counter collection: a device-to-device copy of a
+    known size (calibrates FETCH_SIZE / WRITE_SIZE per byte moved), then the
+    force kernel alone `reps` times at the headline workload."""
+    import ctypes as C
+
+    import ljamd
+
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = copy_mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    for _ in range(3):
+        hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED) as s:
+        us = s.time_force_kernel(reps)
+    print(json.dumps({"profile": "force", "N": N, "reps": reps, "kernel_us": us,
+                      "copy_bytes": nbytes}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +178,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--profile-force", action="store_true",
+                    help="only a calibration copy and the force kernel (for rocprofv3)")
     a = ap.parse_args()
+    if a.profile_force:
+        profile_force()
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
               "the multi-GPU sharding is a design note, not a runnable path", file=sys.stderr)
@@ -151,6 +194,13 @@ def main():
     value = pairs * a.steps / g["t"]
     ms_per_step = 1000.0 * g["t"] / a.steps
     achieved = FLOP_PER_PAIR * pairs / (g["force_us"] * 1e-6) / 1e12
+    import ljamd
+
+    wall_steps = min(a.steps, 300)
+    gw = gpu_leg(wall_steps, a.warmup, boundary=ljamd.BC_WALL)
+    wall = {"steps": wall_steps, "pair_interactions_per_s": pairs * wall_steps / gw["t"],
+            "ms_per_step": 1000.0 * gw["t"] / wall_steps, "force_kernel_us": gw["force_us"],
+            "force_tflops": FLOP_PER_PAIR * pairs / (gw["force_us"] * 1e-6) / 1e12}
     batched = None if a.no_batch else batch_leg(a.batch_steps)
     cpu = None if a.no_cpu else cpu_leg()
     line = {
@@ -175,8 +225,10 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "frac_scalar_fp32": achieved / PEAK_FP32_SCALAR_TFLOPS,
                      "kernel": "k_force (periodic, fixed-point minimum image)",
-                     "kernel_us": g["force_us"], "traffic": None},
+                     "kernel_us": g["force_us"], "traffic": profiled_traffic(),
+                     "issue_slots_per_pair": 22},
         "cpu_baseline": cpu,
+        "hard_wall": wall,
         "batched": batched,
         "final_T": g["obs"]["T"],
         "final_u": g["obs"]["U"] / N,
