@@ -114,6 +114,25 @@ def batch_leg(steps, plain_steps=10, nboxes=512):
             "force_pass_us": force_us, "ms_per_step": 1000.0 * tb / steps}
 
 
+def adjoint_leg():
+    """Differentiable workload: N=2048, rho*=0.85, T*=1, periodic, 200 EVN steps,
+    dt=0.005, checkpoints every 20; gradient of <E_pot> + (<P> - 1)^2 with
+    respect to eps, sigma and x0.  Cost relative to the forward trajectory."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ljamd
+
+    Na, rho, M, dt = 2048, 0.85, 200, 0.005
+    with ljamd.LJSystem(Na, rho, 1.0, seed=4) as s:
+        s.trajectory_grad(dt, M, 20)  # warm: graph capture, scratch
+        out, g2, _, t20 = s.trajectory_grad(dt, M, 20)
+        _, _, _, t1 = s.trajectory_grad(dt, M, 1)
+    return {"N": Na, "steps": M, "checkpoint_every": 20, "loss": out[0],
+            "dL_deps": g2[0], "dL_dsigma": g2[1], "forward_ms": t20[0],
+            "recompute_ms": t20[1], "reverse_ms": t20[2],
+            "gradient_over_forward_k20": (t20[1] + t20[2]) / t20[0],
+            "gradient_over_forward_k1": t1[2] / t1[0]}
+
+
 def cpu_leg(budget_s=15.0):
     """Reference CPU path (oracle/_ref/libref_cpu.so) on the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -202,6 +221,7 @@ def main():
             "ms_per_step": 1000.0 * gw["t"] / wall_steps, "force_kernel_us": gw["force_us"],
             "force_tflops": FLOP_PER_PAIR * pairs / (gw["force_us"] * 1e-6) / 1e12}
     batched = None if a.no_batch else batch_leg(a.batch_steps)
+    adjoint = adjoint_leg()
     cpu = None if a.no_cpu else cpu_leg()
     line = {
         "metric": METRIC,
@@ -230,6 +250,7 @@ def main():
         "cpu_baseline": cpu,
         "hard_wall": wall,
         "batched": batched,
+        "adjoint": adjoint,
         "final_T": g["obs"]["T"],
         "final_u": g["obs"]["U"] / N,
     }

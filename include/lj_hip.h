@@ -216,6 +216,40 @@ int lj_time_force_kernel(lj_ctx* ctx, int reps, double* us_per_launch);
 int lj_forces_device(lj_ctx* ctx, const float* d_pos4, float* d_force4, double* d_ev,
                      void* stream);
 
+/* Device-to-device state transfer (float4[N]) on `stream` (NULL = context stream). */
+int lj_set_array_device(lj_ctx* ctx, int type, const float* d_src4, void* stream);
+int lj_get_array_device(lj_ctx* ctx, int type, float* d_dst4, void* stream);
+
+/* LJ parameters of the context (the reference uses eps = sigma = 1). */
+int lj_set_lj_params(lj_ctx* ctx, double eps, double sigma);
+
+/* ---- adjoint (differentiable) path ---- */
+/* One adjoint pair pass at positions d_pos4 with vector d_z4 (float4[N], w ignored):
+ *   d_hz4  (double4[N])  (H z)_i, H = dF/dx (Hessian-vector product of the pair force)
+ *   d_f4   (double4[N])  forces, w = sum_j ((s/r)^12 - (s/r)^6)
+ *   d_gw4  (double4[N])  dW/dx_i of the virial W = sum_{i<j} r.f
+ *   d_scal8 (double[8])  {E, W, dE/dsigma, dW/dsigma, (dF/dsigma)^T z, F^T z, 0, 0}
+ * Any output may be NULL.  Synchronous. */
+int lj_adjoint_pass_device(lj_ctx* ctx, const float* d_pos4, const float* d_z4, double* d_hz4,
+                           double* d_f4, double* d_gw4, double* d_scal8, void* stream);
+
+/* Loss of nsteps EVN velocity-Verlet steps (MDSystem.cpp:442-464) from the
+ * context state:  L = (1/M) sum_m E_pot(x_m) + w_p ((1/M) sum_m P_m - p_target)^2
+ * over the states after steps m = 1..M, P as CalculateParameters (MDSystem.cpp:348-350).
+ * out3 = {L, mean E_pot, mean P}.  The context state is left unchanged. */
+int lj_trajectory_loss(lj_ctx* ctx, double dt, int nsteps, double p_target, double w_p,
+                       double* out3);
+/* The same loss and its gradient with respect to eps, sigma (grad_eps_sigma[2])
+ * and the initial positions (dx0_host4: host float4[N], and/or d_dx0_4: device
+ * float4[N]; either may be NULL), by a reverse sweep with one Hessian-vector
+ * product per state, checkpoints every `checkpoint_every` steps and
+ * recomputation inside each segment (checkpoint_every <= 0: one segment).
+ * times_ms3 (may be NULL) = {forward, recompute, reverse} milliseconds.
+ * EVN only; periodic or open boxes.  The context state is left unchanged. */
+int lj_trajectory_grad(lj_ctx* ctx, double dt, int nsteps, int checkpoint_every, double p_target,
+                       double w_p, double* out3, double* grad_eps_sigma, float* dx0_host4,
+                       float* d_dx0_4, double* times_ms3);
+
 /* ------------------------------------------------------------------ */
 /* (4) batched ensemble: many independent boxes in one launch          */
 /* ------------------------------------------------------------------ */

@@ -36,7 +36,7 @@ HIPFLAGS = [
 ]
 
 SOURCES = ["lj_kernels.hip", "lj_batch.hip", "lj_adjoint.hip", "lj_host.cpp", "lj_api.cpp",
-           "lj_batch_api.cpp"]
+           "lj_batch_api.cpp", "lj_adjoint_api.cpp"]
 
 
 def _run(cmd, cwd=None):

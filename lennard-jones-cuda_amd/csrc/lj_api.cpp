@@ -10,8 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "lj_internal.h"
-#include "lj_util.h"
+#include "lj_ctx.h"
 
 using namespace lj;
 
@@ -33,44 +32,7 @@ int fail(int code, const char* fmt, ...)
 const std::string& last_error() { return g_err; }
 }  // namespace lj
 
-// ---------------------------------------------------------------------------
-// context
-
-struct lj_ctx {
-  lj_config cfg{};
-  double L = 0.;
-  hipStream_t stream = nullptr;
-  ForcePlan plan;
-  PairScalars ps;
-  float4* pos = nullptr;
-  float4* vel = nullptr;
-  float4* force = nullptr;
-  float4* tF = nullptr;
-  uint4* q = nullptr;
-  float4* fpart = nullptr;
-  float* apart = nullptr;
-  double* blk = nullptr;
-  int nblk = 0;
-  lj_obs* obs = nullptr;
-  double* hist = nullptr;
-  int hist_cap = 0;
-  int64_t* hist_base = nullptr;
-  double* ev = nullptr;
-  unsigned long long* rdf = nullptr;
-  int rdf_cap = 0;
-  int rdf_nbins = 0, rdf_rule = -1;
-  float rdf_dr2 = 0.f;
-  double rdf_rmax = 0.;
-  float* edges2 = nullptr;
-  int edges_cap = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  double g_dt = 0.;
-  int g_canon = -1, g_bc = -1;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-};
-
-namespace {
+namespace lj {
 
 int ctx_setdev(lj_ctx* c)
 {
@@ -128,7 +90,7 @@ int enqueue_force(lj_ctx* c, hipStream_t st, const float4* d_pos)
 {
   if (c->ps.periodic) LJ_HIP(launch_prep_q(st, d_pos, c->q, c->cfg.N, c->ps.qscale));
   LJ_HIP(launch_force(st, c->plan, c->ps, c->q, d_pos, c->fpart, c->apart,
-                      c->cfg.precision == LJ_PREC_COMPENSATED));
+                      c->cfg.precision == LJ_PREC_COMPENSATED, true));
   return LJ_OK;
 }
 
@@ -138,7 +100,7 @@ int enqueue_step(lj_ctx* c, hipStream_t st, double dt)
   LJ_HIP(launch_drift(st, a));
   // the drift kernel already wrote the fixed-point copy for periodic boxes
   LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart,
-                      c->cfg.precision == LJ_PREC_COMPENSATED));
+                      c->cfg.precision == LJ_PREC_COMPENSATED, false));
   if (c->cfg.canonical == LJ_EVN) {
     LJ_HIP(launch_kick_evn(st, a));
   } else {
@@ -199,7 +161,7 @@ int compute_forces_internal(lj_ctx* c)
   return LJ_OK;
 }
 
-}  // namespace
+}  // namespace lj
 
 extern "C" {
 
@@ -240,6 +202,7 @@ int lj_destroy(lj_ctx* c)
   (void)hipSetDevice(c->cfg.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
+  adj_free(c);
   dev_free(c->pos);
   dev_free(c->vel);
   dev_free(c->force);
@@ -567,10 +530,10 @@ int lj_time_force_kernel(lj_ctx* c, int reps, double* us)
   hipStream_t st = c->stream;
   if (c->ps.periodic) LJ_HIP(launch_prep_q(st, c->pos, c->q, c->cfg.N, c->ps.qscale));
   const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
-  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp));  // warm
+  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));  // warm
   LJ_HIP(hipEventRecord(c->e0, st));
   for (int k = 0; k < reps; ++k)
-    LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp));
+    LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));
   LJ_HIP(hipEventRecord(c->e1, st));
   LJ_HIP(hipEventSynchronize(c->e1));
   float ms = 0.f;
@@ -681,7 +644,7 @@ int compat_forces(float* Pos, float* Force, float* host_pressure, int N, float h
   hipStream_t st = s->st;
   const float4* p = reinterpret_cast<const float4*>(Pos);
   if (ps.periodic) LJ_HIP(launch_prep_q(st, p, s->q, N, ps.qscale));
-  LJ_HIP(launch_force(st, pl, ps, s->q, p, s->fpart, s->apart, false));
+  LJ_HIP(launch_force(st, pl, ps, s->q, p, s->fpart, s->apart, false, true));
   LJ_HIP(launch_reduce_forces(st, pl, ps, s->fpart, s->apart, reinterpret_cast<float4*>(Force), s->blk));
   LJ_HIP(launch_finalize_forces(st, s->blk, (N + kBlock - 1) / kBlock, 1., nullptr, s->pressure));
   if (host_RDF) {

@@ -12,7 +12,7 @@ namespace {
 
 using namespace dev;
 
-template <bool PERIODIC, int IPL>
+template <bool PERIODIC, int IPL, bool TIES>
 __device__ __forceinline__ void batch_item(const uint4* __restrict__ q, const float4* __restrict__ pf,
                                            float4* __restrict__ fpart, float* __restrict__ apart,
                                            const BoxParam& bp, int i0, int s, int lane)
@@ -34,11 +34,11 @@ __device__ __forceinline__ void batch_item(const uint4* __restrict__ q, const fl
   }
   const int d0 = max(j0, i0), d1 = min(j1, i0 + 64 * IPL);
   if (d0 >= d1) {
-    j_range<PERIODIC, 0, IPL, false>(qb, pb, j0, j1, pi, ii, bp.C1, 0.f, hi, lo);
+    j_range<PERIODIC, 0, IPL, false, TIES>(qb, pb, j0, j1, pi, ii, bp.C1, 0.f, hi, lo);
   } else {
-    j_range<PERIODIC, 0, IPL, false>(qb, pb, j0, d0, pi, ii, bp.C1, 0.f, hi, lo);
-    j_range<PERIODIC, 1, IPL, false>(qb, pb, d0, d1, pi, ii, bp.C1, 0.f, hi, lo);
-    j_range<PERIODIC, 0, IPL, false>(qb, pb, d1, j1, pi, ii, bp.C1, 0.f, hi, lo);
+    j_range<PERIODIC, 0, IPL, false, TIES>(qb, pb, j0, d0, pi, ii, bp.C1, 0.f, hi, lo);
+    j_range<PERIODIC, 1, IPL, false, TIES>(qb, pb, d0, d1, pi, ii, bp.C1, 0.f, hi, lo);
+    j_range<PERIODIC, 0, IPL, false, TIES>(qb, pb, d1, j1, pi, ii, bp.C1, 0.f, hi, lo);
   }
 #pragma unroll
   for (int k = 0; k < IPL; ++k) {
@@ -52,7 +52,7 @@ __device__ __forceinline__ void batch_item(const uint4* __restrict__ q, const fl
 
 // PERSIST: waves pull items from the queue head until it passes nitems (every
 // wave reaches that exit); otherwise wave w of the grid takes item w.
-template <bool PERIODIC, int IPL, bool PERSIST>
+template <bool PERIODIC, int IPL, bool PERSIST, bool TIES>
 __global__ __launch_bounds__(kBlock) void k_batch_force(const uint4* __restrict__ q,
                                                         const float4* __restrict__ pf,
                                                         float4* __restrict__ fpart,
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(const uint4* __restrict_
       if (it >= (unsigned)nitems) break;
       const BatchItem w = items[it];
       const BoxParam bp = box[w.box];
-      batch_item<PERIODIC, IPL>(q, pf, fpart, apart, bp, w.i0, w.s, lane);
+      batch_item<PERIODIC, IPL, TIES>(q, pf, fpart, apart, bp, w.i0, w.s, lane);
     }
   } else {
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(const uint4* __restrict_
     if (it >= nitems) return;
     const BatchItem w = items[it];
     const BoxParam bp = box[w.box];
-    batch_item<PERIODIC, IPL>(q, pf, fpart, apart, bp, w.i0, w.s, lane);
+    batch_item<PERIODIC, IPL, TIES>(q, pf, fpart, apart, bp, w.i0, w.s, lane);
   }
 }
 
@@ -340,14 +340,15 @@ __global__ void k_batch_zero_stats(double* stats, int n, lj_obs* obs, int nboxes
   }
 }
 
-template <bool PERIODIC, int IPL>
+template <bool PERIODIC, int IPL, bool TIES>
 void launch_batch_force_t(hipStream_t st, const BatchArgs& a)
 {
   if (a.persistent_grid > 0)
-    hipLaunchKernelGGL((k_batch_force<PERIODIC, IPL, true>), dim3(a.persistent_grid), dim3(kBlock),
-                       0, st, a.q, a.pos, a.fpart, a.apart, a.box, a.items, a.counter, a.nitems);
+    hipLaunchKernelGGL((k_batch_force<PERIODIC, IPL, true, TIES>), dim3(a.persistent_grid),
+                       dim3(kBlock), 0, st, a.q, a.pos, a.fpart, a.apart, a.box, a.items,
+                       a.counter, a.nitems);
   else
-    hipLaunchKernelGGL((k_batch_force<PERIODIC, IPL, false>),
+    hipLaunchKernelGGL((k_batch_force<PERIODIC, IPL, false, TIES>),
                        dim3((a.nitems + kWaves - 1) / kWaves), dim3(kBlock), 0, st, a.q, a.pos,
                        a.fpart, a.apart, a.box, a.items, a.counter, a.nitems);
 }
@@ -369,14 +370,19 @@ hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a)
   return hipGetLastError();
 }
 
-hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic)
+hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic, bool ties)
 {
   if (periodic) {
-    if (a.ipl == 2) launch_batch_force_t<true, 2>(st, a);
-    else launch_batch_force_t<true, 1>(st, a);
+    if (ties) {
+      if (a.ipl == 2) launch_batch_force_t<true, 2, true>(st, a);
+      else launch_batch_force_t<true, 1, true>(st, a);
+    } else {
+      if (a.ipl == 2) launch_batch_force_t<true, 2, false>(st, a);
+      else launch_batch_force_t<true, 1, false>(st, a);
+    }
   } else {
-    if (a.ipl == 2) launch_batch_force_t<false, 2>(st, a);
-    else launch_batch_force_t<false, 1>(st, a);
+    if (a.ipl == 2) launch_batch_force_t<false, 2, false>(st, a);
+    else launch_batch_force_t<false, 1, false>(st, a);
   }
   return hipGetLastError();
 }

@@ -183,7 +183,7 @@ int enqueue_batch_step(lj_batch* b, hipStream_t st, double dt)
 {
   const BatchArgs a = bargs(b, dt);
   LJ_HIP(launch_batch_drift(st, a));
-  LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC));
+  LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC, false));
   if (b->canonical == LJ_EVN) {
     LJ_HIP(launch_batch_kick_evn(st, a));
   } else {
@@ -194,11 +194,11 @@ int enqueue_batch_step(lj_batch* b, hipStream_t st, double dt)
   return LJ_OK;
 }
 
-int enqueue_batch_forces(lj_batch* b, hipStream_t st)
+int enqueue_batch_forces(lj_batch* b, hipStream_t st, bool ties)
 {
   const BatchArgs a = bargs(b, 0.);
   LJ_HIP(launch_batch_prep_q(st, a));  // also resets the queue head
-  LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC));
+  LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC, ties));
   return LJ_OK;
 }
 
@@ -356,7 +356,7 @@ int lj_batch_compute_forces(lj_batch* b)
 {
   LJ_TRY(bsetdev(b));
   hipStream_t st = b->stream;
-  LJ_TRY(enqueue_batch_forces(b, st));
+  LJ_TRY(enqueue_batch_forces(b, st, true));
   const BatchArgs a = bargs(b, 0.);
   LJ_HIP(launch_batch_reduce_forces(st, a));
   LJ_HIP(launch_batch_kinetic(st, a));
@@ -436,9 +436,9 @@ int lj_batch_time_force(lj_batch* b, int reps, double* us)
   LJ_TRY(bsetdev(b));
   if (reps <= 0 || !us) return fail(LJ_ERR_ARG, "lj_batch_time_force: bad arguments");
   hipStream_t st = b->stream;
-  LJ_TRY(enqueue_batch_forces(b, st));  // warm
+  LJ_TRY(enqueue_batch_forces(b, st, false));  // warm
   LJ_HIP(hipEventRecord(b->e0, st));
-  for (int k = 0; k < reps; ++k) LJ_TRY(enqueue_batch_forces(b, st));
+  for (int k = 0; k < reps; ++k) LJ_TRY(enqueue_batch_forces(b, st, false));
   LJ_HIP(hipEventRecord(b->e1, st));
   LJ_HIP(hipEventSynchronize(b->e1));
   float ms = 0.f;

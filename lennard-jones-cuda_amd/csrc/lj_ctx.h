@@ -1,0 +1,55 @@
+// lj_ctx.h -- the context object behind lj_ctx* and the internal helpers
+// shared by the C-ABI implementation files.
+#pragma once
+
+#include "lj_internal.h"
+#include "lj_util.h"
+
+struct AdjState;  // adjoint scratch (lj_adjoint_api.cpp), allocated on first use
+
+struct lj_ctx {
+  lj_config cfg{};
+  double L = 0.;
+  hipStream_t stream = nullptr;
+  lj::ForcePlan plan;
+  lj::PairScalars ps;
+  float4* pos = nullptr;
+  float4* vel = nullptr;
+  float4* force = nullptr;
+  float4* tF = nullptr;
+  uint4* q = nullptr;
+  float4* fpart = nullptr;
+  float* apart = nullptr;
+  double* blk = nullptr;
+  int nblk = 0;
+  lj_obs* obs = nullptr;
+  double* hist = nullptr;
+  int hist_cap = 0;
+  int64_t* hist_base = nullptr;
+  double* ev = nullptr;
+  unsigned long long* rdf = nullptr;
+  int rdf_cap = 0;
+  int rdf_nbins = 0, rdf_rule = -1;
+  float rdf_dr2 = 0.f;
+  double rdf_rmax = 0.;
+  float* edges2 = nullptr;
+  int edges_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0.;
+  int g_canon = -1, g_bc = -1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  AdjState* adj = nullptr;
+};
+
+namespace lj {
+int ctx_setdev(lj_ctx* c);
+// step graph captured once per (dt, ensemble, boundary) and replayed
+int ensure_graph(lj_ctx* c, double dt);
+// device forces, E and W of the context positions into c->force and c->obs
+int compute_forces_internal(lj_ctx* c);
+void refresh_scalars(lj_ctx* c);
+void drop_graph(lj_ctx* c);
+// releases the adjoint scratch of a context
+void adj_free(lj_ctx* c);
+}  // namespace lj

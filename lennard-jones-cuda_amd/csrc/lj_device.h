@@ -57,16 +57,31 @@ __device__ __forceinline__ PosT<PERIODIC> load_pos(const uint4* q, const float4*
   return p;
 }
 
+// Minimum-image difference of two fixed-point coordinates: the modular
+// difference.  A separation of exactly half the box (0x80000000) is its own
+// negative; TIES resolves it as the reference does (fast_round / lround round
+// half away from zero, MDSystem.cpp:732-739, MDSystem.cu:31-33): +L/2 maps to
+// -L/2 when qi > qj and -L/2 to +L/2 when qi < qj, so the two orderings of the
+// pair take opposite images.
+template <bool TIES>
+__device__ __forceinline__ float mi_fixed(unsigned qi, unsigned qj)
+{
+  const unsigned d = qi - qj;
+  float f = (float)(int)d;
+  if constexpr (TIES) f = (d == 0x80000000u && qi < qj) ? 2147483648.f : f;
+  return f;
+}
+
 // GUARD: 0 no exclusion, 1 self by index (i == j), 2 pairs with r2 <= thr
-template <bool PERIODIC, int GUARD>
+template <bool PERIODIC, int GUARD, bool TIES = false>
 __device__ __forceinline__ void pair(const PosT<PERIODIC>& pi, const PosT<PERIODIC>& pj, int i,
                                      int j, float C1, float thr, Acc& c)
 {
   float dx, dy, dz;
   if constexpr (PERIODIC) {
-    dx = (float)(int)(pi.x - pj.x);  // modular difference == minimum image
-    dy = (float)(int)(pi.y - pj.y);
-    dz = (float)(int)(pi.z - pj.z);
+    dx = mi_fixed<TIES>(pi.x, pj.x);
+    dy = mi_fixed<TIES>(pi.y, pj.y);
+    dz = mi_fixed<TIES>(pi.z, pj.z);
   } else {
     dx = pi.x - pj.x;
     dy = pi.y - pj.y;
@@ -86,7 +101,7 @@ __device__ __forceinline__ void pair(const PosT<PERIODIC>& pi, const PosT<PERIOD
   c.b += w;
 }
 
-template <bool PERIODIC, int GUARD, int IPL, bool COMP>
+template <bool PERIODIC, int GUARD, int IPL, bool COMP, bool TIES = false>
 __device__ __forceinline__ void j_range(const uint4* __restrict__ q, const float4* __restrict__ pf,
                                         int jb, int je, const PosT<PERIODIC> (&pi)[IPL],
                                         const int (&ii)[IPL], float C1, float thr, Acc (&hi)[IPL],
@@ -101,7 +116,7 @@ __device__ __forceinline__ void j_range(const uint4* __restrict__ q, const float
     for (int j = b0; j < b1; ++j) {
       const PosT<PERIODIC> pj = load_pos<PERIODIC>(q, pf, j);
 #pragma unroll
-      for (int k = 0; k < IPL; ++k) pair<PERIODIC, GUARD>(pi[k], pj, ii[k], j, C1, thr, t[k]);
+      for (int k = 0; k < IPL; ++k) pair<PERIODIC, GUARD, TIES>(pi[k], pj, ii[k], j, C1, thr, t[k]);
     }
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {

@@ -50,9 +50,11 @@ PairScalars make_pair_scalars(double L, int periodic, double eps, double sigma, 
 
 // Kernel launchers (enqueue only).
 hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, double qscale);
+// ties: resolve exact half-box separations as the reference does (single
+// evaluations of arbitrary states); the step graph uses ties = false
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
-                        bool compensated);
+                        bool compensated, bool ties);
 // sum slices -> force (w = e_i); block partials of e, b
 hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                                 const float4* fpart, const float* apart, float4* force,
@@ -163,7 +165,7 @@ struct BatchArgs {
   int ipl, persistent_grid;
 };
 hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a);
-hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic);
+hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic, bool ties);
 hipError_t launch_batch_kick_evn(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_tvn_a(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_tvn_b(hipStream_t st, const BatchArgs& a);

@@ -31,7 +31,7 @@ using namespace dev;
 
 // grid: (gx, S); block: kBlock.  fpart[s*Npad + i] = (fx, fy, fz, b), apart = a,
 // in internal units (force = FS * f).
-template <bool PERIODIC, bool GUARDALL, int IPL, bool COMP>
+template <bool PERIODIC, bool GUARDALL, int IPL, bool COMP, bool TIES>
 __global__ __launch_bounds__(kBlock) void k_force(const uint4* __restrict__ q,
                                                   const float4* __restrict__ pf,
                                                   float4* __restrict__ fpart,
@@ -58,16 +58,16 @@ __global__ __launch_bounds__(kBlock) void k_force(const uint4* __restrict__ q,
   }
 
   if constexpr (GUARDALL) {
-    j_range<PERIODIC, 2, IPL, COMP>(q, pf, j0, j1, pi, ii, C1, thr, hi, lo);
+    j_range<PERIODIC, 2, IPL, COMP, TIES>(q, pf, j0, j1, pi, ii, C1, thr, hi, lo);
   } else {
     // the self pair lies only in [i0, i0 + 64*IPL): guard that segment alone
     const int d0 = max(j0, i0), d1 = min(j1, i0 + 64 * IPL);
     if (d0 >= d1) {
-      j_range<PERIODIC, 0, IPL, COMP>(q, pf, j0, j1, pi, ii, C1, thr, hi, lo);
+      j_range<PERIODIC, 0, IPL, COMP, TIES>(q, pf, j0, j1, pi, ii, C1, thr, hi, lo);
     } else {
-      j_range<PERIODIC, 0, IPL, COMP>(q, pf, j0, d0, pi, ii, C1, thr, hi, lo);
-      j_range<PERIODIC, 1, IPL, COMP>(q, pf, d0, d1, pi, ii, C1, thr, hi, lo);
-      j_range<PERIODIC, 0, IPL, COMP>(q, pf, d1, j1, pi, ii, C1, thr, hi, lo);
+      j_range<PERIODIC, 0, IPL, COMP, TIES>(q, pf, j0, d0, pi, ii, C1, thr, hi, lo);
+      j_range<PERIODIC, 1, IPL, COMP, TIES>(q, pf, d0, d1, pi, ii, C1, thr, hi, lo);
+      j_range<PERIODIC, 0, IPL, COMP, TIES>(q, pf, d1, j1, pi, ii, C1, thr, hi, lo);
     }
   }
 
@@ -475,26 +475,26 @@ __global__ void k_fill_u64(unsigned long long* p, int n)
 
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
 
-template <bool PERIODIC, bool GUARDALL, int IPL, bool COMP>
+template <bool PERIODIC, bool GUARDALL, int IPL, bool COMP, bool TIES>
 hipError_t launch_force_t(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                           const uint4* q, const float4* posf, float4* fpart, float* apart)
 {
   dim3 grid(pl.gx, pl.S);
-  hipLaunchKernelGGL((k_force<PERIODIC, GUARDALL, IPL, COMP>), grid, dim3(kBlock), 0, st, q,
+  hipLaunchKernelGGL((k_force<PERIODIC, GUARDALL, IPL, COMP, TIES>), grid, dim3(kBlock), 0, st, q,
                      posf, fpart, apart, pl.N, pl.Npad, pl.jchunk, ps.C1, ps.thr);
   return hipGetLastError();
 }
 
-template <bool PERIODIC, bool GUARDALL>
+template <bool PERIODIC, bool GUARDALL, bool TIES>
 hipError_t launch_force_c(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                           const uint4* q, const float4* posf, float4* fpart, float* apart,
                           bool comp)
 {
   if (pl.ipl == 2)
-    return comp ? launch_force_t<PERIODIC, GUARDALL, 2, true>(st, pl, ps, q, posf, fpart, apart)
-                : launch_force_t<PERIODIC, GUARDALL, 2, false>(st, pl, ps, q, posf, fpart, apart);
-  return comp ? launch_force_t<PERIODIC, GUARDALL, 1, true>(st, pl, ps, q, posf, fpart, apart)
-              : launch_force_t<PERIODIC, GUARDALL, 1, false>(st, pl, ps, q, posf, fpart, apart);
+    return comp ? launch_force_t<PERIODIC, GUARDALL, 2, true, TIES>(st, pl, ps, q, posf, fpart, apart)
+                : launch_force_t<PERIODIC, GUARDALL, 2, false, TIES>(st, pl, ps, q, posf, fpart, apart);
+  return comp ? launch_force_t<PERIODIC, GUARDALL, 1, true, TIES>(st, pl, ps, q, posf, fpart, apart)
+              : launch_force_t<PERIODIC, GUARDALL, 1, false, TIES>(st, pl, ps, q, posf, fpart, apart);
 }
 
 }  // namespace
@@ -537,14 +537,9 @@ PairScalars make_pair_scalars(double L, int periodic, double eps, double sigma, 
   ps.sigma = sigma;
   const double s2 = sigma * sigma;
   if (periodic) {
-    // One fixed-point unit is u = L / (2^32 (1 + 2^-30)): the box spans
-    // 2^32 + 4 units, so a separation of exactly L/2 maps to 2^31 + 2 +- 1
-    // units and never to the self-symmetric 2^31.  Its two orderings then wrap
-    // to opposite images, as the reference's half-away-from-zero rounding
-    // does (MDSystem.cpp:732-739, MDSystem.cu:31-33).  The price is that
-    // wrapped pairs see the image at L (1 - 2^-30): 2.5e-8 at L = 26.8.
-    const double u = L / (4294967296.0 * (1.0 + 0x1p-30));
-    ps.qscale = 1.0 / u;
+    // one fixed-point unit is u = L / 2^32: the box spans exactly 2^32 units
+    const double u = L / 4294967296.0;
+    ps.qscale = 4294967296.0 / L;
     ps.C1 = (float)(s2 / (u * u));
     ps.FS = 24. * eps * u / s2;
     ps.thr = guard_all ? (float)(1e-6 / (u * u)) : 0.f;
@@ -565,14 +560,18 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
-                        bool compensated)
+                        bool compensated, bool ties)
 {
   const bool ga = ps.thr > 0.f;
-  if (ps.periodic)
-    return ga ? launch_force_c<true, true>(st, pl, ps, q, posf, fpart, apart, compensated)
-              : launch_force_c<true, false>(st, pl, ps, q, posf, fpart, apart, compensated);
-  return ga ? launch_force_c<false, true>(st, pl, ps, q, posf, fpart, apart, compensated)
-            : launch_force_c<false, false>(st, pl, ps, q, posf, fpart, apart, compensated);
+  if (ps.periodic) {
+    if (ties)
+      return ga ? launch_force_c<true, true, true>(st, pl, ps, q, posf, fpart, apart, compensated)
+                : launch_force_c<true, false, true>(st, pl, ps, q, posf, fpart, apart, compensated);
+    return ga ? launch_force_c<true, true, false>(st, pl, ps, q, posf, fpart, apart, compensated)
+              : launch_force_c<true, false, false>(st, pl, ps, q, posf, fpart, apart, compensated);
+  }
+  return ga ? launch_force_c<false, true, false>(st, pl, ps, q, posf, fpart, apart, compensated)
+            : launch_force_c<false, false, false>(st, pl, ps, q, posf, fpart, apart, compensated);
 }
 
 hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,

@@ -95,6 +95,12 @@ SIGNATURES = [
     ("lj_device_arrays", _i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     ("lj_time_force_kernel", _i, [_vp, _i, C.POINTER(_d)]),
     ("lj_forces_device", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("lj_set_array_device", _i, [_vp, _i, _vp, _vp]),
+    ("lj_get_array_device", _i, [_vp, _i, _vp, _vp]),
+    ("lj_set_lj_params", _i, [_vp, _d, _d]),
+    ("lj_adjoint_pass_device", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("lj_trajectory_loss", _i, [_vp, _d, _i, _d, _d, _vp]),
+    ("lj_trajectory_grad", _i, [_vp, _d, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp]),
     ("lj_batch_create", _i, [_i, _vp, _vp, _vp, _i, _i, _i, C.POINTER(_vp)]),
     ("lj_batch_destroy", _i, [_vp]),
     ("lj_batch_num_particles", _i, [_vp]),

@@ -191,3 +191,47 @@ class CompatArrays:
         frc = np.zeros(4 * self.N, np.float32)
         lib().copyArrayFromDevice(ptr(frc), self.d_force, 0, self.N)
         return frc, pressure.value, rdf
+
+
+# ---- differentiable path (lj_trajectory_*, lj_adjoint_pass_device) ----
+
+def _set_lj_params(self, eps, sigma):
+    check(lib().lj_set_lj_params(self.h, float(eps), float(sigma)))
+
+
+def _trajectory_loss(self, dt, nsteps, p_target=1.0, w_p=1.0):
+    """(loss, mean E_pot, mean P) of nsteps EVN steps from the current state."""
+    out = np.zeros(3)
+    check(lib().lj_trajectory_loss(self.h, float(dt), int(nsteps), float(p_target), float(w_p),
+                                   ptr(out)))
+    return out
+
+
+def _trajectory_grad(self, dt, nsteps, checkpoint_every=20, p_target=1.0, w_p=1.0, d_dx0=None):
+    """loss3, (dL/deps, dL/dsigma), dL/dx0 float32[N,4], times {fwd, recompute, reverse} ms."""
+    out, g2, times = np.zeros(3), np.zeros(2), np.zeros(3)
+    dx0 = np.zeros(4 * self.N, np.float32)
+    check(lib().lj_trajectory_grad(self.h, float(dt), int(nsteps), int(checkpoint_every),
+                                   float(p_target), float(w_p), ptr(out), ptr(g2), ptr(dx0),
+                                   C.c_void_p(d_dx0) if d_dx0 else None, ptr(times)))
+    return out, g2, dx0.reshape(self.N, 4), times
+
+
+def _adjoint_pass(self, pos4, z4):
+    """(H z, F, dW/dx) as float64 [N,4] and the 8 scalars, at host positions pos4."""
+    from ._hip import DeviceBuffer
+    N = self.N
+    dp = DeviceBuffer.from_array(np.ascontiguousarray(pos4, np.float32).reshape(N, 4))
+    dz = DeviceBuffer.from_array(np.ascontiguousarray(z4, np.float32).reshape(N, 4))
+    bufs = [DeviceBuffer(32 * N) for _ in range(3)]
+    sc = DeviceBuffer(64)
+    check(lib().lj_adjoint_pass_device(self.h, dp.ptr, dz.ptr, bufs[0].ptr, bufs[1].ptr,
+                                       bufs[2].ptr, sc.ptr, None))
+    hz, f, gw = (b.download(np.float64, (N, 4)) for b in bufs)
+    return hz, f, gw, sc.download(np.float64, (8,))
+
+
+LJSystem.set_lj_params = _set_lj_params
+LJSystem.trajectory_loss = _trajectory_loss
+LJSystem.trajectory_grad = _trajectory_grad
+LJSystem.adjoint_pass = _adjoint_pass

@@ -543,3 +543,198 @@ void orc_fp64_evn_step(int N, double* x, double* v, double* f, double L, int bc,
   if (Epot) *Epot = E;
   if (Ekin) *Ekin = K;
 }
+
+/* ---------------- adjoint (FP64) ----------------
+ * Pair quantities for the LJ pair of MDSystem.cpp:289-298 with eps, sigma:
+ *   phi(r^2)  = 24 eps (2 s^12 r^-14 - s^6 r^-8)        F_i = sum_j phi r_ij
+ *   phi'      = d phi / d(r^2) = 24 eps (-14 s^12 r^-16 + 4 s^6 r^-10)
+ *   psi(r^2)  = 24 eps (2 s^12 r^-12 - s^6 r^-6)        W = sum_{i<j} psi
+ *   psi'      = 24 eps (-12 s^12 r^-14 + 3 s^6 r^-8)
+ *   (H z)_i   = sum_j [phi (z_i - z_j) + 2 phi' (r.(z_i - z_j)) r]
+ *   dW/dx_i   = sum_j 2 psi' r_ij
+ *   dE/ds     = sum_{i<j} 4 eps (12 s^11 r^-12 - 6 s^5 r^-6)
+ *   dW/ds     = sum_{i<j} 24 eps (24 s^11 r^-12 - 6 s^5 r^-6)
+ *   dphi/ds   = 24 eps (24 s^11 r^-14 - 6 s^5 r^-8)
+ */
+void orc_adjoint_pass_fp64(int N, const double* x, const double* z, double L, int periodic,
+                           double eps, double s, double* hz3, double* f3, double* gw3,
+                           double* scal8, int nthreads)
+{
+  double E = 0., W = 0., Es = 0., Ws = 0., Fsz = 0., Fz = 0.;
+  const double s6 = pow(s, 6.), s12 = s6 * s6, s5 = s6 / s, s11 = s12 / s;
+  if (nthreads <= 0) {
+#ifdef _OPENMP
+    nthreads = omp_get_num_procs();
+#else
+    nthreads = 1;
+#endif
+  }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads) reduction(+ : E, W, Es, Ws, Fsz, Fz)
+  for (int i = 0; i < N; ++i) {
+    double h[3] = {0., 0., 0.}, f[3] = {0., 0., 0.}, g[3] = {0., 0., 0.};
+    for (int j = 0; j < N; ++j) {
+      if (j == i) continue;
+      double r[3], dz[3];
+      for (int c = 0; c < 3; ++c) {
+        r[c] = x[3 * i + c] - x[3 * j + c];
+        if (periodic) r[c] -= L * round(r[c] / L);
+        dz[c] = z[3 * i + c] - z[3 * j + c];
+      }
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      const double i2 = 1. / r2, i4 = i2 * i2, i6 = i4 * i2, i8 = i4 * i4, i10 = i8 * i2;
+      const double i12 = i6 * i6, i14 = i12 * i2, i16 = i8 * i8;
+      const double phi = 24. * eps * (2. * s12 * i14 - s6 * i8);
+      const double dphi = 24. * eps * (-14. * s12 * i16 + 4. * s6 * i10);
+      const double dpsi = 24. * eps * (-12. * s12 * i14 + 3. * s6 * i8);
+      const double phis = 24. * eps * (24. * s11 * i14 - 6. * s5 * i8);
+      const double rdz = r[0] * dz[0] + r[1] * dz[1] + r[2] * dz[2];
+      for (int c = 0; c < 3; ++c) {
+        h[c] += phi * dz[c] + 2. * dphi * rdz * r[c];
+        f[c] += phi * r[c];
+        g[c] += 2. * dpsi * r[c];
+      }
+      /* per ordered pair: half of each unordered-pair quantity */
+      E += 0.5 * 4. * eps * (s12 * i12 - s6 * i6);
+      W += 0.5 * 24. * eps * (2. * s12 * i12 - s6 * i6);
+      Es += 0.5 * 4. * eps * (12. * s11 * i12 - 6. * s5 * i6);
+      Ws += 0.5 * 24. * eps * (24. * s11 * i12 - 6. * s5 * i6);
+      Fsz += 0.5 * phis * rdz;
+    }
+    for (int c = 0; c < 3; ++c) {
+      if (hz3) hz3[3 * i + c] = h[c];
+      if (f3) f3[3 * i + c] = f[c];
+      if (gw3) gw3[3 * i + c] = g[c];
+      Fz += f[c] * z[3 * i + c];
+    }
+  }
+  if (scal8) {
+    scal8[0] = E;
+    scal8[1] = W;
+    scal8[2] = Es;
+    scal8[3] = Ws;
+    scal8[4] = Fsz;
+    scal8[5] = Fz;
+    scal8[6] = scal8[7] = 0.;
+  }
+}
+
+/* states x_m, v_m for m = 0..M (double[(M+1)*3N] each); E_m, W_m for m = 0..M */
+static void fp64_traj(int N, const double* x0, const double* v0, double L, int periodic,
+                      double eps, double s, double dt, int M, double* xs, double* vs,
+                      double* Es, double* Ws, int nthreads)
+{
+  const int n3 = 3 * N;
+  double* f = (double*)malloc(sizeof(double) * n3);
+  memcpy(xs, x0, sizeof(double) * n3);
+  memcpy(vs, v0, sizeof(double) * n3);
+  orc_forces_fp64d(N, xs, L, periodic, eps, s, f, NULL, &Es[0], &Ws[0], nthreads);
+  for (int m = 1; m <= M; ++m) {
+    const double* xp = xs + (size_t)(m - 1) * n3;
+    const double* vp = vs + (size_t)(m - 1) * n3;
+    double* xn = xs + (size_t)m * n3;
+    double* vn = vs + (size_t)m * n3;
+    for (int k = 0; k < n3; ++k) {
+      xn[k] = xp[k] + dt * vp[k] + 0.5 * dt * dt * f[k];
+      vn[k] = vp[k] + 0.5 * dt * f[k];
+    }
+    orc_forces_fp64d(N, xn, L, periodic, eps, s, f, NULL, &Es[m], &Ws[m], nthreads);
+    for (int k = 0; k < n3; ++k) vn[k] += 0.5 * dt * f[k];
+  }
+  free(f);
+}
+
+static void traj_loss_from(int N, int M, double L, const double* vs, const double* Es,
+                           const double* Ws, double pt, double wp, double* out3)
+{
+  const int n3 = 3 * N;
+  const double vol = L * L * L;
+  double se = 0., sp = 0.;
+  for (int m = 1; m <= M; ++m) {
+    double v2 = 0.;
+    const double* v = vs + (size_t)m * n3;
+    for (int k = 0; k < n3; ++k) v2 += v[k] * v[k];
+    se += Es[m];
+    sp += (Ws[m] / 3. + v2 / 3.) / vol;
+  }
+  const double mE = se / M, mP = sp / M;
+  out3[0] = mE + wp * (mP - pt) * (mP - pt);
+  out3[1] = mE;
+  out3[2] = mP;
+}
+
+void orc_traj_loss_fp64(int N, const double* x0, const double* v0, double L, int periodic,
+                        double eps, double s, double dt, int M, double pt, double wp,
+                        double* out3, int nthreads)
+{
+  const size_t n3 = (size_t)3 * N;
+  double* xs = (double*)malloc(sizeof(double) * n3 * (M + 1));
+  double* vs = (double*)malloc(sizeof(double) * n3 * (M + 1));
+  double* Es = (double*)malloc(sizeof(double) * (M + 1));
+  double* Ws = (double*)malloc(sizeof(double) * (M + 1));
+  fp64_traj(N, x0, v0, L, periodic, eps, s, dt, M, xs, vs, Es, Ws, nthreads);
+  traj_loss_from(N, M, L, vs, Es, Ws, pt, wp, out3);
+  free(xs);
+  free(vs);
+  free(Es);
+  free(Ws);
+}
+
+void orc_traj_grad_fp64(int N, const double* x0, const double* v0, double L, int periodic,
+                        double eps, double s, double dt, int M, double pt, double wp,
+                        double* out3, double* grad2, double* dx0, int nthreads)
+{
+  const size_t n3 = (size_t)3 * N;
+  double* xs = (double*)malloc(sizeof(double) * n3 * (M + 1));
+  double* vs = (double*)malloc(sizeof(double) * n3 * (M + 1));
+  double* Es = (double*)malloc(sizeof(double) * (M + 1));
+  double* Ws = (double*)malloc(sizeof(double) * (M + 1));
+  fp64_traj(N, x0, v0, L, periodic, eps, s, dt, M, xs, vs, Es, Ws, nthreads);
+  traj_loss_from(N, M, L, vs, Es, Ws, pt, wp, out3);
+  const double vol = L * L * L, invM = 1. / M, cP = 2. * wp * (out3[2] - pt);
+  const double cw = cP / (3. * vol), gvs = invM * cP * (2. / 3.) / vol;
+  double* u = (double*)calloc(n3, sizeof(double));
+  double* lv = (double*)malloc(sizeof(double) * n3);
+  double* z = (double*)malloc(sizeof(double) * n3);
+  double* w = (double*)malloc(sizeof(double) * n3);
+  double* hz = (double*)malloc(sizeof(double) * n3);
+  double* f = (double*)malloc(sizeof(double) * n3);
+  double* gw = (double*)malloc(sizeof(double) * n3);
+  double sc[8];
+  double ge = 0., gs = 0.;
+  /* top: lv_M = g_v(M), z_M = dt/2 lv_M */
+  for (size_t k = 0; k < n3; ++k) {
+    lv[k] = gvs * vs[(size_t)M * n3 + k];
+    z[k] = 0.5 * dt * lv[k];
+  }
+  for (int m = M; m >= 0; --m) {
+    orc_adjoint_pass_fp64(N, xs + (size_t)m * n3, z, L, periodic, eps, s, hz, f, gw, sc, nthreads);
+    ge += sc[5] / eps;
+    gs += sc[4];
+    if (m >= 1) {
+      ge += invM * (sc[0] / eps + cw * sc[1] / eps);
+      gs += invM * (sc[2] + cw * sc[3]);
+      for (size_t k = 0; k < n3; ++k) {
+        u[k] += invM * (-f[k] + cw * gw[k]) + hz[k];            /* u_{m-1} */
+        w[k] = 0.5 * dt * dt * u[k] + 0.5 * dt * lv[k];         /* w_{m-1} */
+        const double gv = (m - 1 >= 1) ? gvs * vs[(size_t)(m - 1) * n3 + k] : 0.;
+        lv[k] = dt * u[k] + lv[k] + gv;                         /* lv_{m-1} */
+        z[k] = (m - 1 >= 1) ? w[k] + 0.5 * dt * lv[k] : w[k];
+      }
+    } else if (dx0) {
+      for (size_t k = 0; k < n3; ++k) dx0[k] = u[k] + hz[k];
+    }
+  }
+  grad2[0] = ge;
+  grad2[1] = gs;
+  free(u);
+  free(lv);
+  free(z);
+  free(w);
+  free(hz);
+  free(f);
+  free(gw);
+  free(xs);
+  free(vs);
+  free(Es);
+  free(Ws);
+}

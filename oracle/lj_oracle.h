@@ -150,6 +150,30 @@ void orc_fp64_evn_step(int N, double* pos3, double* vel3, double* f3,
                        double L, int bc, double dt, double* Epot, double* Ekin,
                        int nthreads);
 
+/* ---------------- adjoint (FP64) ---------------- */
+
+/* One adjoint pass in FP64 at double positions x3 with vector z3 (double[3N]):
+ *   hz3 = H z (H = dF/dx),  f3 = F,  gw3 = dW/dx,
+ *   scal8 = {E, W, dE/dsigma, dW/dsigma, (dF/dsigma)^T z, F^T z, 0, 0}.
+ * Minimum image as orc_forces_fp64; self pair excluded by index. */
+void orc_adjoint_pass_fp64(int N, const double* x3, const double* z3, double L, int periodic,
+                           double eps, double sigma, double* hz3, double* f3, double* gw3,
+                           double* scal8, int nthreads);
+
+/* FP64 trajectory of M EVN velocity-Verlet steps from (x0, v0) (no wrap: the
+ * minimum image makes forces wrap-invariant) and the loss
+ *   L = (1/M) sum_{m=1..M} E(x_m) + w_p ((1/M) sum_m P_m - p_target)^2,
+ *   P_m = (W(x_m)/3 + sum |v_m|^2 / 3) / L^3.
+ * out3 = {L, mean E, mean P}. */
+void orc_traj_loss_fp64(int N, const double* x0, const double* v0, double L, int periodic,
+                        double eps, double sigma, double dt, int M, double p_target, double w_p,
+                        double* out3, int nthreads);
+/* Its gradient by the adjoint sweep over all stored states:
+ * grad2 = {dL/deps, dL/dsigma}, dx0 = dL/dx0 (double[3N], may be NULL). */
+void orc_traj_grad_fp64(int N, const double* x0, const double* v0, double L, int periodic,
+                        double eps, double sigma, double dt, int M, double p_target, double w_p,
+                        double* out3, double* grad2, double* dx0, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,15 @@ def _load_oracle():
     lib.orc_fp64_evn_step.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, C.c_int,
                                       C.c_double, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.c_int]
+    lib.orc_adjoint_pass_fp64.argtypes = [C.c_int, _f64p, _f64p, C.c_double, C.c_int, C.c_double,
+                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_int]
+    lib.orc_traj_loss_fp64.argtypes = [C.c_int, _f64p, _f64p, C.c_double, C.c_int, C.c_double,
+                                       C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                       _f64p, C.c_int]
+    lib.orc_traj_grad_fp64.argtypes = [C.c_int, _f64p, _f64p, C.c_double, C.c_int, C.c_double,
+                                       C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                       _f64p, _f64p, C.c_void_p, C.c_int]
     return lib
 
 
@@ -269,3 +278,33 @@ class RefSystem:
         f = np.zeros(4 * self.N, np.float32)
         self.lib.refh_get_state(self.h, p, v, f)
         return p, v, f
+
+
+def adjoint_pass_fp64(x3, z3, L, periodic, eps=1.0, sigma=1.0, nthreads=0):
+    """FP64 (H z, F, dW/dx, scalars{E, W, dE/ds, dW/ds, (dF/ds)^T z, F^T z})."""
+    x3 = np.ascontiguousarray(x3, np.float64).ravel()
+    z3 = np.ascontiguousarray(z3, np.float64).ravel()
+    N = x3.size // 3
+    hz, f, gw, sc = (np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N), np.zeros(8))
+    ORC.orc_adjoint_pass_fp64(N, x3, z3, L, int(periodic), eps, sigma, hz.ctypes.data,
+                              f.ctypes.data, gw.ctypes.data, sc.ctypes.data, nthreads)
+    return hz.reshape(N, 3), f.reshape(N, 3), gw.reshape(N, 3), sc
+
+
+def traj_loss_fp64(x0, v0, L, periodic, eps, sigma, dt, M, pt=1.0, wp=1.0, nthreads=0):
+    x0 = np.ascontiguousarray(x0, np.float64).ravel()
+    v0 = np.ascontiguousarray(v0, np.float64).ravel()
+    out = np.zeros(3)
+    ORC.orc_traj_loss_fp64(x0.size // 3, x0, v0, L, int(periodic), eps, sigma, dt, M, pt, wp,
+                           out, nthreads)
+    return out
+
+
+def traj_grad_fp64(x0, v0, L, periodic, eps, sigma, dt, M, pt=1.0, wp=1.0, nthreads=0):
+    x0 = np.ascontiguousarray(x0, np.float64).ravel()
+    v0 = np.ascontiguousarray(v0, np.float64).ravel()
+    N = x0.size // 3
+    out, g2, dx0 = np.zeros(3), np.zeros(2), np.zeros(3 * N)
+    ORC.orc_traj_grad_fp64(N, x0, v0, L, int(periodic), eps, sigma, dt, M, pt, wp, out, g2,
+                           dx0.ctypes.data, nthreads)
+    return out, g2, dx0.reshape(N, 3)
