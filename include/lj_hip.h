@@ -117,8 +117,12 @@ enum {
                          (float)((k*(rmax/nbins))^2) in double; r^2 >= edges2[nbins] dropped */
 };
 enum {
-  LJ_PREC_FP32 = 0,       /* FP32 pair math and per-slice sums, FP64 cross-slice sums */
-  LJ_PREC_COMPENSATED = 1 /* + TwoSum-compensated per-particle sums inside each slice */
+  LJ_PREC_FP32 = 0,        /* FP32 pair math and per-slice sums, FP64 cross-slice sums */
+  LJ_PREC_COMPENSATED = 1, /* + TwoSum-compensated per-particle sums inside each slice */
+  LJ_PREC_FP64 = 2         /* double state, double pair math, the integrator formulas of
+                              MDSystem.cpp:442-510 in double: the reference of the
+                              precision study.  The float arrays are rounded mirrors
+                              of the double state; lj_set/get_array_f64 reach it. */
 };
 enum { LJ_ARRAY_POS = 0, LJ_ARRAY_VEL = 1, LJ_ARRAY_FORCE = 2 }; /* MDSystem.cpp:586-630 */
 
@@ -166,6 +170,11 @@ int lj_sample_initial_state(lj_ctx* ctx, uint64_t seed);
 /* host float[4N] <-> device state           MDSystem.cpp:586-630 */
 int lj_set_array(lj_ctx* ctx, int type, const float* host4N);
 int lj_get_array(lj_ctx* ctx, int type, float* host4N);
+/* host double[4N] <-> device state.  In LJ_PREC_FP64 these reach the double
+ * state itself (set also refreshes the float mirror); in the FP32 modes the
+ * values are rounded to / widened from the float arrays. */
+int lj_set_array_f64(lj_ctx* ctx, int type, const double* host4N);
+int lj_get_array_f64(lj_ctx* ctx, int type, double* host4N);
 
 /* CalculateForces (MDSystem.cpp:232-311): device forces of the current
  * positions, E_pot and virial into the observables (out may be NULL). */
@@ -212,7 +221,9 @@ int lj_time_force_kernel(lj_ctx* ctx, int reps, double* us_per_launch);
 /* Forces of caller-owned positions d_pos4 (float4[N], device) into d_force4
  * (float4[N], w = per-particle sum ((s/r)^12-(s/r)^6)); d_ev (device, may be
  * NULL) receives {E_pot, W}.  Box, boundary, eps, sigma from ctx; N must equal
- * the context's N.  Enqueued on `stream` (NULL = context stream), async. */
+ * the context's N.  Enqueued on `stream` (NULL = context stream), async.
+ * Always the FP32 pair pass (the compensated one in LJ_PREC_COMPENSATED),
+ * also in an LJ_PREC_FP64 context. */
 int lj_forces_device(lj_ctx* ctx, const float* d_pos4, float* d_force4, double* d_ev,
                      void* stream);
 

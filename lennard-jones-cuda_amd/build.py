@@ -35,8 +35,8 @@ HIPFLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-SOURCES = ["lj_kernels.hip", "lj_batch.hip", "lj_adjoint.hip", "lj_host.cpp", "lj_api.cpp",
-           "lj_batch_api.cpp", "lj_adjoint_api.cpp"]
+SOURCES = ["lj_kernels.hip", "lj_batch.hip", "lj_adjoint.hip", "lj_fp64.hip", "lj_host.cpp",
+           "lj_api.cpp", "lj_batch_api.cpp", "lj_adjoint_api.cpp"]
 
 
 def _run(cmd, cwd=None):
@@ -70,6 +70,17 @@ def build_product(verbose=False):
     return target
 
 
+def build_driver():
+    """The C++ program that drives the header-only MDSystem class over the
+    C-ABI (include/lj_mdsystem.hpp); run by the GPU tests."""
+    src = os.path.join(ROOT, "tests", "cpp", "mdsystem_driver.cpp")
+    out = os.path.join(LIB, "mdsystem_driver")
+    _run(["g++", "-std=c++11", "-O2", "-Wall", "-Wextra", "-Werror", f"-I{INCLUDE}", src,
+          f"-L{LIB}", "-lljhip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_checkers():
     _run(["make", "-s", "-C", ORACLE, "oracle"])
     if os.path.isdir(os.path.join(REFERENCE, "src", "library")):
@@ -78,6 +89,7 @@ def build_checkers():
 
 def build_all(product_only=False, verbose=False):
     t = build_product(verbose=verbose)
+    build_driver()
     if not product_only:
         build_checkers()
     return t

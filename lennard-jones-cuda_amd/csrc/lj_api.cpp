@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "lj_ctx.h"
+#include "lj_fp64.h"
 
 using namespace lj;
 
@@ -85,6 +86,57 @@ void refresh_scalars(lj_ctx* c)
                             false);
 }
 
+Step64 step64_args(lj_ctx* c, double dt)
+{
+  Step64 a{};
+  a.x = c->xd;
+  a.v = c->vd;
+  a.f = c->fd;
+  a.tf = c->tfd;
+  a.pos = c->pos;
+  a.vel = c->vel;
+  a.force = c->force;
+  a.part = c->part64;
+  a.apart = c->apart64;
+  a.blk = c->blk;
+  a.N = c->cfg.N;
+  a.S = c->plan.S;
+  a.Npad = c->plan.Npad;
+  a.bc = c->cfg.boundary;
+  a.dt = dt;
+  a.L = c->L;
+  a.T0 = c->cfg.T0;
+  a.FS = 24. * c->cfg.eps / (c->cfg.sigma * c->cfg.sigma);
+  return a;
+}
+
+double4* state64(lj_ctx* c, int type)
+{
+  return type == LJ_ARRAY_POS ? c->xd : type == LJ_ARRAY_VEL ? c->vd : c->fd;
+}
+
+hipError_t enqueue_force64(lj_ctx* c, hipStream_t st)
+{
+  return launch_force64(st, c->plan, c->xd, c->cfg.boundary == LJ_BC_PERIODIC, c->L,
+                        c->cfg.sigma, c->part64, c->apart64);
+}
+
+// the FP64 step: MDSystem.cpp:442-510 in double
+int enqueue_step64(lj_ctx* c, hipStream_t st, double dt)
+{
+  const Step64 a = step64_args(c, dt);
+  LJ_HIP(launch_drift64(st, a, c->cfg.canonical));
+  LJ_HIP(enqueue_force64(c, st));
+  if (c->cfg.canonical == LJ_EVN) {
+    LJ_HIP(launch_kick64(st, a));
+  } else {
+    LJ_HIP(launch_tvn64_a(st, a));
+    LJ_HIP(launch_tvn64_b(st, a));
+  }
+  LJ_HIP(launch_finalize_step(st, step_args(c, dt)));
+  return LJ_OK;
+}
+
 // force pass into fpart/apart from the context positions (or external d_pos)
 int enqueue_force(lj_ctx* c, hipStream_t st, const float4* d_pos)
 {
@@ -96,6 +148,7 @@ int enqueue_force(lj_ctx* c, hipStream_t st, const float4* d_pos)
 
 int enqueue_step(lj_ctx* c, hipStream_t st, double dt)
 {
+  if (is_fp64(c)) return enqueue_step64(c, st, dt);
   const StepArgs a = step_args(c, dt);
   LJ_HIP(launch_drift(st, a));
   // the drift kernel already wrote the fixed-point copy for periodic boxes
@@ -153,8 +206,13 @@ int ensure_graph(lj_ctx* c, double dt)
 int compute_forces_internal(lj_ctx* c)
 {
   hipStream_t st = c->stream;
-  LJ_TRY(enqueue_force(c, st, c->pos));
-  LJ_HIP(launch_reduce_forces(st, c->plan, c->ps, c->fpart, c->apart, c->force, c->blk));
+  if (is_fp64(c)) {
+    LJ_HIP(enqueue_force64(c, st));
+    LJ_HIP(launch_reduce64(st, step64_args(c, 0.)));
+  } else {
+    LJ_TRY(enqueue_force(c, st, c->pos));
+    LJ_HIP(launch_reduce_forces(st, c->plan, c->ps, c->fpart, c->apart, c->force, c->blk));
+  }
   LJ_HIP(launch_finalize_forces(st, c->blk, c->nblk, c->cfg.eps, c->ev, nullptr));
   // V and W into the observables block (CalculateForces sets V and P, MDSystem.cpp:257-309)
   LJ_HIP(launch_store_vw(st, c->ev, c->obs));
@@ -217,6 +275,12 @@ int lj_destroy(lj_ctx* c)
   dev_free(c->ev);
   dev_free(c->rdf);
   dev_free(c->edges2);
+  dev_free(c->xd);
+  dev_free(c->vd);
+  dev_free(c->fd);
+  dev_free(c->tfd);
+  dev_free(c->part64);
+  dev_free(c->apart64);
   if (c->e0) (void)hipEventDestroy(c->e0);
   if (c->e1) (void)hipEventDestroy(c->e1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -235,7 +299,8 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
   if (cfg->boundary < 0 || cfg->boundary > 2) return fail(LJ_ERR_ARG, "lj_create: boundary must be 0, 1 or 2");
   if (cfg->canonical != 0 && cfg->canonical != 1) return fail(LJ_ERR_ARG, "lj_create: canonical must be 0 or 1");
   if (!(cfg->eps > 0.) || !(cfg->sigma > 0.)) return fail(LJ_ERR_ARG, "lj_create: eps and sigma must be positive");
-  if (cfg->precision != LJ_PREC_FP32 && cfg->precision != LJ_PREC_COMPENSATED)
+  if (cfg->precision != LJ_PREC_FP32 && cfg->precision != LJ_PREC_COMPENSATED &&
+      cfg->precision != LJ_PREC_FP64)
     return fail(LJ_ERR_ARG, "lj_create: unknown precision mode %d", cfg->precision);
   LJ_TRY(check_device());
   int ndev = 0;
@@ -267,6 +332,14 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
   A(dev_alloc(&c->obs, 1));
   A(dev_alloc(&c->hist_base, 1));
   A(dev_alloc(&c->ev, 2));
+  if (is_fp64(c)) {
+    A(dev_alloc(&c->xd, N));
+    A(dev_alloc(&c->vd, N));
+    A(dev_alloc(&c->fd, N));
+    A(dev_alloc(&c->tfd, N));
+    A(dev_alloc(&c->part64, (size_t)c->plan.S * c->plan.Npad));
+    A(dev_alloc(&c->apart64, (size_t)c->plan.S * c->plan.Npad));
+  }
   if (rc == LJ_OK && hipEventCreate(&c->e0) != hipSuccess) A(fail(LJ_ERR_DEVICE, "hipEventCreate"));
   if (rc == LJ_OK && hipEventCreate(&c->e1) != hipSuccess) A(fail(LJ_ERR_DEVICE, "hipEventCreate"));
   if (rc == LJ_OK) A(ensure_hist(c, 1024));
@@ -281,6 +354,15 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
         hipMemcpy(c->tF, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->obs, &o, sizeof(o), hipMemcpyHostToDevice) != hipSuccess)
       A(fail(LJ_ERR_DEVICE, "initial upload failed"));
+    if (rc == LJ_OK && is_fp64(c)) {
+      std::vector<double> zd((size_t)4 * N, 0.);
+      const size_t db = sizeof(double) * 4 * (size_t)N;
+      if (hipMemcpy(c->xd, zd.data(), db, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(c->vd, zd.data(), db, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(c->fd, zd.data(), db, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(c->tfd, zd.data(), db, hipMemcpyHostToDevice) != hipSuccess)
+        A(fail(LJ_ERR_DEVICE, "initial upload failed"));
+    }
   }
   if (rc != LJ_OK) {
     std::string msg = g_err;
@@ -299,7 +381,46 @@ int lj_set_array(lj_ctx* c, int type, const float* host)
   float4* d = type == LJ_ARRAY_POS ? c->pos : type == LJ_ARRAY_VEL ? c->vel : type == LJ_ARRAY_FORCE ? c->force : nullptr;
   if (!d) return fail(LJ_ERR_ARG, "lj_set_array: unknown array type %d", type);
   LJ_HIP(hipMemcpyAsync(d, host, sizeof(float4) * c->cfg.N, hipMemcpyHostToDevice, c->stream));
+  if (is_fp64(c)) LJ_HIP(launch_widen(c->stream, d, state64(c, type), c->cfg.N));
   LJ_HIP(hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+int lj_set_array_f64(lj_ctx* c, int type, const double* host)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!host) return fail(LJ_ERR_ARG, "lj_set_array_f64: null host pointer");
+  float4* d = type == LJ_ARRAY_POS ? c->pos : type == LJ_ARRAY_VEL ? c->vel : type == LJ_ARRAY_FORCE ? c->force : nullptr;
+  if (!d) return fail(LJ_ERR_ARG, "lj_set_array_f64: unknown array type %d", type);
+  const int N = c->cfg.N;
+  if (is_fp64(c)) {
+    double4* dd = state64(c, type);
+    LJ_HIP(hipMemcpyAsync(dd, host, sizeof(double4) * N, hipMemcpyHostToDevice, c->stream));
+    LJ_HIP(launch_narrow(c->stream, dd, d, N));
+    LJ_HIP(hipStreamSynchronize(c->stream));
+    return LJ_OK;
+  }
+  std::vector<float> f((size_t)4 * N);
+  for (size_t k = 0; k < f.size(); ++k) f[k] = (float)host[k];
+  return lj_set_array(c, type, f.data());
+}
+
+int lj_get_array_f64(lj_ctx* c, int type, double* host)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!host) return fail(LJ_ERR_ARG, "lj_get_array_f64: null host pointer");
+  if (type != LJ_ARRAY_POS && type != LJ_ARRAY_VEL && type != LJ_ARRAY_FORCE)
+    return fail(LJ_ERR_ARG, "lj_get_array_f64: unknown array type %d", type);
+  const int N = c->cfg.N;
+  if (is_fp64(c)) {
+    LJ_HIP(hipMemcpyAsync(host, state64(c, type), sizeof(double4) * N, hipMemcpyDeviceToHost,
+                          c->stream));
+    LJ_HIP(hipStreamSynchronize(c->stream));
+    return LJ_OK;
+  }
+  std::vector<float> f((size_t)4 * N);
+  LJ_TRY(lj_get_array(c, type, f.data()));
+  for (size_t k = 0; k < f.size(); ++k) host[k] = f[k];
   return LJ_OK;
 }
 
@@ -336,7 +457,10 @@ int lj_compute_parameters(lj_ctx* c, lj_obs* out)
 {
   LJ_TRY(ctx_setdev(c));
   const StepArgs a = step_args(c, 0.);
-  LJ_HIP(launch_kinetic_partials(c->stream, c->vel, c->cfg.N, c->blk));
+  if (is_fp64(c))
+    LJ_HIP(launch_kinetic64(c->stream, c->vd, c->cfg.N, c->blk));
+  else
+    LJ_HIP(launch_kinetic_partials(c->stream, c->vel, c->cfg.N, c->blk));
   LJ_HIP(launch_params(c->stream, a, 0));
   if (out) return lj_get_observables(c, out);
   LJ_HIP(hipStreamSynchronize(c->stream));
@@ -460,11 +584,17 @@ int lj_renormalize_velocities(lj_ctx* c, int recompute_tkin)
     LJ_TRY(lj_get_observables(c, &o));
     if (!(o.T > 0.)) return fail(LJ_ERR_STATE, "lj_renormalize_velocities: T is not positive");
     factor = std::sqrt(c->cfg.T0 / o.T);
+  } else if (is_fp64(c)) {
+    LJ_HIP(launch_kinetic64(c->stream, c->vd, c->cfg.N, c->blk));
   } else {
     LJ_HIP(launch_kinetic_partials(c->stream, c->vel, c->cfg.N, c->blk));
   }
-  LJ_HIP(launch_scale_velocities(c->stream, c->vel, c->cfg.N, c->blk, c->nblk, c->cfg.T0, factor,
-                                 c->obs));
+  if (is_fp64(c))
+    LJ_HIP(launch_scale64(c->stream, c->vd, c->vel, c->cfg.N, c->blk, c->nblk, c->cfg.T0, factor,
+                          c->obs));
+  else
+    LJ_HIP(launch_scale_velocities(c->stream, c->vel, c->cfg.N, c->blk, c->nblk, c->cfg.T0,
+                                   factor, c->obs));
   LJ_HIP(hipStreamSynchronize(c->stream));
   return LJ_OK;
 }
@@ -478,8 +608,12 @@ int lj_renormalize_to_energy(lj_ctx* c, double ust)
   if (!(Kdes > 0.) || !(o.K > 0.))
     return fail(LJ_ERR_ARG, "lj_renormalize_to_energy: target kinetic energy %g not reachable", Kdes);
   const double factor = std::sqrt(Kdes / o.K);
-  LJ_HIP(launch_scale_velocities(c->stream, c->vel, c->cfg.N, c->blk, c->nblk, c->cfg.T0, factor,
-                                 nullptr));
+  if (is_fp64(c))
+    LJ_HIP(launch_scale64(c->stream, c->vd, c->vel, c->cfg.N, c->blk, c->nblk, c->cfg.T0, factor,
+                          nullptr));
+  else
+    LJ_HIP(launch_scale_velocities(c->stream, c->vel, c->cfg.N, c->blk, c->nblk, c->cfg.T0,
+                                   factor, nullptr));
   o.K = Kdes;
   o.U = o.K + o.V;
   LJ_HIP(hipMemcpyAsync(c->obs, &o, sizeof(o), hipMemcpyHostToDevice, c->stream));
@@ -490,7 +624,10 @@ int lj_renormalize_to_energy(lj_ctx* c, double ust)
 int lj_apply_boundary_conditions(lj_ctx* c)
 {
   LJ_TRY(ctx_setdev(c));
-  LJ_HIP(launch_apply_bc(c->stream, c->pos, c->vel, c->cfg.N, c->cfg.boundary, c->L));
+  if (is_fp64(c))
+    LJ_HIP(launch_bc64(c->stream, c->xd, c->vd, c->pos, c->vel, c->cfg.N, c->cfg.boundary, c->L));
+  else
+    LJ_HIP(launch_apply_bc(c->stream, c->pos, c->vel, c->cfg.N, c->cfg.boundary, c->L));
   LJ_HIP(hipStreamSynchronize(c->stream));
   return LJ_OK;
 }
@@ -528,6 +665,17 @@ int lj_time_force_kernel(lj_ctx* c, int reps, double* us)
   LJ_TRY(ctx_setdev(c));
   if (reps <= 0 || !us) return fail(LJ_ERR_ARG, "lj_time_force_kernel: bad arguments");
   hipStream_t st = c->stream;
+  if (is_fp64(c)) {
+    LJ_HIP(enqueue_force64(c, st));  // warm
+    LJ_HIP(hipEventRecord(c->e0, st));
+    for (int k = 0; k < reps; ++k) LJ_HIP(enqueue_force64(c, st));
+    LJ_HIP(hipEventRecord(c->e1, st));
+    LJ_HIP(hipEventSynchronize(c->e1));
+    float ms = 0.f;
+    LJ_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+    *us = 1000.0 * ms / reps;
+    return LJ_OK;
+  }
   if (c->ps.periodic) LJ_HIP(launch_prep_q(st, c->pos, c->q, c->cfg.N, c->ps.qscale));
   const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
   LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));  // warm

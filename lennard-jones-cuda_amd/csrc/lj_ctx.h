@@ -40,6 +40,13 @@ struct lj_ctx {
   int g_canon = -1, g_bc = -1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   AdjState* adj = nullptr;
+  // LJ_PREC_FP64: double state (authoritative; pos/vel/force are its mirrors)
+  double4* xd = nullptr;
+  double4* vd = nullptr;
+  double4* fd = nullptr;
+  double4* tfd = nullptr;
+  double4* part64 = nullptr;
+  double* apart64 = nullptr;
 };
 
 namespace lj {
@@ -52,4 +59,5 @@ void refresh_scalars(lj_ctx* c);
 void drop_graph(lj_ctx* c);
 // releases the adjoint scratch of a context
 void adj_free(lj_ctx* c);
+inline bool is_fp64(const lj_ctx* c) { return c->cfg.precision == LJ_PREC_FP64; }
 }  // namespace lj

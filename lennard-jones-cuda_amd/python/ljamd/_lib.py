@@ -14,7 +14,7 @@ LJ_ERR_ARG, LJ_ERR_DEVICE, LJ_ERR_ALLOC, LJ_ERR_STATE, LJ_ERR_UNSUPPORTED = -1, 
 BC_PERIODIC, BC_WALL, BC_NONE = 0, 1, 2
 EVN, TVN = 0, 1
 RDF_REF_CPU, RDF_REF_GPU, RDF_RBINS = 0, 1, 2
-PREC_FP32, PREC_COMPENSATED = 0, 1
+PREC_FP32, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2
 ARRAY_POS, ARRAY_VEL, ARRAY_FORCE = 0, 1, 2
 
 
@@ -80,6 +80,8 @@ SIGNATURES = [
     ("lj_sample_initial_state", _i, [_vp, C.c_uint64]),
     ("lj_set_array", _i, [_vp, _i, _vp]),
     ("lj_get_array", _i, [_vp, _i, _vp]),
+    ("lj_set_array_f64", _i, [_vp, _i, _vp]),
+    ("lj_get_array_f64", _i, [_vp, _i, _vp]),
     ("lj_compute_forces", _i, [_vp, C.POINTER(LJObs)]),
     ("lj_compute_parameters", _i, [_vp, C.POINTER(LJObs)]),
     ("lj_get_observables", _i, [_vp, C.POINTER(LJObs)]),

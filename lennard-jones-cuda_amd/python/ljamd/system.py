@@ -67,6 +67,17 @@ class LJSystem:
         check(lib().lj_get_array(self.h, int(kind), ptr(out)))
         return out
 
+    def set_array_f64(self, kind, a4):
+        a4 = np.ascontiguousarray(a4, np.float64).reshape(-1)
+        if a4.size != 4 * self.N:
+            raise ValueError("expected float64[4N]")
+        check(lib().lj_set_array_f64(self.h, int(kind), ptr(a4)))
+
+    def get_array_f64(self, kind):
+        out = np.empty(4 * self.N, np.float64)
+        check(lib().lj_get_array_f64(self.h, int(kind), ptr(out)))
+        return out
+
     def set_state(self, pos4, vel4=None, force4=None):
         self.set_array(_L.ARRAY_POS, pos4)
         if vel4 is not None:

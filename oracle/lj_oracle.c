@@ -515,33 +515,52 @@ void orc_sys_integrate(orc_sys* s, double dt) /* :438-510, :578-582 */
 
 /* ---------------- FP64 step for the precision study ---------------- */
 
-void orc_fp64_evn_step(int N, double* x, double* v, double* f, double L, int bc,
-                       double dt, double* Epot, double* Ekin, int nthreads)
+/* Integrate(dt) of MDSystem.cpp:442-510 with every quantity in double:
+ * EVN :442-464, TVN :465-510 (KineticTemperature :361-373 over the
+ * half-step velocities), then the boundary conditions :406-436 applied
+ * after the step as in :578. */
+void orc_fp64_step(int N, double* x, double* v, double* f, double L, int bc, int canonical,
+                   double T0, double eps, double sigma, double dt, double* obs3, int nthreads)
 {
-  for (int i = 0; i < 3 * N; ++i) {
-    x[i] += dt * v[i] + 0.5 * dt * dt * f[i];
-    v[i] += 0.5 * dt * f[i];
+  double* tf = NULL;
+  for (int i = 0; i < 3 * N; ++i) x[i] += dt * v[i] + dt * dt * f[i] / 2.;
+  if (!canonical) {
+    for (int i = 0; i < 3 * N; ++i) v[i] += dt * f[i] / 2.;
+  } else {
+    tf = (double*)malloc(sizeof(double) * 3 * (size_t)N);
+    for (int i = 0; i < 3 * N; ++i) tf[i] = 0.5 * f[i];
   }
-  double E = 0.;
-  orc_forces_fp64d(N, x, L, bc == 0, 1., 1., f, NULL, &E, NULL, nthreads);
-  double K = 0.;
-  for (int i = 0; i < 3 * N; ++i) {
-    v[i] += 0.5 * dt * f[i];
+  double E = 0., W = 0.;
+  orc_forces_fp64d(N, x, L, bc == 0, eps, sigma, f, NULL, &E, &W, nthreads);
+  if (!canonical) {
+    for (int i = 0; i < 3 * N; ++i) v[i] += dt * f[i] / 2.;
+  } else {
+    double tv2 = 0.;
+    for (int i = 0; i < 3 * N; ++i) {
+      tf[i] += 0.5 * f[i];
+      const double tv = v[i] + dt * tf[i] / 2.;
+      tv2 += tv * tv;
+    }
+    const double chi = sqrt(T0 / (tv2 / 3. / N));
+    for (int i = 0; i < 3 * N; ++i) v[i] = (2. * chi - 1.) * v[i] + chi * dt * tf[i];
+    free(tf);
   }
   if (bc == 0) {
     for (int i = 0; i < 3 * N; ++i) {
       if (x[i] < 0.) x[i] += L;
       if (x[i] > L) x[i] -= L;
     }
-  } else if (bc != 2) {
+  } else if (bc == 1) {
     for (int i = 0; i < 3 * N; ++i) {
       if (x[i] < 0. && v[i] < 0.) v[i] = -v[i];
       if (x[i] > L && v[i] > 0.) v[i] = -v[i];
     }
   }
-  for (int i = 0; i < 3 * N; ++i) K += 0.5 * v[i] * v[i];
-  if (Epot) *Epot = E;
-  if (Ekin) *Ekin = K;
+  double K = 0.;
+  for (int i = 0; i < 3 * N; ++i) K += v[i] * v[i];
+  obs3[0] = E;
+  obs3[1] = K / 2.;
+  obs3[2] = W;
 }
 
 /* ---------------- adjoint (FP64) ----------------

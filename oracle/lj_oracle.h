@@ -142,13 +142,13 @@ void orc_sys_apply_bc(orc_sys* s);
 /* Integrate(dt): EVN or TVN, BCs, params, t += dt   MDSystem.cpp:438-510,578-582 */
 void orc_sys_integrate(orc_sys* s, double dt);
 
-/* FP64 velocity-Verlet (EVN) reference on double state, for the precision
- * study: pos3/vel3/f3 double[3N]; forces from orc_forces_fp64d; hard wall
- * reflection as MDSystem.cpp:425-435 or periodic wrap :414-424.
- * Returns E_pot and E_kin after the step in *Epot, *Ekin. */
-void orc_fp64_evn_step(int N, double* pos3, double* vel3, double* f3,
-                       double L, int bc, double dt, double* Epot, double* Ekin,
-                       int nthreads);
+/* One Integrate(dt) step (MDSystem.cpp:442-510, EVN or TVN, then the
+ * boundary conditions) with every quantity in double, for the precision
+ * study: x/v/f double[3N] (f holds the forces of x on entry and on exit),
+ * forces from orc_forces_fp64d with eps, sigma. obs3 = {E_pot, E_kin, W}
+ * after the step. */
+void orc_fp64_step(int N, double* x, double* v, double* f, double L, int bc, int canonical,
+                   double T0, double eps, double sigma, double dt, double* obs3, int nthreads);
 
 /* ---------------- adjoint (FP64) ---------------- */
 

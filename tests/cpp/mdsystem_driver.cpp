@@ -1,0 +1,57 @@
+// mdsystem_driver.cpp -- drives ljhip::MDSystem (include/lj_mdsystem.hpp) the
+// way the reference's task programs drive their MDSystem (run-isotherm.cpp:
+// configure, Integrate in a loop, read the public fields) and prints what a
+// test compares against the oracle:
+//   init U T K V P
+//   step k t U T K V P      (one line per step)
+//   rdf <256 counts>
+//   pos <4N floats>          (h_Pos after the last step)
+//   cpu_mode_rejected 1
+// usage: mdsystem_driver N rho T0 canonical boundary dt steps seed
+#include <cstdio>
+#include <cstdlib>
+#include <exception>
+
+#include "lj_mdsystem.hpp"
+
+int main(int argc, char** argv)
+{
+  if (argc != 9) {
+    std::fprintf(stderr, "usage: %s N rho T0 canonical boundary dt steps seed\n", argv[0]);
+    return 2;
+  }
+  ljhip::MDSystem::MDSystemConfiguration cfg;
+  cfg.N = std::atoi(argv[1]);
+  cfg.rho = std::atof(argv[2]);
+  cfg.T0 = std::atof(argv[3]);
+  cfg.canonical = std::atoi(argv[4]) != 0;
+  cfg.boundaryConditions = std::atoi(argv[5]);
+  const double dt = std::atof(argv[6]);
+  const int steps = std::atoi(argv[7]);
+  cfg.seed = std::strtoull(argv[8], nullptr, 10);
+  try {
+    ljhip::MDSystem s(cfg);
+    std::printf("init %.17g %.17g %.17g %.17g %.17g\n", s.U, s.T, s.K, s.V, s.P);
+    for (int k = 0; k < steps; ++k) {
+      s.Integrate(dt);
+      std::printf("step %d %.17g %.17g %.17g %.17g %.17g %.17g\n", k, s.t, s.U, s.T, s.K, s.V, s.P);
+    }
+    s.CalculateForces();
+    std::printf("rdf");
+    for (int c : s.NdNdr2) std::printf(" %d", c);
+    std::printf("\npos");
+    for (int i = 0; i < 4 * cfg.N; ++i) std::printf(" %.9g", s.h_Pos[i]);
+    std::printf("\n");
+    int rejected = 0;
+    try {
+      s.setHardwareMode(false);
+    } catch (const std::runtime_error&) {
+      rejected = 1;
+    }
+    std::printf("cpu_mode_rejected %d\n", rejected);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -70,9 +70,8 @@ def _load_oracle():
     for n in ("orc_sys_forces", "orc_sys_params", "orc_sys_apply_bc"):
         getattr(lib, n).argtypes = [C.POINTER(OrcSys)]
     lib.orc_sys_integrate.argtypes = [C.POINTER(OrcSys), C.c_double]
-    lib.orc_fp64_evn_step.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, C.c_int,
-                                      C.c_double, C.POINTER(C.c_double),
-                                      C.POINTER(C.c_double), C.c_int]
+    lib.orc_fp64_step.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, C.c_int, C.c_int,
+                                  C.c_double, C.c_double, C.c_double, C.c_double, _f64p, C.c_int]
     lib.orc_adjoint_pass_fp64.argtypes = [C.c_int, _f64p, _f64p, C.c_double, C.c_int, C.c_double,
                                           C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_int]
@@ -308,3 +307,19 @@ def traj_grad_fp64(x0, v0, L, periodic, eps, sigma, dt, M, pt=1.0, wp=1.0, nthre
     ORC.orc_traj_grad_fp64(N, x0, v0, L, int(periodic), eps, sigma, dt, M, pt, wp, out, g2,
                            dx0.ctypes.data, nthreads)
     return out, g2, dx0.reshape(N, 3)
+
+
+def fp64_steps(x3, v3, L, bc, canonical, T0, dt, nsteps, eps=1.0, sigma=1.0, nthreads=0):
+    """nsteps FP64 Integrate(dt) steps from double state (x3, v3) [N,3].
+    Returns (x, v, f, obs[nsteps, 3] = (E_pot, E_kin, W) after each step)."""
+    x = np.ascontiguousarray(x3, np.float64).copy()
+    v = np.ascontiguousarray(v3, np.float64).copy()
+    N = x.shape[0]
+    f = forces_fp64d(x, L, bc == 0, eps, sigma, nthreads)[0].copy()
+    obs = np.zeros((nsteps, 3))
+    o = np.zeros(3)
+    for k in range(nsteps):
+        ORC.orc_fp64_step(N, x.ravel(), v.ravel(), f.ravel(), L, int(bc), int(canonical), T0, eps,
+                          sigma, dt, o, nthreads)
+        obs[k] = o
+    return x, v, f, obs

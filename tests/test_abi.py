@@ -122,3 +122,14 @@ def test_compat_null_arguments_reported():
     ljamd.lib().copyArrayToDevice(None, None, 4)
     assert b"copyArrayToDevice" in ljamd.lib().lj_last_error()
     ljamd.lib().registerGLBufferObject(3)  # no-op as in the reference
+
+
+def test_mdsystem_class_header_compiles():
+    """The C++ class over the C-ABI builds with the reference's C++ standard
+    (C++11) and with C++17, warnings as errors."""
+    src = os.path.join(ROOT, "tests", "cpp", "mdsystem_driver.cpp")
+    for std in ("c++11", "c++17"):
+        r = subprocess.run(["g++", f"-std={std}", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                            "-I" + os.path.join(ROOT, "include"), src],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        assert r.returncode == 0, r.stdout

@@ -1,0 +1,61 @@
+"""The header-only C++ MDSystem class (include/lj_mdsystem.hpp) driven by a
+C++ program the way the reference's task programs drive MDSystem
+(configure, Integrate in a loop, read the public fields), against the
+reference CPU path restated in the oracle from the same seeded state:
+per-step U within |dU|/N <= 2e-5 over 200 EVN steps (N = 400, rho* = 0.05,
+T* = 1.4, periodic, dt = 0.005) and NdNdr2 bit-exact with the reference GPU
+binning on the final positions."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleSystem, lattice_state, rdf
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "lennard-jones-cuda_amd", "lib", "mdsystem_driver")
+
+
+def _run(*args):
+    if not os.path.exists(DRIVER):
+        raise RuntimeError(f"{DRIVER} is not built (run __graft_entry__.build())")
+    r = subprocess.run([DRIVER, *map(str, args)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = {"step": []}
+    for line in r.stdout.splitlines():
+        key, *vals = line.split()
+        if key == "step":
+            out["step"].append([float(v) for v in vals])
+        else:
+            out[key] = [float(v) for v in vals]
+    return out
+
+
+def test_mdsystem_class_vs_reference_cpu():
+    N, rho, T0, dt, steps, seed = 400, 0.05, 1.4, 0.005, 200, 3
+    print(f"\n[case] C++ MDSystem N={N} EVN periodic {steps} steps", flush=True)
+    out = _run(N, rho, T0, 0, 0, dt, steps, seed)
+    L, pos, vel = lattice_state(N, rho, T0, seed)
+    o = OracleSystem(N, T0, rho, 0, 0, nthreads=8)
+    o.set_state(pos, vel)
+    o.forces()
+    o.params()
+    U0, T0g, K0, V0, P0 = out["init"]
+    assert abs(U0 - o.s.U) <= 1e-6 * abs(o.s.U) and abs(V0 - o.s.V) <= 1e-5 * abs(o.s.V)
+    ho = o.integrate(dt, steps)
+    st = np.array(out["step"])
+    assert np.array_equal(st[:, 0], np.arange(steps))
+    assert np.allclose(st[:, 1], dt * (np.arange(steps) + 1), rtol=0, atol=1e-12)
+    dU = np.abs(st[:, 2] - ho[:, 0]) / N
+    print(f"  max |dU|/N {dU.max():.2e}  final U {st[-1, 2]:.9g} vs {ho[-1, 0]:.9g}")
+    assert dU.max() <= 2e-5
+    assert np.max(np.abs(st[:, 3] - ho[:, 1])) <= 2e-4
+    # NdNdr2 of the final positions: the reference GPU rule (MDSystem.cu:39), bit-exact
+    p = np.array(out["pos"], np.float32)
+    counts = rdf(p, L, True, 1, 256, dr2=o.s.rdf_dr2)
+    assert np.array_equal(np.array(out["rdf"], np.int64), counts)
+    assert out["cpu_mode_rejected"] == [1.0]
