@@ -113,8 +113,8 @@ int dforces(lj_ctx* c, hipStream_t st, const double4* x)
 {
   AdjState* a = c->adj;
   LJ_HIP(launch_dprep(st, x, c->cfg.N, qscale_of(c), c->q, a->xf));
-  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, a->xf, c->fpart, c->apart, false, true));
-  LJ_HIP(launch_dkick(st, c->plan, c->ps.FS, 0., c->fpart, c->apart, a->fd, a->vd, a->blk));
+  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, a->xf, c->fpart, c->apart, false, true));
+  LJ_HIP(launch_dkick(st, fplan(c), c->ps.FS, 0., c->fpart, c->apart, a->fd, a->vd, a->blk));
   return LJ_OK;
 }
 
@@ -124,8 +124,8 @@ int enqueue_dstep(lj_ctx* c, hipStream_t st, double dt)
   AdjState* a = c->adj;
   const double vol = std::pow(c->L, 3.);
   LJ_HIP(launch_ddrift(st, a->xd, a->vd, a->fd, c->cfg.N, dt, qscale_of(c), c->q, a->xf));
-  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, a->xf, c->fpart, c->apart, false, false));
-  LJ_HIP(launch_dkick(st, c->plan, c->ps.FS, dt, c->fpart, c->apart, a->fd, a->vd, a->blk));
+  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, a->xf, c->fpart, c->apart, false, false));
+  LJ_HIP(launch_dkick(st, fplan(c), c->ps.FS, dt, c->fpart, c->apart, a->fd, a->vd, a->blk));
   LJ_HIP(launch_dfinal(st, a->blk, a->nblk, c->cfg.eps, vol, 1, a->acc));
   return LJ_OK;
 }

@@ -75,8 +75,8 @@ StepArgs step_args(lj_ctx* c, double dt)
   a.FS = c->ps.FS;
   a.bc = c->cfg.boundary;
   a.canonical = c->cfg.canonical;
-  a.S = c->plan.S;
-  a.Npad = c->plan.Npad;
+  a.S = fplan(c).S;
+  a.Npad = fplan(c).Npad;
   return a;
 }
 
@@ -141,7 +141,7 @@ int enqueue_step64(lj_ctx* c, hipStream_t st, double dt)
 int enqueue_force(lj_ctx* c, hipStream_t st, const float4* d_pos)
 {
   if (c->ps.periodic) LJ_HIP(launch_prep_q(st, d_pos, c->q, c->cfg.N, c->ps.qscale));
-  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, d_pos, c->fpart, c->apart,
+  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, d_pos, c->fpart, c->apart,
                       c->cfg.precision == LJ_PREC_COMPENSATED, true));
   return LJ_OK;
 }
@@ -152,7 +152,7 @@ int enqueue_step(lj_ctx* c, hipStream_t st, double dt)
   const StepArgs a = step_args(c, dt);
   LJ_HIP(launch_drift(st, a));
   // the drift kernel already wrote the fixed-point copy for periodic boxes
-  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart,
+  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart,
                       c->cfg.precision == LJ_PREC_COMPENSATED, false));
   if (c->cfg.canonical == LJ_EVN) {
     LJ_HIP(launch_kick_evn(st, a));
@@ -211,7 +211,7 @@ int compute_forces_internal(lj_ctx* c)
     LJ_HIP(launch_reduce64(st, step64_args(c, 0.)));
   } else {
     LJ_TRY(enqueue_force(c, st, c->pos));
-    LJ_HIP(launch_reduce_forces(st, c->plan, c->ps, c->fpart, c->apart, c->force, c->blk));
+    LJ_HIP(launch_reduce_forces(st, fplan(c), c->ps, c->fpart, c->apart, c->force, c->blk));
   }
   LJ_HIP(launch_finalize_forces(st, c->blk, c->nblk, c->cfg.eps, c->ev, nullptr));
   // V and W into the observables block (CalculateForces sets V and P, MDSystem.cpp:257-309)
@@ -275,6 +275,7 @@ int lj_destroy(lj_ctx* c)
   dev_free(c->ev);
   dev_free(c->rdf);
   dev_free(c->edges2);
+  dev_free(c->n3items);
   dev_free(c->xd);
   dev_free(c->vd);
   dev_free(c->fd);
@@ -313,6 +314,11 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
   c->L = box_length(cfg->N, cfg->rho);
   refresh_scalars(c);
   c->plan = make_force_plan(cfg->N, cfg->jslices, 1);
+  // the pair-once kernel serves the FP32 mode at sizes that fill the GPU; an
+  // explicit j-slice count keeps the sliced kernel
+  if (cfg->precision == LJ_PREC_FP32 && cfg->jslices == 0 && n3_eligible(cfg->N))
+    c->plan3 = make_n3_plan(cfg->N);
+  const size_t npart = (size_t)std::max(c->plan.S, c->plan3.S) * c->plan.Npad;
   const int N = cfg->N;
   c->nblk = (N + kBlock - 1) / kBlock;
   int rc = LJ_OK;
@@ -326,8 +332,17 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
   A(dev_alloc(&c->force, N));
   A(dev_alloc(&c->tF, N));
   A(dev_alloc(&c->q, N));
-  A(dev_alloc(&c->fpart, (size_t)c->plan.S * c->plan.Npad));
-  A(dev_alloc(&c->apart, (size_t)c->plan.S * c->plan.Npad));
+  A(dev_alloc(&c->fpart, npart));
+  A(dev_alloc(&c->apart, npart));
+  if (c->plan3.n3) {
+    std::vector<int2> it((size_t)c->plan3.nitems);
+    n3_items(c->plan3.ng, it.data());
+    A(dev_alloc(&c->n3items, it.size()));
+    if (rc == LJ_OK &&
+        hipMemcpy(c->n3items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice) != hipSuccess)
+      A(fail(LJ_ERR_DEVICE, "item table upload failed"));
+    c->plan3.items = c->n3items;
+  }
   A(dev_alloc(&c->blk, (size_t)c->nblk * kBlkFields));
   A(dev_alloc(&c->obs, 1));
   A(dev_alloc(&c->hist_base, 1));
@@ -678,10 +693,10 @@ int lj_time_force_kernel(lj_ctx* c, int reps, double* us)
   }
   if (c->ps.periodic) LJ_HIP(launch_prep_q(st, c->pos, c->q, c->cfg.N, c->ps.qscale));
   const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
-  LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));  // warm
+  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));  // warm
   LJ_HIP(hipEventRecord(c->e0, st));
   for (int k = 0; k < reps; ++k)
-    LJ_HIP(launch_force(st, c->plan, c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));
+    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));
   LJ_HIP(hipEventRecord(c->e1, st));
   LJ_HIP(hipEventSynchronize(c->e1));
   float ms = 0.f;
@@ -697,7 +712,7 @@ int lj_forces_device(lj_ctx* c, const float* d_pos4, float* d_force4, double* d_
   hipStream_t st = stream ? (hipStream_t)stream : c->stream;
   const float4* p = reinterpret_cast<const float4*>(d_pos4);
   LJ_TRY(enqueue_force(c, st, p));
-  LJ_HIP(launch_reduce_forces(st, c->plan, c->ps, c->fpart, c->apart,
+  LJ_HIP(launch_reduce_forces(st, fplan(c), c->ps, c->fpart, c->apart,
                               reinterpret_cast<float4*>(d_force4), c->blk));
   LJ_HIP(launch_finalize_forces(st, c->blk, c->nblk, c->cfg.eps, d_ev ? d_ev : c->ev, nullptr));
   return LJ_OK;

@@ -11,7 +11,9 @@ struct lj_ctx {
   lj_config cfg{};
   double L = 0.;
   hipStream_t stream = nullptr;
-  lj::ForcePlan plan;
+  lj::ForcePlan plan;   // sliced force pass (also the adjoint pass geometry)
+  lj::ForcePlan plan3;  // pair-once force pass, when enabled (FP32 mode, large N)
+  int2* n3items = nullptr;
   lj::PairScalars ps;
   float4* pos = nullptr;
   float4* vel = nullptr;
@@ -60,4 +62,6 @@ void drop_graph(lj_ctx* c);
 // releases the adjoint scratch of a context
 void adj_free(lj_ctx* c);
 inline bool is_fp64(const lj_ctx* c) { return c->cfg.precision == LJ_PREC_FP64; }
+// plan of the force pass that fills fpart/apart
+inline const ForcePlan& fplan(const lj_ctx* c) { return c->plan3.n3 ? c->plan3 : c->plan; }
 }  // namespace lj

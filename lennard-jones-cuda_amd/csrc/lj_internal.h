@@ -33,8 +33,18 @@ struct ForcePlan {
   int Npad = 0;     // row pitch of the partial buffers
   int ipl = 1;      // i per lane
   int gx = 0;       // workgroups along i
+  // pair-once kernel (lj_force_n3.hip): S = ng groups, one workgroup per item
+  int n3 = 0;
+  int ng = 0;
+  int nitems = 0;
+  const int2* items = nullptr;  // device table of (g, h) group pairs
 };
 ForcePlan make_force_plan(int N, int jslices_override, int ipl);
+ForcePlan make_n3_plan(int N);
+// sizes at which the pair-once kernel fills the GPU (about 4 waves per SIMD)
+bool n3_eligible(int N);
+// the ng (ng + 1) / 2 group pairs of the pair-once kernel, in launch order
+void n3_items(int ng, int2* out);
 
 // Scalars of the pair kernel for one box.
 struct PairScalars {
@@ -55,6 +65,9 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
                         bool compensated, bool ties);
+hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                           const uint4* q, const float4* posf, float4* fpart, float* apart,
+                           bool ties);
 // sum slices -> force (w = e_i); block partials of e, b
 hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                                 const float4* fpart, const float* apart, float4* force,

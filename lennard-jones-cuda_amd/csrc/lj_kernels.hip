@@ -563,6 +563,7 @@ hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& 
                         bool compensated, bool ties)
 {
   const bool ga = ps.thr > 0.f;
+  if (pl.n3 && !compensated && !ga) return launch_force_n3(st, pl, ps, q, posf, fpart, apart, ties);
   if (ps.periodic) {
     if (ties)
       return ga ? launch_force_c<true, true, true>(st, pl, ps, q, posf, fpart, apart, compensated)

@@ -159,3 +159,31 @@ def test_newton_third_law():
     F = xyz(f4)
     tot = np.abs(F.sum(axis=0)).max()
     assert tot <= 1e-5 * np.abs(F).sum() / N * np.sqrt(N)
+
+
+@pytest.mark.parametrize("N,bc", [(12800, ljamd.BC_PERIODIC), (13001, ljamd.BC_PERIODIC),
+                                  (13001, ljamd.BC_WALL), (16384, ljamd.BC_NONE)])
+def test_pair_once_kernel(N, bc):
+    """The pair-once kernel (FP32 mode, N >= 11777, groups of 512; 13001 ends in
+    a partial group) against FP64 and against the sliced kernel (jslices = 32)
+    on the same state; two evaluations are bit-identical."""
+    rho = 0.85
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=13)
+    print(f"\n[case] pair-once N={N} bc={bc}", flush=True)
+    with ljamd.LJSystem(N, rho, 1.0, boundary=bc) as s:
+        s.set_state(pos, np.zeros_like(pos))
+        obs = s.CalculateForces()
+        f4 = s.force
+        obs2 = s.CalculateForces()
+        assert np.array_equal(s.force, f4) and obs2["V"] == obs["V"] and obs2["W"] == obs["W"]
+    f_sl, obs_sl = _gpu_forces(N, rho, bc, pos, jslices=32)
+    F64, ew, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
+    err = force_errors(xyz(f4), F64)
+    err_sl = force_errors(xyz(f_sl), F64)
+    ew_err = np.max(np.abs(f4.reshape(N, 4)[:, 3] - ew[:, 0]) / np.maximum(np.abs(ew[:, 0]), 1.0))
+    print(f"  pair-once {err} sliced {err_sl} E {rel(obs['V'], E):.2e} W {rel(obs['W'], W):.2e} "
+          f"Force.w {ew_err:.2e}")
+    assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5
+    assert rel(obs["V"], E) <= 1e-6 and rel(obs["W"], W) <= 1e-5
+    assert ew_err <= 1e-5
+    assert force_errors(xyz(f4), xyz(f_sl))["global_rel"] <= 2e-6
