@@ -11,7 +11,10 @@ replayed from a captured HIP graph.  value = N^2 * K / t pair interactions/s.
 
 roofline: the force kernel alone, timed with HIP events on the stream it is
 launched on; algorithmic work 20 FLOP per ordered pair, N^2 pairs per launch,
-against the FP32 vector peak.
+against the FP32 vector peak.  At this size the kernel evaluates each unordered
+pair once (Newton's third law, N (N - 1) / 2 evaluations); it is still
+credited N^2 ordered pairs, and the sliced kernel that evaluates both orders
+is timed beside it.
 cpu_baseline: the reference's own CPU path (MDSystem.cpp compiled from
 source into oracle/_ref) running the same workload for a bounded number of
 steps on one host thread, as the reference does.
@@ -36,11 +39,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def profiled_traffic():
+def profiled_traffic(kernel):
     """HBM bytes per force launch from the committed rocprofv3 summary."""
     import re
 
-    path = os.path.join(ROOT, "profiles", "k_force_n16384_periodic.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_n16384_periodic.md")
     try:
         m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
         return float(m.group(1)) if m else None
@@ -64,8 +67,18 @@ def gpu_leg(steps, warmup, boundary=None):
     obs = s.observables()
     reps = 50
     us = s.time_force_kernel(reps)
+    pair_once, nslots = s.force_plan()
     s.close()
-    return {"t": t, "hist": hist, "obs": obs, "force_us": us, "reps": reps}
+    return {"t": t, "hist": hist, "obs": obs, "force_us": us, "reps": reps,
+            "pair_once": pair_once, "nslots": nslots}
+
+
+def sliced_kernel_us(reps=50):
+    """The sliced kernel (both orders of every pair) on the same workload."""
+    import ljamd
+
+    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED, jslices=32) as s:
+        return s.time_force_kernel(reps)
 
 
 def batch_leg(steps, plain_steps=10, nboxes=512):
@@ -164,7 +177,7 @@ def cpu_leg(budget_s=15.0):
             "steps_per_s": steps / tt}
 
 
-def profile_force(reps=20, copy_mib=256):
+def profile_force(kernel, reps=20, copy_mib=256):
     """Launch pattern for counter collection: a device-to-device copy of a
     known size (calibrates FETCH_SIZE / WRITE_SIZE per byte moved), then the
     force kernel alone `reps` times at the headline workload."""
@@ -183,10 +196,12 @@ def profile_force(reps=20, copy_mib=256):
     hip.hipDeviceSynchronize()
     hip.hipFree(src)
     hip.hipFree(dst)
-    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED) as s:
+    jsl = 32 if kernel == "sliced" else 0
+    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED, jslices=jsl) as s:
         us = s.time_force_kernel(reps)
+        pair_once, nslots = s.force_plan()
     print(json.dumps({"profile": "force", "N": N, "reps": reps, "kernel_us": us,
-                      "copy_bytes": nbytes}))
+                      "pair_once": pair_once, "nslots": nslots, "copy_bytes": nbytes}))
 
 
 def main():
@@ -197,11 +212,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=500)
-    ap.add_argument("--profile-force", action="store_true",
+    ap.add_argument("--profile-force", nargs="?", const="pair-once", default=None,
+                    choices=["pair-once", "sliced"],
                     help="only a calibration copy and the force kernel (for rocprofv3)")
     a = ap.parse_args()
     if a.profile_force:
-        profile_force()
+        profile_force(a.profile_force)
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
@@ -213,6 +229,7 @@ def main():
     value = pairs * a.steps / g["t"]
     ms_per_step = 1000.0 * g["t"] / a.steps
     achieved = FLOP_PER_PAIR * pairs / (g["force_us"] * 1e-6) / 1e12
+    sliced_us = sliced_kernel_us()
     import ljamd
 
     wall_steps = min(a.steps, 300)
@@ -244,9 +261,16 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "frac_scalar_fp32": achieved / PEAK_FP32_SCALAR_TFLOPS,
-                     "kernel": "k_force (periodic, fixed-point minimum image)",
-                     "kernel_us": g["force_us"], "traffic": profiled_traffic(),
-                     "issue_slots_per_pair": 22},
+                     "kernel": ("k_force_n3 (each unordered pair once, periodic fixed point)"
+                                if g["pair_once"] else "k_force (sliced, periodic fixed point)"),
+                     "kernel_us": g["force_us"],
+                     "traffic": profiled_traffic("k_force_n3" if g["pair_once"] else "k_force"),
+                     "newton3": g["pair_once"],
+                     "pair_evaluations_per_launch": (N * (N - 1) // 2 if g["pair_once"]
+                                                     else N * N),
+                     "issue_slots_per_ordered_pair": 13.6 if g["pair_once"] else 22,
+                     "sliced_kernel_us": sliced_us,
+                     "sliced_kernel_tflops": FLOP_PER_PAIR * pairs / (sliced_us * 1e-6) / 1e12},
         "cpu_baseline": cpu,
         "hard_wall": wall,
         "batched": batched,

@@ -705,6 +705,15 @@ int lj_time_force_kernel(lj_ctx* c, int reps, double* us)
   return LJ_OK;
 }
 
+int lj_force_plan_info(lj_ctx* c, int* pair_once, int* nslots)
+{
+  if (!c) return fail(LJ_ERR_ARG, "null context");
+  const ForcePlan& pl = is_fp64(c) ? c->plan : fplan(c);
+  if (pair_once) *pair_once = pl.n3;
+  if (nslots) *nslots = pl.S;
+  return LJ_OK;
+}
+
 int lj_forces_device(lj_ctx* c, const float* d_pos4, float* d_force4, double* d_ev, void* stream)
 {
   LJ_TRY(ctx_setdev(c));

@@ -156,6 +156,12 @@ class LJSystem:
         check(lib().lj_time_force_kernel(self.h, int(reps), C.byref(us)))
         return us.value
 
+    def force_plan(self):
+        """(pair_once, nslots) of the force kernel this context runs."""
+        p, n = C.c_int(), C.c_int()
+        check(lib().lj_force_plan_info(self.h, C.byref(p), C.byref(n)))
+        return bool(p.value), n.value
+
     def forces_device(self, d_pos, d_force, d_ev=None, stream=None):
         check(lib().lj_forces_device(self.h, C.c_void_p(d_pos), C.c_void_p(d_force),
                                      C.c_void_p(d_ev) if d_ev else None,

@@ -171,12 +171,15 @@ def test_pair_once_kernel(N, bc):
     L, pos, _ = perturbed_state(N, rho, 1.0, seed=13)
     print(f"\n[case] pair-once N={N} bc={bc}", flush=True)
     with ljamd.LJSystem(N, rho, 1.0, boundary=bc) as s:
+        assert s.force_plan() == (True, (N + 511) // 512)
         s.set_state(pos, np.zeros_like(pos))
         obs = s.CalculateForces()
         f4 = s.force
         obs2 = s.CalculateForces()
         assert np.array_equal(s.force, f4) and obs2["V"] == obs["V"] and obs2["W"] == obs["W"]
     f_sl, obs_sl = _gpu_forces(N, rho, bc, pos, jslices=32)
+    with ljamd.LJSystem(N, rho, 1.0, boundary=bc, jslices=32) as s:
+        assert s.force_plan() == (False, 32)
     F64, ew, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
     err = force_errors(xyz(f4), F64)
     err_sl = force_errors(xyz(f_sl), F64)
