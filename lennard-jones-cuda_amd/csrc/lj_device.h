@@ -202,12 +202,35 @@ struct Partial {
   double fx, fy, fz, a, b;
 };
 
+// Sum of the S partial entries of particle i in slot order.  The loads are
+// issued 16 at a time ahead of the (sequential, fixed-order) adds: one thread
+// per particle leaves few waves in flight, so the sum is bound by load latency
+// unless each thread keeps many loads outstanding.
 __device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
                                               const float* __restrict__ apart, int i, int S,
                                               int Npad)
 {
+  constexpr int B = 16;
   Partial p{0., 0., 0., 0., 0.};
-  for (int s = 0; s < S; ++s) {
+  int s = 0;
+  for (; s + B <= S; s += B) {
+    float4 f[B];
+    float a[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      f[k] = fpart[(size_t)(s + k) * Npad + i];
+      a[k] = apart[(size_t)(s + k) * Npad + i];
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      p.fx += f[k].x;
+      p.fy += f[k].y;
+      p.fz += f[k].z;
+      p.b += f[k].w;
+      p.a += a[k];
+    }
+  }
+  for (; s < S; ++s) {
     const float4 f = fpart[(size_t)s * Npad + i];
     p.fx += f.x;
     p.fy += f.y;

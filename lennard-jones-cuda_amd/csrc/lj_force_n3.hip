@@ -4,18 +4,18 @@
 // Physics as the reference's pair loop (MDSystem.cu:21-138, MDSystem.cpp:
 // 260-309); only the order of evaluation differs.
 //
-// Decomposition.  The particles form groups of 512.  A work item is a pair of
-// groups (g, h), h >= g, run by one workgroup of 16 waves laid out as a 4 x 4
-// grid: wave (wi, wj) takes the 128 particles 512 g + 128 wi + [0, 128) as i
-// (two per lane) and the 128 particles 512 h + 128 wj + [0, 128) as j.
+// Decomposition.  The particles form groups of G = 128 W (W = 4: 512, W = 2:
+// 256).  A work item is a pair of groups (g, h), h >= g, run by one workgroup
+// of W x W waves: wave (wi, wj) takes the 128 particles G g + 128 wi + [0, 128)
+// as i (two per lane) and the 128 particles G h + 128 wj + [0, 128) as j.
 //   * The j particles sit in LDS, stored twice in a row, so that at step t
 //     lane l reads j = (l + t) mod 64 of a 64-particle sub-tile with a plain
 //     immediate offset; after 64 steps every lane has met every j.
 //   * i sums (force, (s/r)^6, (s/r)^6 (2 (s/r)^6 - 1)) stay in registers.
-//   * j sums (minus the force, (s/r)^6 - w) go to the wave's own LDS
-//     accumulators with ds_add_f32.  Within a step the 64 lanes hit 64
-//     different addresses and a wave owns its accumulators, so every sum is
-//     taken in a fixed order and the result is bit-reproducible.
+//   * j sums (minus the force, (s/r)^6 - w) travel with their j particle:
+//     after each step every lane passes its four j sums to the next lower
+//     lane (DPP wave_rol:1), so no atomics are involved and every sum is
+//     taken in a fixed order (bit-reproducible results).
 //   * Diagonal items (g == h): waves with wi < wj evaluate their tile pair
 //     once as above, waves with wi == wj evaluate both orders of their own
 //     tile (i sums only, self pair excluded by index), waves with wi > wj
@@ -29,8 +29,11 @@
 //   i side of a once-evaluated pair: bf = 2 b, af = b + a;
 //   j side:                          bf = 0,   af = sum ((s/r)^6 - w);
 //   both orders (self tile):         bf = b,   af = a.
-// Per ordered pair this takes about 13.5 VALU issue slots against 22 for the
-// sliced kernel (which evaluates each pair twice).
+// Per ordered pair this takes about 15.9 VALU issue slots (15.4 instructions,
+// v_rcp_f32 at half rate) against 22 for the sliced kernel, which evaluates
+// each pair twice.
+#include <cstdlib>
+
 #include "lj_device.h"
 
 namespace lj {
@@ -39,10 +42,8 @@ namespace {
 
 using namespace dev;
 
-constexpr int kG = 512;    // particles per group
-constexpr int kTile = 128; // particles per wave tile
-constexpr int kNT = 1024;  // threads per workgroup (4 x 4 waves)
-constexpr int kAccPerWave = 2 * 4 * 128;  // sub-tiles x components x doubled tile
+constexpr int kTile = 128;  // particles per wave tile
+constexpr int kAccPerWave = 2 * 4 * 64;  // sub-tiles x components x tile
 
 template <bool PERIODIC, bool TIES, bool GUARD>
 __device__ __forceinline__ void terms(const PosT<PERIODIC>& pi, const uint4& pj, float C1, bool ok,
@@ -66,23 +67,29 @@ __device__ __forceinline__ void terms(const PosT<PERIODIC>& pi, const uint4& pj,
   s = w * ir2;
 }
 
-// one tile pair evaluated once: i sums in a1/a2, j sums into the wave's LDS
-// accumulators A (4 components x 128, per sub-tile)
+// rotate a value one lane down the wave: lane l receives lane (l + 1) mod 64
+__device__ __forceinline__ float rot1(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134 /* wave_rol:1 */,
+                                                    0xf, 0xf, false));
+}
+
+// one tile pair evaluated once: i sums in a1/a2; j sums in registers that
+// travel with their j particle (lane l holds the sums of j slot (l + t) mod 64
+// at step t and passes them to lane l - 1 afterwards), so after 64 steps lane l
+// holds those of slot l.  J receives (-fx, -fy, -fz, sum (s/r)^6 - w) per
+// sub-tile.
 template <bool PERIODIC, bool TIES, bool PARTIAL>
-__device__ __forceinline__ void pair_tile(const uint4* P0, float* A0, int lane,
-                                          const PosT<PERIODIC>& pi1, const PosT<PERIODIC>& pi2,
-                                          bool v1, bool v2, float C1, Acc& a1, Acc& a2)
+__device__ __forceinline__ void pair_tile(const uint4* P0, int lane, const PosT<PERIODIC>& pi1,
+                                          const PosT<PERIODIC>& pi2, bool v1, bool v2, float C1,
+                                          Acc& a1, Acc& a2, float (&J)[2][4])
 {
   for (int st = 0; st < 2; ++st) {
     const uint4* P = P0 + st * 128;
-    float* AX = A0 + st * 512;
-    float* AY = AX + 128;
-    float* AZ = AX + 256;
-    float* AM = AX + 384;
+    float jx = 0.f, jy = 0.f, jz = 0.f, jm = 0.f;
 #pragma unroll 8
     for (int t = 0; t < 64; ++t) {
-      const int jx = lane + t;
-      const uint4 pj = P[jx];
+      const uint4 pj = P[lane + t];
       bool ok1 = true, ok2 = true;
       if constexpr (PARTIAL) {
         const bool jv = pj.w != 0u;
@@ -102,15 +109,19 @@ __device__ __forceinline__ void pair_tile(const uint4* P0, float* A0, int lane,
       a2.fz = __builtin_fmaf(dz2, s2, a2.fz);
       a2.a += e2;
       a2.b += w2;
-      const float cx = __builtin_fmaf(-dx2, s2, -(dx1 * s1));
-      const float cy = __builtin_fmaf(-dy2, s2, -(dy1 * s1));
-      const float cz = __builtin_fmaf(-dz2, s2, -(dz1 * s1));
-      const float cm = (e1 - w1) + (e2 - w2);
-      atomicAdd(AX + jx, cx);
-      atomicAdd(AY + jx, cy);
-      atomicAdd(AZ + jx, cz);
-      atomicAdd(AM + jx, cm);
+      jx = __builtin_fmaf(-dx2, s2, __builtin_fmaf(-dx1, s1, jx));
+      jy = __builtin_fmaf(-dy2, s2, __builtin_fmaf(-dy1, s1, jy));
+      jz = __builtin_fmaf(-dz2, s2, __builtin_fmaf(-dz1, s1, jz));
+      jm += (e1 - w1) + (e2 - w2);
+      jx = rot1(jx);
+      jy = rot1(jy);
+      jz = rot1(jz);
+      jm = rot1(jm);
     }
+    J[st][0] = jx;
+    J[st][1] = jy;
+    J[st][2] = jz;
+    J[st][3] = jm;
   }
 }
 
@@ -161,22 +172,27 @@ __device__ __forceinline__ void encode(const Acc& a, int cls, float (&v)[5])
   }
 }
 
-template <bool PERIODIC, bool TIES>
-__global__ __launch_bounds__(kNT) void k_force_n3(const uint4* __restrict__ q,
+// W x W waves per workgroup, groups of G = 128 W particles.  Eight waves per
+// SIMD (at most 64 VGPRs; the loop body is unchanged by the limit, no spills).
+template <bool PERIODIC, bool TIES, int W>
+__global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8))) void k_force_n3(const uint4* __restrict__ q,
                                                   const float4* __restrict__ pf,
                                                   const int2* __restrict__ items,
                                                   float4* __restrict__ fpart,
                                                   float* __restrict__ apart, int N, int Npad,
                                                   float C1)
 {
-  __shared__ uint4 s_pos[4 * 2 * 128];          // [wj][sub-tile][doubled 64]
-  __shared__ float s_acc[16 * kAccPerWave];     // [wave][sub-tile][component][doubled 64]
+  constexpr int kG = kTile * W;
+  constexpr int kAcc = W * W * kAccPerWave > W * 5 * kG ? W * W * kAccPerWave : W * 5 * kG;
+  __shared__ uint4 s_pos[W * 2 * 128];  // [wj][sub-tile][doubled 64]
+  // j sums [wave][sub-tile][component][64], then i sums [wj][component][G]
+  __shared__ float s_acc[kAcc];
   const int2 it = items[blockIdx.x];
   const int g = it.x, h = it.y;
   const int T = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(T >> 6);
   const int lane = T & 63;
-  const int wi = wave >> 2, wj = wave & 3;
+  const int wi = wave / W, wj = wave % W;
   const bool diag = g == h;
   const bool partial = kG * (h + 1) > N;
 
@@ -195,8 +211,6 @@ __global__ __launch_bounds__(kNT) void k_force_n3(const uint4* __restrict__ q,
     d[T & 63] = p;
     d[(T & 63) + 64] = p;
   }
-  for (int k = T; k < 16 * kAccPerWave / 4; k += kNT)
-    reinterpret_cast<float4*>(s_acc)[k] = make_float4(0.f, 0.f, 0.f, 0.f);
   __syncthreads();
 
   const int cls = diag ? (wi < wj ? 1 : (wi == wj ? 2 : 0)) : 1;  // pair tile / self tile / idle
@@ -208,30 +222,37 @@ __global__ __launch_bounds__(kNT) void k_force_n3(const uint4* __restrict__ q,
   acc_zero(a1);
   acc_zero(a2);
   const uint4* P0 = s_pos + wj * 256;
+  float J[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
   if (cls == 1) {
-    float* A0 = s_acc + wave * kAccPerWave;
     if (partial)
-      pair_tile<PERIODIC, TIES, true>(P0, A0, lane, pi1, pi2, v1, v2, C1, a1, a2);
+      pair_tile<PERIODIC, TIES, true>(P0, lane, pi1, pi2, v1, v2, C1, a1, a2, J);
     else
-      pair_tile<PERIODIC, TIES, false>(P0, A0, lane, pi1, pi2, v1, v2, C1, a1, a2);
+      pair_tile<PERIODIC, TIES, false>(P0, lane, pi1, pi2, v1, v2, C1, a1, a2, J);
   } else if (cls == 2) {
     self_tile<PERIODIC, TIES>(P0, lane, i1, i2, kG * h + kTile * wj, pi1, pi2, v1, v2, C1, a1, a2);
   }
+  {
+    float* A = s_acc + wave * kAccPerWave;
+#pragma unroll
+    for (int st = 0; st < 2; ++st)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) A[(st * 4 + c) * 64 + lane] = J[st][c];
+  }
   __syncthreads();
 
-  // j sums of particle 512 h + T over the four i rows, in row order
+  // j sums of particle G h + T over the W i rows, in row order
   float jf[4] = {0.f, 0.f, 0.f, 0.f};
   if (T < kG) {
     const int col = T >> 7, st = (T >> 6) & 1, idx = T & 63;
-    for (int r = 0; r < 4; ++r) {
-      const float* A = s_acc + (r * 4 + col) * kAccPerWave + st * 512;
+    for (int r = 0; r < W; ++r) {
+      const float* A = s_acc + (r * W + col) * kAccPerWave + st * 256;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) jf[c] += A[c * 128 + idx] + A[c * 128 + idx + 64];
+      for (int c = 0; c < 4; ++c) jf[c] += A[c * 64 + idx];
     }
   }
   __syncthreads();
 
-  // i sums through LDS: ib[wj][component][512]
+  // i sums through LDS: ib[wj][component][G]
   float* ib = s_acc;
   {
     float e1[5], e2[5];
@@ -251,7 +272,7 @@ __global__ __launch_bounds__(kNT) void k_force_n3(const uint4* __restrict__ q,
 
   if (T < kG) {
     float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int c2 = 0; c2 < 4; ++c2)
+    for (int c2 = 0; c2 < W; ++c2)
 #pragma unroll
       for (int c = 0; c < 5; ++c) v[c] += ib[(c2 * 5 + c) * kG + T];
     if (diag) {
@@ -277,20 +298,40 @@ __global__ __launch_bounds__(kNT) void k_force_n3(const uint4* __restrict__ q,
 
 }  // namespace
 
+int n3_waves(int /*N*/)
+{
+  // W = 2 (groups of 256, 4-wave workgroups, up to 8 resident per CU) measured
+  // 116 us at N = 16384 against 145 us for W = 4 (groups of 512, 16-wave
+  // workgroups, at most 2 per CU); LJ_PAIR_ONCE_WAVES=4 selects the latter
+  const char* e = std::getenv("LJ_PAIR_ONCE_WAVES");
+  if (e && std::atoi(e) == 4) return 4;
+  return 2;
+}
+
 ForcePlan make_n3_plan(int N)
 {
   ForcePlan pl;
+  const int W = n3_waves(N);
+  const int G = kTile * W;
   pl.N = N;
-  pl.n3 = 1;
-  pl.ng = (N + kG - 1) / kG;
+  pl.n3 = W;
+  pl.ng = (N + G - 1) / G;
   pl.S = pl.ng;
-  pl.jchunk = kG;
+  pl.jchunk = G;
   pl.Npad = (N + 63) & ~63;
   pl.nitems = pl.ng * (pl.ng + 1) / 2;
   return pl;
 }
 
-bool n3_eligible(int N) { return (N + kG - 1) / kG >= 24; }
+bool n3_eligible(int N)
+{
+  // LJ_PAIR_ONCE=0 / 1 turns the kernel off / on at any size (tuning)
+  const char* e = std::getenv("LJ_PAIR_ONCE");
+  if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1' && N >= 256;
+  // measured crossover against the sliced kernel: N = 10240 (55 vs 60 us),
+  // N = 8192 (42.5 vs 39.6 us)
+  return N >= 9472;
+}
 
 void n3_items(int ng, int2* out)
 {
@@ -301,23 +342,35 @@ void n3_items(int ng, int2* out)
   for (int g = 0; g < ng; ++g) out[k++] = make_int2(g, g);
 }
 
+namespace {
+
+template <int W>
+hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
+                       const float4* posf, float4* fpart, float* apart, bool ties)
+{
+  dim3 grid(pl.nitems), block(64 * W * W);
+  if (ps.periodic) {
+    if (ties)
+      hipLaunchKernelGGL((k_force_n3<true, true, W>), grid, block, 0, st, q, posf, pl.items, fpart,
+                         apart, pl.N, pl.Npad, ps.C1);
+    else
+      hipLaunchKernelGGL((k_force_n3<true, false, W>), grid, block, 0, st, q, posf, pl.items,
+                         fpart, apart, pl.N, pl.Npad, ps.C1);
+  } else {
+    hipLaunchKernelGGL((k_force_n3<false, false, W>), grid, block, 0, st, q, posf, pl.items, fpart,
+                       apart, pl.N, pl.Npad, ps.C1);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,
                            bool ties)
 {
-  dim3 grid(pl.nitems), block(kNT);
-  if (ps.periodic) {
-    if (ties)
-      hipLaunchKernelGGL((k_force_n3<true, true>), grid, block, 0, st, q, posf, pl.items, fpart,
-                         apart, pl.N, pl.Npad, ps.C1);
-    else
-      hipLaunchKernelGGL((k_force_n3<true, false>), grid, block, 0, st, q, posf, pl.items, fpart,
-                         apart, pl.N, pl.Npad, ps.C1);
-  } else {
-    hipLaunchKernelGGL((k_force_n3<false, false>), grid, block, 0, st, q, posf, pl.items, fpart,
-                       apart, pl.N, pl.Npad, ps.C1);
-  }
-  return hipGetLastError();
+  if (pl.n3 == 2) return launch_n3_w<2>(st, pl, ps, q, posf, fpart, apart, ties);
+  return launch_n3_w<4>(st, pl, ps, q, posf, fpart, apart, ties);
 }
 
 }  // namespace lj
