@@ -200,6 +200,12 @@ int lj_integrate_async(lj_ctx* ctx, double dt, int nsteps);
 int lj_rdf(lj_ctx* ctx, int rule, int nbins, float dr2, double rmax, uint64_t* counts,
            int accumulate);
 
+/* Speed histogram of the current velocities as MDSystem::initvelo /
+ * updatevelo bin them (MDSystem.cpp:651-694): counts[b] = number of particles
+ * with (int)(sqrt((double)fl32(vx*vx + vy*vy + vz*vz)) / step) == b, for
+ * b < nbins (the reference's maxind = (int)(vmax / step) + 1); nbins <= 16384. */
+int lj_velocity_histogram(lj_ctx* ctx, double step, int nbins, uint64_t* counts);
+
 int lj_renormalize_velocities(lj_ctx* ctx, int recompute_tkin); /* MDSystem.cpp:375-389 */
 int lj_renormalize_to_energy(lj_ctx* ctx, double ust);          /* MDSystem.cpp:391-404 */
 int lj_reset_averaging(lj_ctx* ctx);                            /* MDSystem.cpp:701-705 */

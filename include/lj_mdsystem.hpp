@@ -16,12 +16,12 @@
  *   - setArray writes the device state, which here is authoritative (on the
  *     reference's GPU branch the next CalculateForces overwrites it from h_Pos);
  *   - NdNdr2 is refreshed by CalculateForces() and UpdateRDF(), not by every
- *     step of Integrate() (an extra O(N^2) pass per step); RDF() returns the
- *     normalised curve as two vectors instead of a SplineFunction;
+ *     step of Integrate() (an extra O(N^2) pass per step); RDF() and getvelo()
+ *     return their curves as two vectors instead of a SplineFunction;
  *   - there is no CPU branch: setHardwareMode(false) throws.  The reference's
  *     own class linked against libljhip.so keeps its CPU/GPU switch intact;
  *   - every failure throws std::runtime_error with lj_last_error().
- *   Velocity histogram and Fluctuations (MDSystem.cpp:651-730) are not provided.
+ *   Fluctuations (MDSystem.cpp:708-730, unused there) is not provided.
  */
 #pragma once
 
@@ -57,8 +57,8 @@ class MDSystem {
     }
   };
 
-  struct RdfCurve {
-    std::vector<double> r, g;
+  struct Curve {  // the points of the reference's SplineFunction results
+    std::vector<double> x, y;
   };
 
   MDSystemConfiguration m_config;
@@ -74,6 +74,8 @@ class MDSystem {
   double L = 0;
   std::vector<int> NdNdr2;
   float rdf_dr2 = 0.f;
+  std::vector<std::pair<double, double> > curvelo;  // speed distribution (MDSystem.h:86)
+  int veloIters = 0;
   double av_U_tot = 0, av_T_tot = 0, av_p_tot = 0;
   int av_iters = 0;
 
@@ -115,6 +117,8 @@ class MDSystem {
     SampleInitialConditions();
     CalculateForces();
     CalculateParameters();
+    veloIters = 0;
+    initvelo();
     resetAveraging();
   }
 
@@ -237,18 +241,57 @@ class MDSystem {
   }
 
   // MDSystem.cpp:633-649: the normalised g(r) of NdNdr2 (arguments unused, as there)
-  RdfCurve RDF(double /*rmax*/ = 5., double /*shag*/ = 0.1) const
+  Curve RDF(double /*rmax*/ = 5., double /*shag*/ = 0.1) const
   {
-    const double pi = 3.14159265358979323846;
     const int N = m_config.N;
     const double n0 = N / L / L / L;
-    RdfCurve out;
+    Curve out;
     for (size_t ir = 0; ir < NdNdr2.size(); ++ir) {
       const double r = std::sqrt((ir + 0.5) * rdf_dr2);
-      out.r.push_back(r);
-      out.g.push_back(NdNdr2[ir] / rdf_dr2 / 2. / pi / r / n0 / static_cast<double>(N));
+      out.x.push_back(r);
+      out.y.push_back(NdNdr2[ir] / rdf_dr2 / 2. / pi() / r / n0 / static_cast<double>(N));
     }
     return out;
+  }
+
+  // MDSystem.cpp:651-669: speed distribution of the current velocities,
+  // maxind = (int)(vmax / step) + 1 points at step (i + 1/2)
+  void initvelo(double vmax = 12., double step = 0.12)
+  {
+    const int maxind = (int)(vmax / step) + 1;
+    std::vector<uint64_t> dens = speed_counts(step, maxind);
+    curvelo.clear();
+    for (int i = 0; i < maxind; ++i)
+      curvelo.push_back(std::make_pair(step * (0.5 + i), (int)dens[i] / step / m_config.N));
+    veloIters = 1;
+  }
+
+  // MDSystem.cpp:676-694: running average of the distribution
+  void updatevelo()
+  {
+    if (curvelo.size() < 2) throw std::runtime_error("ljhip: updatevelo before initvelo");
+    const int maxind = (int)curvelo.size();
+    const double shag = curvelo[1].first - curvelo[0].first;
+    std::vector<uint64_t> dens = speed_counts(shag, maxind);
+    for (int i = 0; i < maxind; ++i)
+      curvelo[i].second =
+          (curvelo[i].second * veloIters + (int)dens[i] / shag / m_config.N) / (veloIters + 1);
+    veloIters++;
+  }
+
+  Curve getvelo() const  // MDSystem.cpp:671-674
+  {
+    Curve out;
+    for (size_t i = 0; i < curvelo.size(); ++i) {
+      out.x.push_back(curvelo[i].first);
+      out.y.push_back(curvelo[i].second);
+    }
+    return out;
+  }
+
+  double Maxwell(double v)  // MDSystem.cpp:696-699, at the current temperature T
+  {
+    return 4 * pi() * std::pow(1. / 2. / pi() / T, 3. / 2.) * v * v * std::exp(-v * v / 2. / T);
   }
 
   int fast_round(float x)  // MDSystem.cpp:732-739
@@ -288,6 +331,16 @@ class MDSystem {
   lj_ctx* ctx_ = nullptr;
   bool sync_ = true;
   uint64_t seed_ = 1;
+
+  static double pi() { return 3.141592653589793238462643; }  // MDSystem.cpp:7
+
+  std::vector<uint64_t> speed_counts(double step, int nbins)
+  {
+    if (nbins > 16384) throw std::runtime_error("ljhip: speed histogram limited to 16384 bins");
+    std::vector<uint64_t> c((size_t)nbins);
+    check(lj_velocity_histogram(ctx_, step, nbins, c.data()));
+    return c;
+  }
 
   static void check(int rc)
   {

@@ -276,6 +276,7 @@ int lj_destroy(lj_ctx* c)
   dev_free(c->rdf);
   dev_free(c->edges2);
   dev_free(c->n3items);
+  dev_free(c->vh);
   dev_free(c->xd);
   dev_free(c->vd);
   dev_free(c->fd);
@@ -585,6 +586,27 @@ int lj_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, uint64_t* cou
   }
   LJ_HIP(launch_rdf(st, a));
   LJ_HIP(hipMemcpyAsync(counts, c->rdf, sizeof(uint64_t) * nbins, hipMemcpyDeviceToHost, st));
+  LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_velocity_histogram(lj_ctx* c, double step, int nbins, uint64_t* counts)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!(step > 0.) || !std::isfinite(step)) return fail(LJ_ERR_ARG, "lj_velocity_histogram: step must be positive");
+  if (nbins <= 0 || nbins > 16384) return fail(LJ_ERR_ARG, "lj_velocity_histogram: nbins must be in [1, 16384]");
+  if (!counts) return fail(LJ_ERR_ARG, "lj_velocity_histogram: null counts");
+  hipStream_t st = c->stream;
+  if (nbins > c->vh_cap) {
+    LJ_HIP(hipStreamSynchronize(st));
+    dev_free(c->vh);
+    c->vh_cap = 0;
+    LJ_TRY(dev_alloc(&c->vh, nbins));
+    c->vh_cap = nbins;
+  }
+  LJ_HIP(launch_fill_u64(st, c->vh, nbins));
+  LJ_HIP(launch_velocity_hist(st, c->vel, c->cfg.N, step, nbins, c->vh));
+  LJ_HIP(hipMemcpyAsync(counts, c->vh, sizeof(uint64_t) * nbins, hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
   return LJ_OK;
 }

@@ -36,6 +36,8 @@ struct lj_ctx {
   double rdf_rmax = 0.;
   float* edges2 = nullptr;
   int edges_cap = 0;
+  unsigned long long* vh = nullptr;  // speed histogram counts
+  int vh_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   double g_dt = 0.;

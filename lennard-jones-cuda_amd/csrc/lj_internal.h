@@ -130,6 +130,9 @@ struct RdfArgs {
 };
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);
+// speed histogram (MDSystem.cpp:651-694); hist is added to (clear it first)
+hipError_t launch_velocity_hist(hipStream_t st, const float4* vel, int N, double step, int nbins,
+                                unsigned long long* hist);
 
 // ---------------------------------------------------------------------------
 // batched ensemble (lj_batch.hip)

@@ -467,6 +467,28 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
     if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
 }
 
+// speed histogram of MDSystem::initvelo / updatevelo (MDSystem.cpp:651-694):
+// bin = (int)(sqrt((double)fl32(vx vx + vy vy + vz vz)) / step), kept below
+// nbins; LDS counters per workgroup, one 64-bit atomic per bin
+__global__ __launch_bounds__(kBlock) void k_velocity_hist(const float4* __restrict__ vel, int N,
+                                                          double step, int nbins,
+                                                          unsigned long long* __restrict__ hist)
+{
+#pragma clang fp contract(off)
+  extern __shared__ unsigned int sh_vh[];
+  for (int b = threadIdx.x; b < nbins; b += kBlock) sh_vh[b] = 0u;
+  __syncthreads();
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+    const float4 v = vel[i];
+    const float s = v.x * v.x + v.y * v.y + v.z * v.z;
+    const double t = __builtin_sqrt((double)s) / step;
+    if (t < (double)nbins) atomicAdd(&sh_vh[(int)t], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += kBlock)
+    if (sh_vh[b]) atomicAdd(&hist[b], (unsigned long long)sh_vh[b]);
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int n)
 {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -698,6 +720,15 @@ hipError_t launch_rdf(hipStream_t st, const RdfArgs& a)
   else
     hipLaunchKernelGGL(k_rdf<LJ_RDF_RBINS>, grid, blk, lds, st, LJ_RDF_ARGS);
 #undef LJ_RDF_ARGS
+  return hipGetLastError();
+}
+
+hipError_t launch_velocity_hist(hipStream_t st, const float4* vel, int N, double step, int nbins,
+                                unsigned long long* hist)
+{
+  const int g = max(1, min(nblocks(N), 512));
+  hipLaunchKernelGGL(k_velocity_hist, dim3(g), dim3(kBlock), sizeof(unsigned) * nbins, st, vel, N,
+                     step, nbins, hist);
   return hipGetLastError();
 }
 

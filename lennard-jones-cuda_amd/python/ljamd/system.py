@@ -156,6 +156,12 @@ class LJSystem:
         check(lib().lj_time_force_kernel(self.h, int(reps), C.byref(us)))
         return us.value
 
+    def velocity_histogram(self, step, nbins):
+        """Speed histogram with the binning of MDSystem::initvelo (uint64[nbins])."""
+        out = np.zeros(int(nbins), np.uint64)
+        check(lib().lj_velocity_histogram(self.h, float(step), int(nbins), ptr(out)))
+        return out
+
     def force_plan(self):
         """(pair_once, nslots) of the force kernel this context runs."""
         p, n = C.c_int(), C.c_int()

@@ -513,6 +513,22 @@ void orc_sys_integrate(orc_sys* s, double dt) /* :438-510, :578-582 */
   s->t += dt;          /* :582 */
 }
 
+/* ---------------- velocity distribution ---------------- */
+
+/* Speed histogram of MDSystem::initvelo / updatevelo (MDSystem.cpp:651-694):
+ * bin = (int)(sqrt((double)fl32(vx*vx + vy*vy + vz*vz)) / step), counted
+ * when below nbins (the reference's maxind).  sqrt of the float sum resolves
+ * to the double sqrt there (<cmath>, no using-directive in MDSystem.cpp). */
+void orc_velocity_histogram(int N, const float* vel4, double step, int nbins, int64_t* counts)
+{
+  for (int b = 0; b < nbins; ++b) counts[b] = 0;
+  for (int i = 0; i < 4 * N; i += 4) {
+    const float s = vel4[i] * vel4[i] + vel4[i + 1] * vel4[i + 1] + vel4[i + 2] * vel4[i + 2];
+    const int t = (int)(sqrt((double)s) / step);
+    if (t < nbins) counts[t]++;
+  }
+}
+
 /* ---------------- FP64 step for the precision study ---------------- */
 
 /* Integrate(dt) of MDSystem.cpp:442-510 with every quantity in double:

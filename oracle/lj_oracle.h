@@ -142,6 +142,10 @@ void orc_sys_apply_bc(orc_sys* s);
 /* Integrate(dt): EVN or TVN, BCs, params, t += dt   MDSystem.cpp:438-510,578-582 */
 void orc_sys_integrate(orc_sys* s, double dt);
 
+/* Speed histogram of initvelo / updatevelo (MDSystem.cpp:651-694): counts
+ * of bin (int)(sqrt((double)fl32(|v|^2)) / step) for bins < nbins. */
+void orc_velocity_histogram(int N, const float* vel4, double step, int nbins, int64_t* counts);
+
 /* One Integrate(dt) step (MDSystem.cpp:442-510, EVN or TVN, then the
  * boundary conditions) with every quantity in double, for the precision
  * study: x/v/f double[3N] (f holds the forces of x on entry and on exit),

@@ -121,4 +121,24 @@ void refh_rdf_normalised(void* h, double* x, double* y)
   }
 }
 
+// velocity distribution (MDSystem.cpp:651-694): initvelo on the current
+// velocities, or updatevelo (running average) when update != 0; returns the
+// number of curve points written to x, y (at most cap)
+int refh_velo(void* h, int update, double vmax, double step, double* x, double* y, int cap)
+{
+  MDSystem* m = static_cast<MDSystem*>(h);
+  if (update)
+    m->updatevelo();
+  else
+    m->initvelo(vmax, step);
+  const int n = (int)m->curvelo.vals.size();
+  for (int k = 0; k < n && k < cap; ++k) {
+    x[k] = m->curvelo.vals[k].first;
+    y[k] = m->curvelo.vals[k].second;
+  }
+  return n;
+}
+
+double refh_maxwell(void* h, double v) { return static_cast<MDSystem*>(h)->Maxwell(v); }
+
 }  // extern "C"

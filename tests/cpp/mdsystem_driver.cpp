@@ -6,6 +6,9 @@
 //   step k t U T K V P      (one line per step)
 //   rdf <256 counts>
 //   pos <4N floats>          (h_Pos after the last step)
+//   vel0 / velo0             velocities, then initvelo(3.0, 0.05) of them
+//   vel1 / velo1             after 10 more steps, then updatevelo()
+//   maxwell T M(0) M(0.02) ... M(8)
 //   cpu_mode_rejected 1
 // usage: mdsystem_driver N rho T0 canonical boundary dt steps seed
 #include <cstdio>
@@ -13,6 +16,20 @@
 #include <exception>
 
 #include "lj_mdsystem.hpp"
+
+static void print_floats(const char* tag, const float* a, int n)
+{
+  std::printf("%s", tag);
+  for (int i = 0; i < n; ++i) std::printf(" %.9g", a[i]);
+  std::printf("\n");
+}
+
+static void print_curve(const char* tag, const ljhip::MDSystem::Curve& c)
+{
+  std::printf("%s", tag);
+  for (size_t i = 0; i < c.x.size(); ++i) std::printf(" %.17g %.17g", c.x[i], c.y[i]);
+  std::printf("\n");
+}
 
 int main(int argc, char** argv)
 {
@@ -41,6 +58,18 @@ int main(int argc, char** argv)
     for (int c : s.NdNdr2) std::printf(" %d", c);
     std::printf("\npos");
     for (int i = 0; i < 4 * cfg.N; ++i) std::printf(" %.9g", s.h_Pos[i]);
+    std::printf("\n");
+    // speed distribution: initvelo on the current velocities, 10 more steps,
+    // updatevelo (MDSystem.cpp:651-694)
+    print_floats("vel0", s.getArray(1), 4 * cfg.N);
+    s.initvelo(3.0, 0.05);
+    print_curve("velo0", s.getvelo());
+    s.Integrate(dt, 10);
+    print_floats("vel1", s.getArray(1), 4 * cfg.N);
+    s.updatevelo();
+    print_curve("velo1", s.getvelo());
+    std::printf("maxwell %.17g", s.T);
+    for (int k = 0; k <= 400; ++k) std::printf(" %.17g", s.Maxwell(0.02 * k));
     std::printf("\n");
     int rejected = 0;
     try {

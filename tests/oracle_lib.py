@@ -63,6 +63,7 @@ def _load_oracle():
     lib.orc_rdf.argtypes = [C.c_int, _f32p, C.c_double, C.c_int, C.c_int, C.c_int,
                             C.c_float, C.c_double, _i64p, C.c_int]
     lib.orc_rdf_edges2.argtypes = [C.c_int, C.c_double, _f32p]
+    lib.orc_velocity_histogram.argtypes = [C.c_int, _f32p, C.c_double, C.c_int, _i64p]
     lib.orc_sys_init.restype = C.c_int
     lib.orc_sys_init.argtypes = [C.POINTER(OrcSys), C.c_int, C.c_double, C.c_double,
                                  C.c_int, C.c_int]
@@ -135,6 +136,13 @@ def forces_fp64d(x3, L, periodic, eps=1.0, sigma=1.0, nthreads=0):
 def rdf(pos, L, periodic, rule, nbins, dr2=0.1, rmax=0.0, nthreads=0):
     counts = np.zeros(nbins, np.int64)
     ORC.orc_rdf(pos.size // 4, pos, L, int(periodic), rule, nbins, dr2, rmax, counts, nthreads)
+    return counts
+
+
+def velocity_histogram(vel, step, nbins):
+    counts = np.zeros(nbins, np.int64)
+    ORC.orc_velocity_histogram(vel.size // 4, np.ascontiguousarray(vel, np.float32), step, nbins,
+                               counts)
     return counts
 
 
@@ -219,6 +227,10 @@ def ref_lib(kind="cpu"):
     lib.refh_fast_round.restype = C.c_int
     lib.refh_fast_round.argtypes = [C.c_void_p, C.c_float]
     lib.refh_rdf_normalised.argtypes = [C.c_void_p, _f64p, _f64p]
+    lib.refh_velo.restype = C.c_int
+    lib.refh_velo.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _f64p, _f64p, C.c_int]
+    lib.refh_maxwell.restype = C.c_double
+    lib.refh_maxwell.argtypes = [C.c_void_p, C.c_double]
     return lib
 
 
@@ -270,6 +282,13 @@ class RefSystem:
         hist = np.zeros((n, 5), np.float64)
         self.lib.refh_integrate(self.h, dt, n, hist.ctypes.data)
         return hist
+
+    def velo(self, update=False, vmax=12.0, step=0.12):
+        """initvelo(vmax, step) or updatevelo(): the curve (x, y) (MDSystem.cpp:651-694)."""
+        x = np.zeros(100000, np.float64)
+        y = np.zeros(100000, np.float64)
+        n = self.lib.refh_velo(self.h, int(update), vmax, step, x, y, x.size)
+        return x[:n].copy(), y[:n].copy()
 
     def state(self):
         p = np.zeros(4 * self.N, np.float32)

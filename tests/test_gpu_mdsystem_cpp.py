@@ -4,14 +4,15 @@ C++ program the way the reference's task programs drive MDSystem
 reference CPU path restated in the oracle from the same seeded state:
 per-step U within |dU|/N <= 2e-5 over 200 EVN steps (N = 400, rho* = 0.05,
 T* = 1.4, periodic, dt = 0.005) and NdNdr2 bit-exact with the reference GPU
-binning on the final positions."""
+binning on the final positions; the speed distribution (initvelo /
+updatevelo) bit-exact against the oracle's counts."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from oracle_lib import OracleSystem, lattice_state, rdf
+from oracle_lib import OracleSystem, lattice_state, rdf, velocity_histogram
 
 pytestmark = pytest.mark.gpu
 
@@ -59,3 +60,22 @@ def test_mdsystem_class_vs_reference_cpu():
     counts = rdf(p, L, True, 1, 256, dr2=o.s.rdf_dr2)
     assert np.array_equal(np.array(out["rdf"], np.int64), counts)
     assert out["cpu_mode_rejected"] == [1.0]
+    # speed distribution: initvelo(3.0, 0.05), then updatevelo after 10 steps,
+    # against the oracle's counts with the reference's normalisation
+    # (MDSystem.cpp:651-694), bit for bit
+    nb = int(3.0 / 0.05) + 1
+    c0 = velocity_histogram(np.array(out["vel0"], np.float32), 0.05, nb)
+    v0 = np.array(out["velo0"]).reshape(-1, 2)
+    assert v0.shape[0] == nb
+    assert np.array_equal(v0[:, 0], 0.05 * (0.5 + np.arange(nb)))
+    y0 = c0 / 0.05 / N
+    assert np.array_equal(v0[:, 1], y0)
+    shag = v0[1, 0] - v0[0, 0]
+    c1 = velocity_histogram(np.array(out["vel1"], np.float32), shag, nb)
+    v1 = np.array(out["velo1"]).reshape(-1, 2)
+    assert np.array_equal(v1[:, 1], (y0 * 1 + c1 / shag / N) / 2)
+    # Maxwell(v) (MDSystem.cpp:696-699): unit normalisation and mean speed sqrt(8T/pi)
+    T, m = out["maxwell"][0], np.array(out["maxwell"][1:])
+    v = 0.02 * np.arange(m.size)
+    assert abs(np.trapz(m, v) - 1.0) <= 1e-4
+    assert abs(np.trapz(v * m, v) - np.sqrt(8 * T / np.pi)) <= 1e-3

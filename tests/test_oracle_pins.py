@@ -17,7 +17,7 @@ import pytest
 from helpers import perturbed_state
 from oracle_lib import (ORC, RDF_RBINS, RDF_REF_CPU, RDF_REF_GPU, OracleSystem, RefSystem,
                         forces_fp64, forces_fp64d, forces_ref, lattice_state, rdf, rdf_edges2,
-                        ref_lib)
+                        ref_lib, velocity_histogram)
 
 REF = ref_lib("cpu")
 needs_ref = pytest.mark.skipif(REF is None, reason="reference not built (oracle/_ref)")
@@ -80,6 +80,30 @@ def test_rdf_normalisation_matches_reference():
         q = (counts.astype(np.float32) / np.float32(R.dr2)).astype(np.float64)
         g = q / 2.0 / math.pi / r / n0 / N
         assert np.array_equal(x, r) and np.array_equal(y, g)
+    finally:
+        R.close()
+
+
+def test_velocity_distribution_matches_reference():
+    """initvelo / updatevelo of the reference (MDSystem.cpp:651-694) from the
+    oracle's speed counts, bit for bit."""
+    N, rho = 500, 0.3
+    L, pos, vel = perturbed_state(N, rho, 1.3, seed=6)
+    R = RefSystem(REF, N, 1.3, rho, 0, 0)
+    try:
+        R.set_state(pos, vel)
+        x, y = R.velo(False, vmax=4.0, step=0.07)
+        nb = int(4.0 / 0.07) + 1
+        c = velocity_histogram(vel, 0.07, nb)
+        assert x.size == nb and c.sum() > 0.9 * N
+        assert np.array_equal(x, 0.07 * (0.5 + np.arange(nb)))
+        assert np.array_equal(y, c / 0.07 / N)
+        vel2 = (vel * np.float32(1.1)).astype(np.float32)
+        R.set_state(pos, vel2)
+        x2, y2 = R.velo(True)
+        shag = x[1] - x[0]
+        c2 = velocity_histogram(vel2, shag, nb)
+        assert np.array_equal(x2, x) and np.array_equal(y2, (y * 1 + c2 / shag / N) / 2)
     finally:
         R.close()
 
