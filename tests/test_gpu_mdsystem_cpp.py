@@ -77,5 +77,5 @@ def test_mdsystem_class_vs_reference_cpu():
     # Maxwell(v) (MDSystem.cpp:696-699): unit normalisation and mean speed sqrt(8T/pi)
     T, m = out["maxwell"][0], np.array(out["maxwell"][1:])
     v = 0.02 * np.arange(m.size)
-    assert abs(np.trapz(m, v) - 1.0) <= 1e-4
-    assert abs(np.trapz(v * m, v) - np.sqrt(8 * T / np.pi)) <= 1e-3
+    assert abs(np.trapezoid(m, v) - 1.0) <= 1e-4
+    assert abs(np.trapezoid(v * m, v) - np.sqrt(8 * T / np.pi)) <= 1e-3
