@@ -186,7 +186,11 @@ int lj_get_observables(lj_ctx* ctx, lj_obs* out);
 /* nsteps of Integrate(dt) (MDSystem.cpp:438-583) fully on the device:
  * velocity Verlet EVN or TVN, boundary conditions, observables; captured
  * once into a HIP graph and replayed.  hist (may be NULL) receives 5 doubles
- * per step {U, T, K, V, P}. */
+ * per step {U, T, K, V, P}.  When the context runs the pair-once force kernel
+ * (lj_force_plan_info), the step's force pass sums the potential energy and
+ * virial per workgroup instead of per particle: V, W and P are unchanged, but
+ * the force array's w (per-particle energy, read by no caller of the
+ * reference) is left 0 until the next lj_compute_forces. */
 int lj_integrate(lj_ctx* ctx, double dt, int nsteps, double* hist);
 
 /* Enqueue nsteps of the captured step graph and return without waiting (no

@@ -77,6 +77,8 @@ StepArgs step_args(lj_ctx* c, double dt)
   a.canonical = c->cfg.canonical;
   a.S = fplan(c).S;
   a.Npad = fplan(c).Npad;
+  a.ew = c->step_ew;
+  a.nitems = c->step_ew ? c->plan3.nitems : 0;
   return a;
 }
 
@@ -153,7 +155,7 @@ int enqueue_step(lj_ctx* c, hipStream_t st, double dt)
   LJ_HIP(launch_drift(st, a));
   // the drift kernel already wrote the fixed-point copy for periodic boxes
   LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart,
-                      c->cfg.precision == LJ_PREC_COMPENSATED, false));
+                      c->cfg.precision == LJ_PREC_COMPENSATED, false, c->step_ew));
   if (c->cfg.canonical == LJ_EVN) {
     LJ_HIP(launch_kick_evn(st, a));
   } else {
@@ -276,6 +278,7 @@ int lj_destroy(lj_ctx* c)
   dev_free(c->rdf);
   dev_free(c->edges2);
   dev_free(c->n3items);
+  dev_free(c->step_ew);
   dev_free(c->vh);
   dev_free(c->xd);
   dev_free(c->vd);
@@ -343,6 +346,7 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
         hipMemcpy(c->n3items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice) != hipSuccess)
       A(fail(LJ_ERR_DEVICE, "item table upload failed"));
     c->plan3.items = c->n3items;
+    A(dev_alloc(&c->step_ew, (size_t)2 * c->plan3.nitems));
   }
   A(dev_alloc(&c->blk, (size_t)c->nblk * kBlkFields));
   A(dev_alloc(&c->obs, 1));
@@ -715,10 +719,13 @@ int lj_time_force_kernel(lj_ctx* c, int reps, double* us)
   }
   if (c->ps.periodic) LJ_HIP(launch_prep_q(st, c->pos, c->q, c->cfg.N, c->ps.qscale));
   const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
-  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));  // warm
+  // the force pass of the time step (force-only partials with the pair-once kernel)
+  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false,
+                      c->step_ew));  // warm
   LJ_HIP(hipEventRecord(c->e0, st));
   for (int k = 0; k < reps; ++k)
-    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false));
+    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false,
+                        c->step_ew));
   LJ_HIP(hipEventRecord(c->e1, st));
   LJ_HIP(hipEventSynchronize(c->e1));
   float ms = 0.f;

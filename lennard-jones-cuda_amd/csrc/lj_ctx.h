@@ -14,6 +14,7 @@ struct lj_ctx {
   lj::ForcePlan plan;   // sliced force pass (also the adjoint pass geometry)
   lj::ForcePlan plan3;  // pair-once force pass, when enabled (FP32 mode, large N)
   int2* n3items = nullptr;
+  double* step_ew = nullptr;  // per-workgroup (E, W) of the step's pair-once pass
   lj::PairScalars ps;
   float4* pos = nullptr;
   float4* vel = nullptr;

@@ -206,6 +206,8 @@ struct Partial {
 // issued 16 at a time ahead of the (sequential, fixed-order) adds: one thread
 // per particle leaves few waves in flight, so the sum is bound by load latency
 // unless each thread keeps many loads outstanding.
+// WITH_A = false: force-only partials (apart is not read; a = 0).
+template <bool WITH_A = true>
 __device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
                                               const float* __restrict__ apart, int i, int S,
                                               int Npad)
@@ -219,7 +221,7 @@ __device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
 #pragma unroll
     for (int k = 0; k < B; ++k) {
       f[k] = fpart[(size_t)(s + k) * Npad + i];
-      a[k] = apart[(size_t)(s + k) * Npad + i];
+      a[k] = WITH_A ? apart[(size_t)(s + k) * Npad + i] : 0.f;
     }
 #pragma unroll
     for (int k = 0; k < B; ++k) {
@@ -236,7 +238,7 @@ __device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
     p.fy += f.y;
     p.fz += f.z;
     p.b += f.w;
-    p.a += apart[(size_t)s * Npad + i];
+    if (WITH_A) p.a += apart[(size_t)s * Npad + i];
   }
   return p;
 }

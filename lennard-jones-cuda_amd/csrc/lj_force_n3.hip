@@ -79,7 +79,7 @@ __device__ __forceinline__ float rot1(float v)
 // at step t and passes them to lane l - 1 afterwards), so after 64 steps lane l
 // holds those of slot l.  J receives (-fx, -fy, -fz, sum (s/r)^6 - w) per
 // sub-tile.
-template <bool PERIODIC, bool TIES, bool PARTIAL>
+template <bool PERIODIC, bool TIES, bool PARTIAL, bool ENERGY>
 __device__ __forceinline__ void pair_tile(const uint4* P0, int lane, const PosT<PERIODIC>& pi1,
                                           const PosT<PERIODIC>& pi2, bool v1, bool v2, float C1,
                                           Acc& a1, Acc& a2, float (&J)[2][4])
@@ -112,11 +112,13 @@ __device__ __forceinline__ void pair_tile(const uint4* P0, int lane, const PosT<
       jx = __builtin_fmaf(-dx2, s2, __builtin_fmaf(-dx1, s1, jx));
       jy = __builtin_fmaf(-dy2, s2, __builtin_fmaf(-dy1, s1, jy));
       jz = __builtin_fmaf(-dz2, s2, __builtin_fmaf(-dz1, s1, jz));
-      jm += (e1 - w1) + (e2 - w2);
       jx = rot1(jx);
       jy = rot1(jy);
       jz = rot1(jz);
-      jm = rot1(jm);
+      if constexpr (ENERGY) {
+        jm += (e1 - w1) + (e2 - w2);
+        jm = rot1(jm);
+      }
     }
     J[st][0] = jx;
     J[st][1] = jy;
@@ -174,12 +176,18 @@ __device__ __forceinline__ void encode(const Acc& a, int cls, float (&v)[5])
 
 // W x W waves per workgroup, groups of G = 128 W particles.  Eight waves per
 // SIMD (at most 64 VGPRs; the loop body is unchanged by the limit, no spills).
-template <bool PERIODIC, bool TIES, int W>
+// ENERGY = false (the time step): no per-particle energies; the partial
+// entries carry the force only (w = 0, no apart), and each workgroup writes its
+// totals ew[2 item] = sum over ordered pairs of ((s/r)^12 - (s/r)^6) and
+// ew[2 item + 1] = sum of w = (s/r)^6 (2 (s/r)^6 - 1), the finaliser's E and W
+// terms.  The j side then carries three sums instead of four.
+template <bool PERIODIC, bool TIES, int W, bool ENERGY>
 __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8))) void k_force_n3(const uint4* __restrict__ q,
                                                   const float4* __restrict__ pf,
                                                   const int2* __restrict__ items,
                                                   float4* __restrict__ fpart,
-                                                  float* __restrict__ apart, int N, int Npad,
+                                                  float* __restrict__ apart,
+                                                  double* __restrict__ ew, int N, int Npad,
                                                   float C1)
 {
   constexpr int kG = kTile * W;
@@ -225,9 +233,9 @@ __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8)))
   float J[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
   if (cls == 1) {
     if (partial)
-      pair_tile<PERIODIC, TIES, true>(P0, lane, pi1, pi2, v1, v2, C1, a1, a2, J);
+      pair_tile<PERIODIC, TIES, true, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, a1, a2, J);
     else
-      pair_tile<PERIODIC, TIES, false>(P0, lane, pi1, pi2, v1, v2, C1, a1, a2, J);
+      pair_tile<PERIODIC, TIES, false, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, a1, a2, J);
   } else if (cls == 2) {
     self_tile<PERIODIC, TIES>(P0, lane, i1, i2, kG * h + kTile * wj, pi1, pi2, v1, v2, C1, a1, a2);
   }
@@ -251,6 +259,34 @@ __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8)))
     }
   }
   __syncthreads();
+
+  if constexpr (!ENERGY) {
+    // E and W totals of the item in the ordered-pair convention: pairs of a
+    // pair tile count twice, those of a self tile (both orders) once
+    __shared__ double s_ew[W * W][2];
+    const double m = cls == 1 ? 2. : (cls == 2 ? 1. : 0.);
+    const double se = (double)a1.a + (double)a2.a, sw = (double)a1.b + (double)a2.b;
+    double te = m * 0.5 * (sw - se), tw = m * sw;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      te += __shfl_xor(te, o, 64);
+      tw += __shfl_xor(tw, o, 64);
+    }
+    if (lane == 0) {
+      s_ew[wave][0] = te;
+      s_ew[wave][1] = tw;
+    }
+    __syncthreads();
+    if (T == 0) {
+      double E = 0., Wv = 0.;
+      for (int w = 0; w < W * W; ++w) {
+        E += s_ew[w][0];
+        Wv += s_ew[w][1];
+      }
+      ew[2 * (size_t)blockIdx.x] = E;
+      ew[2 * (size_t)blockIdx.x + 1] = Wv;
+    }
+  }
 
   // i sums through LDS: ib[wj][component][G]
   float* ib = s_acc;
@@ -283,14 +319,14 @@ __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8)))
     }
     const int i = kG * g + T;
     if (i < N) {
-      fpart[(size_t)h * Npad + i] = make_float4(v[0], v[1], v[2], v[3]);
-      apart[(size_t)h * Npad + i] = v[4];
+      fpart[(size_t)h * Npad + i] = make_float4(v[0], v[1], v[2], ENERGY ? v[3] : 0.f);
+      if constexpr (ENERGY) apart[(size_t)h * Npad + i] = v[4];
     }
     if (!diag) {
       const int j = kG * h + T;
       if (j < N) {
         fpart[(size_t)g * Npad + j] = make_float4(jf[0], jf[1], jf[2], 0.f);
-        apart[(size_t)g * Npad + j] = jf[3];
+        if constexpr (ENERGY) apart[(size_t)g * Npad + j] = jf[3];
       }
     }
   }
@@ -344,21 +380,21 @@ void n3_items(int ng, int2* out)
 
 namespace {
 
-template <int W>
+template <int W, bool ENERGY>
 hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
-                       const float4* posf, float4* fpart, float* apart, bool ties)
+                       const float4* posf, float4* fpart, float* apart, double* ew, bool ties)
 {
   dim3 grid(pl.nitems), block(64 * W * W);
   if (ps.periodic) {
     if (ties)
-      hipLaunchKernelGGL((k_force_n3<true, true, W>), grid, block, 0, st, q, posf, pl.items, fpart,
-                         apart, pl.N, pl.Npad, ps.C1);
+      hipLaunchKernelGGL((k_force_n3<true, true, W, ENERGY>), grid, block, 0, st, q, posf,
+                         pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
     else
-      hipLaunchKernelGGL((k_force_n3<true, false, W>), grid, block, 0, st, q, posf, pl.items,
-                         fpart, apart, pl.N, pl.Npad, ps.C1);
+      hipLaunchKernelGGL((k_force_n3<true, false, W, ENERGY>), grid, block, 0, st, q, posf,
+                         pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
   } else {
-    hipLaunchKernelGGL((k_force_n3<false, false, W>), grid, block, 0, st, q, posf, pl.items, fpart,
-                       apart, pl.N, pl.Npad, ps.C1);
+    hipLaunchKernelGGL((k_force_n3<false, false, W, ENERGY>), grid, block, 0, st, q, posf,
+                       pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
   }
   return hipGetLastError();
 }
@@ -367,10 +403,15 @@ hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& p
 
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,
-                           bool ties)
+                           bool ties, double* ew)
 {
-  if (pl.n3 == 2) return launch_n3_w<2>(st, pl, ps, q, posf, fpart, apart, ties);
-  return launch_n3_w<4>(st, pl, ps, q, posf, fpart, apart, ties);
+  // ew != null: the time step's variant (force partials + per-workgroup totals)
+  if (ew) {
+    if (pl.n3 == 2) return launch_n3_w<2, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
+    return launch_n3_w<4, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
+  }
+  if (pl.n3 == 2) return launch_n3_w<2, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
+  return launch_n3_w<4, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
 }
 
 }  // namespace lj

@@ -62,12 +62,14 @@ PairScalars make_pair_scalars(double L, int periodic, double eps, double sigma, 
 hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, double qscale);
 // ties: resolve exact half-box separations as the reference does (single
 // evaluations of arbitrary states); the step graph uses ties = false
+// step_ew (pair-once plans only, may be null): the time step's variant, which
+// writes force-only partials and per-workgroup (E, W) totals to step_ew
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
-                        bool compensated, bool ties);
+                        bool compensated, bool ties, double* step_ew = nullptr);
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,
-                           bool ties);
+                           bool ties, double* ew);
 // sum slices -> force (w = e_i); block partials of e, b
 hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                                 const float4* fpart, const float* apart, float4* force,
@@ -93,6 +95,10 @@ struct StepArgs {
   int N;
   double dt, L, T0, rho, eps, qscale, FS;
   int bc, canonical, S, Npad;
+  // nitems > 0: the force pass wrote force-only partials and per-workgroup
+  // totals ew[2 k] (sum of e over ordered pairs), ew[2 k + 1] (sum of w)
+  const double* ew;
+  int nitems;
 };
 hipError_t launch_drift(hipStream_t st, const StepArgs& a);
 hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a);

@@ -171,7 +171,8 @@ __global__ __launch_bounds__(kBlock) void k_kick_evn(StepArgs a)
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double v3[3] = {0., 0., 0.};
   if (i < a.N) {
-    const Partial pp = sum_slices(a.fpart, a.apart, i, a.S, a.Npad);
+    const Partial pp = a.nitems ? sum_slices<false>(a.fpart, a.apart, i, a.S, a.Npad)
+                                : sum_slices<true>(a.fpart, a.apart, i, a.S, a.Npad);
     const double e = 0.5 * (pp.b - pp.a);
     const float fx = (float)(a.FS * pp.fx), fy = (float)(a.FS * pp.fy), fz = (float)(a.FS * pp.fz);
     a.force[i] = make_float4(fx, fy, fz, (float)e);
@@ -200,7 +201,8 @@ __global__ __launch_bounds__(kBlock) void k_tvn_a(StepArgs a)
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double v3[3] = {0., 0., 0.};
   if (i < a.N) {
-    const Partial pp = sum_slices(a.fpart, a.apart, i, a.S, a.Npad);
+    const Partial pp = a.nitems ? sum_slices<false>(a.fpart, a.apart, i, a.S, a.Npad)
+                                : sum_slices<true>(a.fpart, a.apart, i, a.S, a.Npad);
     const double e = 0.5 * (pp.b - pp.a);
     const float fx = (float)(a.FS * pp.fx), fy = (float)(a.FS * pp.fy), fz = (float)(a.FS * pp.fz);
     a.force[i] = make_float4(fx, fy, fz, (float)e);
@@ -262,6 +264,11 @@ __device__ __forceinline__ void observables(const StepArgs& a, int nblk, bool ad
     v[1] += a.blk[(size_t)r * kBlkFields + BLK_E];
     v[2] += a.blk[(size_t)r * kBlkFields + BLK_B];
   }
+  if (force_terms)  // per-workgroup totals of the time step's force pass
+    for (int r = threadIdx.x; r < a.nitems; r += kBlock) {
+      v[1] += a.ew[2 * (size_t)r];
+      v[2] += a.ew[2 * (size_t)r + 1];
+    }
   block_allsum<3>(v);
   if (threadIdx.x != 0) return;
   lj_obs* o = a.obs;
@@ -582,10 +589,11 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
-                        bool compensated, bool ties)
+                        bool compensated, bool ties, double* step_ew)
 {
   const bool ga = ps.thr > 0.f;
-  if (pl.n3 && !compensated && !ga) return launch_force_n3(st, pl, ps, q, posf, fpart, apart, ties);
+  if (pl.n3 && !compensated && !ga)
+    return launch_force_n3(st, pl, ps, q, posf, fpart, apart, ties, step_ew);
   if (ps.periodic) {
     if (ties)
       return ga ? launch_force_c<true, true, true>(st, pl, ps, q, posf, fpart, apart, compensated)

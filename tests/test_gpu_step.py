@@ -129,3 +129,27 @@ def test_renormalize_and_modes():
         s.Integrate(0.005, 5)
         with pytest.raises(ljamd.LJError):
             s.set_mode(3, 0)
+
+
+@pytest.mark.parametrize("canon", [ljamd.EVN, ljamd.TVN])
+def test_step_totals_of_pair_once_pass(canon):
+    """At N >= 9472 the step's force pass sums E and W per workgroup (no
+    per-particle energies).  Its V and W must equal a full force evaluation
+    at the same positions; Force.w reads 0 after the step, as documented."""
+    N, rho, T0, dt = 16384, 0.85, 1.0, 0.005
+    L, pos, vel = lattice_state(N, rho, T0, seed=5)
+    with ljamd.LJSystem(N, rho, T0, canonical=canon) as s:
+        assert s.force_plan()[0]
+        s.set_state(pos, vel)
+        s.CalculateForces()
+        hist = s.Integrate(dt, 2)
+        o = s.observables()
+        f_step = s.force.reshape(N, 4).copy()
+        o2 = s.CalculateForces()  # full evaluation (per-particle energies) at the same positions
+        f_full = s.force.reshape(N, 4)
+    print(f"\n  V {hist[-1, 3]:.10g} vs {o2['V']:.10g}; W {o['W']:.10g} vs {o2['W']:.10g}")
+    assert abs(hist[-1, 3] - o2["V"]) <= 1e-6 * abs(o2["V"])
+    assert abs(o["W"] - o2["W"]) <= 1e-6 * abs(o2["W"])
+    assert np.all(f_step[:, 3] == 0.0) and np.any(f_full[:, 3] != 0.0)
+    d = np.linalg.norm(f_step[:, :3] - f_full[:, :3]) / np.linalg.norm(f_full[:, :3])
+    assert d <= 1e-6
