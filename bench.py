@@ -268,7 +268,7 @@ def main():
                      "newton3": g["pair_once"],
                      "pair_evaluations_per_launch": (N * (N - 1) // 2 if g["pair_once"]
                                                      else N * N),
-                     "issue_slots_per_ordered_pair": 15.9 if g["pair_once"] else 22,
+                     "issue_slots_per_ordered_pair": 14.4 if g["pair_once"] else 22,
                      "sliced_kernel_us": sliced_us,
                      "sliced_kernel_tflops": FLOP_PER_PAIR * pairs / (sliced_us * 1e-6) / 1e12},
         "cpu_baseline": cpu,
