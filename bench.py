@@ -123,7 +123,8 @@ def batch_leg(steps, plain_steps=10, nboxes=512):
             "plain_loop_steps": plain_steps, "plain_loop_box_steps_per_s": psps,
             "speedup_vs_plain_loop": bsps / psps, "pairs_per_step": pairs,
             "pair_interactions_per_s": pairs * steps / tb,
-            "tuning": {"ipl": tuning[0], "jchunk": tuning[1], "blocks_per_cu": tuning[2]},
+            "tuning": {"ipl": tuning[0], "jchunk": tuning[1], "blocks_per_cu": tuning[2],
+                       "pair_once": tuning[3]},
             "force_pass_us": force_us, "ms_per_step": 1000.0 * tb / steps}
 
 

@@ -294,11 +294,15 @@ typedef struct lj_batch lj_batch;
  *   ipl            i particles per lane (1 or 2): i-tile = 64*ipl particles
  *   jchunk         j particles per work item (waves per box = tiles x slices)
  *   blocks_per_cu  persistent grid of blocks_per_cu workgroups per CU pulling
- *                  items from a queue; 0 = one wave per item (no queue) */
+ *                  items from a queue; 0 = one wave per item (no queue)
+ *   pair_once      1: each unordered pair once, one 4-wave workgroup per pair
+ *                  of 256-particle groups of a box (ipl, jchunk and
+ *                  blocks_per_cu are then unused); 0: the tiling above */
 typedef struct lj_batch_tuning {
   int ipl;
   int jchunk;
   int blocks_per_cu;
+  int pair_once;
 } lj_batch_tuning;
 
 /* per-box statistics, LJ_BATCH_NSTAT doubles per box, accumulated over steps:

@@ -372,6 +372,7 @@ hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a)
 
 hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic, bool ties)
 {
+  if (a.pair_once) return launch_batch_force_n3(st, a, periodic, ties);
   if (periodic) {
     if (ties) {
       if (a.ipl == 2) launch_batch_force_t<true, 2, true>(st, a);

@@ -27,7 +27,7 @@ struct lj_batch {
   std::vector<int> off;
   int Ntot = 0;
   double sub_fraction = 0.5;
-  lj_batch_tuning tune{1, 512, 8};
+  lj_batch_tuning tune{1, 512, 8, 0};
   hipStream_t stream = nullptr;
   // device
   float4 *pos = nullptr, *vel = nullptr, *force = nullptr, *tF = nullptr;
@@ -94,6 +94,8 @@ BatchArgs bargs(lj_batch* b, double dt)
   a.canonical = b->canonical;
   a.ipl = b->tune.ipl;
   a.persistent_grid = b->tune.blocks_per_cu > 0 ? b->tune.blocks_per_cu * b->ncu : 0;
+  a.pair_once = b->tune.pair_once;
+  if (a.pair_once) a.persistent_grid = 0;
   return a;
 }
 
@@ -116,6 +118,7 @@ int build_tables(lj_batch* b)
     p.off = b->off[k];
     int jc = std::min(jc0, (N + 7) & ~7);
     jc = (jc + 7) & ~7;
+    if (b->tune.pair_once) jc = kPairOnceGroup;  // one partial slot per group
     p.jchunk = jc;
     p.S = (N + jc - 1) / jc;
     p.Npad = (N + 63) & ~63;
@@ -135,14 +138,25 @@ int build_tables(lj_batch* b)
     const double side = b->L[k] * std::cbrt(b->sub_fraction);  // MDSystem.cpp:711
     p.sub_lo = 0.5 * b->L[k] - 0.5 * side;
     p.sub_hi = 0.5 * b->L[k] + 0.5 * side;
-    for (int t = 0; t < (N + tile - 1) / tile; ++t)
-      for (int s = 0; s < p.S; ++s) items.push_back(BatchItem{k, t * tile, s, 0});
+    if (b->tune.pair_once) {
+      for (int g = 0; g < p.S; ++g)
+        for (int h = g; h < p.S; ++h) items.push_back(BatchItem{k, g, h, 0});
+    } else {
+      for (int t = 0; t < (N + tile - 1) / tile; ++t)
+        for (int s = 0; s < p.S; ++s) items.push_back(BatchItem{k, t * tile, s, 0});
+    }
   }
   if (pbase > (size_t)0x7fffffff) return fail(LJ_ERR_UNSUPPORTED, "batch partial buffer too large");
   // longest items first (pairs per item), boxes with larger N first on ties
   std::stable_sort(items.begin(), items.end(), [&](const BatchItem& x, const BatchItem& y) {
     auto cost = [&](const BatchItem& w) {
       const BoxParam& p = bp[w.box];
+      if (b->tune.pair_once) {  // pairs of the two groups (a diagonal item has half)
+        const long long G = kPairOnceGroup;
+        const long long ni = std::min(G, (long long)p.N - w.i0 * G);
+        const long long nj = std::min(G, (long long)p.N - w.s * G);
+        return w.i0 == w.s ? ni * nj / 2 : ni * nj;
+      }
       const long long ni = std::min(tile, p.N - w.i0);
       const long long nj = std::min(p.jchunk, p.N - w.s * p.jchunk);
       return ni * nj;
@@ -213,7 +227,7 @@ std::string tune_key(lj_batch* b)
     h *= 1099511628211ull;
   }
   std::ostringstream os;
-  os << dev << ":" << b->ncu << ":" << b->nboxes << ":" << b->boundary << ":" << std::hex << h;
+  os << "v2:" << dev << ":" << b->ncu << ":" << b->nboxes << ":" << b->boundary << ":" << std::hex << h;
   return os.str();
 }
 
@@ -418,8 +432,8 @@ int lj_batch_set_tuning(lj_batch* b, const lj_batch_tuning* t)
 {
   LJ_TRY(bsetdev(b));
   if (!t || (t->ipl != 1 && t->ipl != 2) || t->jchunk < 8 || t->jchunk > (1 << 20) ||
-      t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
-    return fail(LJ_ERR_ARG, "lj_batch_set_tuning: ipl in {1,2}, jchunk in [8, 2^20], blocks_per_cu in [0, 64]");
+      t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->pair_once != 0 && t->pair_once != 1))
+    return fail(LJ_ERR_ARG, "lj_batch_set_tuning: ipl in {1,2}, jchunk in [8, 2^20], blocks_per_cu in [0, 64], pair_once in {0,1}");
   b->tune = *t;
   return build_tables(b);
 }
@@ -458,7 +472,7 @@ int lj_batch_autotune(lj_batch* b, int reps, lj_batch_tuning* best_out, double* 
     std::string k;
     lj_batch_tuning t{};
     double us = 0.;
-    while (in >> k >> t.ipl >> t.jchunk >> t.blocks_per_cu >> us)
+    while (in >> k >> t.ipl >> t.jchunk >> t.blocks_per_cu >> t.pair_once >> us)
       if (k == key) {
         LJ_TRY(lj_batch_set_tuning(b, &t));
         if (best_out) *best_out = t;
@@ -471,22 +485,25 @@ int lj_batch_autotune(lj_batch* b, int reps, lj_batch_tuning* best_out, double* 
   const int bpcs[] = {0, 4, 8};
   lj_batch_tuning best = b->tune;
   double best_t = 1e300;
+  std::vector<lj_batch_tuning> cand;
   for (int ipl : ipls)
     for (int jc : jcs)
-      for (int bpc : bpcs) {
-        lj_batch_tuning t{ipl, jc, bpc};
-        LJ_TRY(lj_batch_set_tuning(b, &t));
-        double us = 0.;
-        LJ_TRY(lj_batch_time_force(b, reps, &us));
-        if (us < best_t) {
-          best_t = us;
-          best = t;
-        }
-      }
+      for (int bpc : bpcs) cand.push_back(lj_batch_tuning{ipl, jc, bpc, 0});
+  cand.push_back(lj_batch_tuning{1, 512, 0, 1});
+  for (const lj_batch_tuning& t : cand) {
+    LJ_TRY(lj_batch_set_tuning(b, &t));
+    double us = 0.;
+    LJ_TRY(lj_batch_time_force(b, reps, &us));
+    if (us < best_t) {
+      best_t = us;
+      best = t;
+    }
+  }
   LJ_TRY(lj_batch_set_tuning(b, &best));
   if (cache) {
     std::ofstream o(cache, std::ios::app);
-    o << key << " " << best.ipl << " " << best.jchunk << " " << best.blocks_per_cu << " " << best_t << "\n";
+    o << key << " " << best.ipl << " " << best.jchunk << " " << best.blocks_per_cu << " "
+      << best.pair_once << " " << best_t << "\n";
   }
   if (best_out) *best_out = best;
   if (best_us) *best_us = best_t;

@@ -181,22 +181,19 @@ __device__ __forceinline__ void encode(const Acc& a, int cls, float (&v)[5])
 // totals ew[2 item] = sum over ordered pairs of ((s/r)^12 - (s/r)^6) and
 // ew[2 item + 1] = sum of w = (s/r)^6 (2 (s/r)^6 - 1), the finaliser's E and W
 // terms.  The j side then carries three sums instead of four.
+// One group pair (g, h) of one box: q/pf/fpart/apart point at the box's
+// arrays, ew (ENERGY = false) at the item's two totals.
 template <bool PERIODIC, bool TIES, int W, bool ENERGY>
-__global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8))) void k_force_n3(const uint4* __restrict__ q,
-                                                  const float4* __restrict__ pf,
-                                                  const int2* __restrict__ items,
-                                                  float4* __restrict__ fpart,
-                                                  float* __restrict__ apart,
-                                                  double* __restrict__ ew, int N, int Npad,
-                                                  float C1)
+__device__ __forceinline__ void n3_item(const uint4* __restrict__ q, const float4* __restrict__ pf,
+                                        float4* __restrict__ fpart, float* __restrict__ apart,
+                                        double* __restrict__ ew, int N, int Npad, float C1, int g,
+                                        int h)
 {
   constexpr int kG = kTile * W;
   constexpr int kAcc = W * W * kAccPerWave > W * 5 * kG ? W * W * kAccPerWave : W * 5 * kG;
   __shared__ uint4 s_pos[W * 2 * 128];  // [wj][sub-tile][doubled 64]
   // j sums [wave][sub-tile][component][64], then i sums [wj][component][G]
   __shared__ float s_acc[kAcc];
-  const int2 it = items[blockIdx.x];
-  const int g = it.x, h = it.y;
   const int T = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(T >> 6);
   const int lane = T & 63;
@@ -283,8 +280,8 @@ __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8)))
         E += s_ew[w][0];
         Wv += s_ew[w][1];
       }
-      ew[2 * (size_t)blockIdx.x] = E;
-      ew[2 * (size_t)blockIdx.x + 1] = Wv;
+      ew[0] = E;
+      ew[1] = Wv;
     }
   }
 
@@ -330,6 +327,30 @@ __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8)))
       }
     }
   }
+}
+
+template <bool PERIODIC, bool TIES, int W, bool ENERGY>
+__global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8))) void k_force_n3(
+    const uint4* __restrict__ q, const float4* __restrict__ pf, const int2* __restrict__ items,
+    float4* __restrict__ fpart, float* __restrict__ apart, double* __restrict__ ew, int N, int Npad,
+    float C1)
+{
+  const int2 it = items[blockIdx.x];
+  n3_item<PERIODIC, TIES, W, ENERGY>(q, pf, fpart, apart, ENERGY ? nullptr : ew + 2 * (size_t)blockIdx.x,
+                                     N, Npad, C1, it.x, it.y);
+}
+
+// batched ensemble: item = (box, g, h) over all boxes; each box's partials at
+// its own rows (pbase, S = groups of 256, Npad), per-particle energies kept
+template <bool PERIODIC, bool TIES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_batch_force_n3(
+    const uint4* __restrict__ q, const float4* __restrict__ pf, float4* __restrict__ fpart,
+    float* __restrict__ apart, const BoxParam* __restrict__ box, const BatchItem* __restrict__ items)
+{
+  const BatchItem w = items[blockIdx.x];
+  const BoxParam bp = box[w.box];
+  n3_item<PERIODIC, TIES, 2, true>(q + bp.off, pf + bp.off, fpart + bp.pbase, apart + bp.pbase,
+                                   nullptr, bp.N, bp.Npad, bp.C1, w.i0, w.s);
 }
 
 }  // namespace
@@ -400,6 +421,23 @@ hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& p
 }
 
 }  // namespace
+
+hipError_t launch_batch_force_n3(hipStream_t st, const BatchArgs& a, bool periodic, bool ties)
+{
+  dim3 grid(a.nitems), block(256);
+  if (periodic) {
+    if (ties)
+      hipLaunchKernelGGL((k_batch_force_n3<true, true>), grid, block, 0, st, a.q, a.pos, a.fpart,
+                         a.apart, a.box, a.items);
+    else
+      hipLaunchKernelGGL((k_batch_force_n3<true, false>), grid, block, 0, st, a.q, a.pos, a.fpart,
+                         a.apart, a.box, a.items);
+  } else {
+    hipLaunchKernelGGL((k_batch_force_n3<false, false>), grid, block, 0, st, a.q, a.pos, a.fpart,
+                       a.apart, a.box, a.items);
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,

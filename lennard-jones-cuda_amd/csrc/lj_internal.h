@@ -155,7 +155,10 @@ struct BoxParam {
   double sub_lo, sub_hi;  // sub-volume cube [lo, hi]^3 of the fluctuation counts
 };
 
-// One work item of the batched force pass: box, i-tile start (local), slice.
+// One work item of the batched force pass: box, i-tile start (local), slice;
+// with the pair-once decomposition (box, group g, group h).
+constexpr int kPairOnceGroup = 256;  // group size of the batched pair-once pass
+
 struct BatchItem {
   int box, i0, s, pad;
 };
@@ -185,9 +188,11 @@ struct BatchArgs {
   double dt, eps;
   int bc, canonical;
   int ipl, persistent_grid;
+  int pair_once;        // items are (box, g, h) group pairs of 256 (k_batch_force_n3)
 };
 hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic, bool ties);
+hipError_t launch_batch_force_n3(hipStream_t st, const BatchArgs& a, bool periodic, bool ties);
 hipError_t launch_batch_kick_evn(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_tvn_a(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_tvn_b(hipStream_t st, const BatchArgs& a);

@@ -36,7 +36,8 @@ class LJObs(C.Structure):
 
 
 class LJBatchTuning(C.Structure):
-    _fields_ = [("ipl", C.c_int), ("jchunk", C.c_int), ("blocks_per_cu", C.c_int)]
+    _fields_ = [("ipl", C.c_int), ("jchunk", C.c_int), ("blocks_per_cu", C.c_int),
+                ("pair_once", C.c_int)]
 
 
 BATCH_NSTAT = 13

@@ -92,14 +92,15 @@ class LJBatch:
     def reset_stats(self):
         check(lib().lj_batch_reset_stats(self.h))
 
-    def set_tuning(self, ipl, jchunk, blocks_per_cu):
-        t = _L.LJBatchTuning(int(ipl), int(jchunk), int(blocks_per_cu))
+    def set_tuning(self, ipl, jchunk, blocks_per_cu, pair_once=0):
+        t = _L.LJBatchTuning(int(ipl), int(jchunk), int(blocks_per_cu), int(pair_once))
         check(lib().lj_batch_set_tuning(self.h, C.byref(t)))
 
     def tuning(self):
+        """(ipl, jchunk, blocks_per_cu, pair_once)"""
         t = _L.LJBatchTuning()
         check(lib().lj_batch_get_tuning(self.h, C.byref(t)))
-        return (t.ipl, t.jchunk, t.blocks_per_cu)
+        return (t.ipl, t.jchunk, t.blocks_per_cu, t.pair_once)
 
     def time_force(self, reps=5):
         us = C.c_double()
@@ -110,4 +111,4 @@ class LJBatch:
         t = _L.LJBatchTuning()
         us = C.c_double()
         check(lib().lj_batch_autotune(self.h, int(reps), C.byref(t), C.byref(us)))
-        return (t.ipl, t.jchunk, t.blocks_per_cu), us.value
+        return (t.ipl, t.jchunk, t.blocks_per_cu, t.pair_once), us.value

@@ -26,7 +26,8 @@ def _perturbed(N, rho, T0, seed):
     return L, p.reshape(-1).copy(), vel
 
 
-@pytest.mark.parametrize("tuning", [(1, 512, 8), (2, 256, 0), (1, 2048, 4), (2, 1024, 8)])
+@pytest.mark.parametrize("tuning", [(1, 512, 8), (2, 256, 0), (1, 2048, 4), (2, 1024, 8),
+                                    (1, 512, 0, 1)])
 def test_batch_forces_vs_oracle(tuning):
     N, rho, T0 = _small_batch()
     print(f"\n[case] batch forces tuning={tuning}", flush=True)
@@ -49,14 +50,19 @@ def test_batch_forces_vs_oracle(tuning):
 def test_batch_tilings_agree_and_persistent_is_bitwise():
     N, rho, T0 = _small_batch(seed=5)
     outs = {}
-    for tuning in [(1, 512, 0), (1, 512, 8), (2, 1024, 4)]:
+    for tuning in [(1, 512, 0), (1, 512, 8), (2, 1024, 4), (1, 512, 0, 1)]:
         with ljamd.LJBatch(N, rho, T0, seed=11) as b:
             b.set_tuning(*tuning)
             b.compute_forces()
             outs[tuning] = b.get_array(ljamd.ARRAY_FORCE)
+            if len(tuning) == 4:  # the pair-once pass is deterministic as well
+                b.compute_forces()
+                assert np.array_equal(b.get_array(ljamd.ARRAY_FORCE), outs[tuning])
     assert np.array_equal(outs[(1, 512, 0)], outs[(1, 512, 8)])  # same items, any wave order
     F1, F2 = xyz(outs[(1, 512, 0)]), xyz(outs[(2, 1024, 4)])
     assert force_errors(F2, F1)["global_rel"] <= 2e-6
+    F3 = xyz(outs[(1, 512, 0, 1)])
+    assert force_errors(F3, F1)["global_rel"] <= 2e-6
 
 
 @pytest.mark.parametrize("canon", [ljamd.EVN, ljamd.TVN])
