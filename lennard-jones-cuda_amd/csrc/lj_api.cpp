@@ -46,8 +46,12 @@ void drop_graph(lj_ctx* c)
 {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->gexec_run) (void)hipGraphExecDestroy(c->gexec_run);
+  if (c->graph_run) (void)hipGraphDestroy(c->graph_run);
   c->gexec = nullptr;
   c->graph = nullptr;
+  c->gexec_run = nullptr;
+  c->graph_run = nullptr;
 }
 
 StepArgs step_args(lj_ctx* c, double dt)
@@ -180,16 +184,14 @@ int ensure_hist(lj_ctx* c, int n)
   return LJ_OK;
 }
 
-// the step captured once per (dt, ensemble, boundary) and replayed
-int ensure_graph(lj_ctx* c, double dt)
+namespace {
+// captures `steps` steps enqueued back to back into one executable graph
+int capture_steps(lj_ctx* c, double dt, int steps, hipGraph_t* graph, hipGraphExec_t* exec)
 {
-  if (c->gexec && c->g_dt == dt && c->g_canon == c->cfg.canonical && c->g_bc == c->cfg.boundary)
-    return LJ_OK;
   hipStream_t st = c->stream;
-  drop_graph(c);
-  LJ_HIP(hipStreamSynchronize(st));
   LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step(c, st, dt);
+  int rc = LJ_OK;
+  for (int k = 0; k < steps && rc == LJ_OK; ++k) rc = enqueue_step(c, st, dt);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(st, &g);
   if (rc != LJ_OK) {
@@ -197,11 +199,36 @@ int ensure_graph(lj_ctx* c, double dt)
     return rc;
   }
   LJ_HIP(ee);
-  c->graph = g;
-  LJ_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  *graph = g;
+  LJ_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return LJ_OK;
+}
+}  // namespace
+
+// the step captured once per (dt, ensemble, boundary) and replayed
+int ensure_graph(lj_ctx* c, double dt)
+{
+  if (c->gexec && c->g_dt == dt && c->g_canon == c->cfg.canonical && c->g_bc == c->cfg.boundary)
+    return LJ_OK;
+  drop_graph(c);
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  LJ_TRY(capture_steps(c, dt, 1, &c->graph, &c->gexec));
+  const int rc = capture_steps(c, dt, kGraphSteps, &c->graph_run, &c->gexec_run);
+  if (rc != LJ_OK) {
+    drop_graph(c);
+    return rc;
+  }
   c->g_dt = dt;
   c->g_canon = c->cfg.canonical;
   c->g_bc = c->cfg.boundary;
+  return LJ_OK;
+}
+
+int launch_steps(lj_ctx* c, int nsteps)
+{
+  int k = 0;
+  for (; k + kGraphSteps <= nsteps; k += kGraphSteps) LJ_HIP(hipGraphLaunch(c->gexec_run, c->stream));
+  for (; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(c->gexec, c->stream));
   return LJ_OK;
 }
 
@@ -521,7 +548,7 @@ int lj_integrate(lj_ctx* c, double dt, int nsteps, double* hist)
   LJ_TRY(ensure_graph(c, dt));
   hipStream_t st = c->stream;
   LJ_HIP(launch_set_hist_base(st, c->obs, c->hist_base));
-  for (int k = 0; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(c->gexec, st));
+  LJ_TRY(launch_steps(c, nsteps));
   if (hist)
     LJ_HIP(hipMemcpyAsync(hist, c->hist, sizeof(double) * 5 * nsteps, hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
@@ -536,7 +563,7 @@ int lj_integrate_async(lj_ctx* c, double dt, int nsteps)
   if (nsteps == 0) return LJ_OK;
   LJ_TRY(ensure_graph(c, dt));
   LJ_HIP(launch_set_hist_base(c->stream, c->obs, c->hist_base));
-  for (int k = 0; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(c->gexec, c->stream));
+  LJ_TRY(launch_steps(c, nsteps));
   return LJ_OK;
 }
 

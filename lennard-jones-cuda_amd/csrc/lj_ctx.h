@@ -39,8 +39,10 @@ struct lj_ctx {
   int edges_cap = 0;
   unsigned long long* vh = nullptr;  // speed histogram counts
   int vh_cap = 0;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;        // one step
   hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph_run = nullptr;    // kGraphSteps steps back to back
+  hipGraphExec_t gexec_run = nullptr;
   double g_dt = 0.;
   int g_canon = -1, g_bc = -1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -55,9 +57,16 @@ struct lj_ctx {
 };
 
 namespace lj {
+// steps per replay of the multi-step graph: each graph launch costs about 8 us
+// of idle device time between replays, against none between the kernels
+// inside one graph
+constexpr int kGraphSteps = 8;
 int ctx_setdev(lj_ctx* c);
-// step graph captured once per (dt, ensemble, boundary) and replayed
+// step graphs (one step, kGraphSteps steps) captured once per (dt, ensemble,
+// boundary) and replayed
 int ensure_graph(lj_ctx* c, double dt);
+// enqueues nsteps steps through the captured graphs
+int launch_steps(lj_ctx* c, int nsteps);
 // device forces, E and W of the context positions into c->force and c->obs
 int compute_forces_internal(lj_ctx* c);
 void refresh_scalars(lj_ctx* c);
