@@ -83,6 +83,7 @@ StepArgs step_args(lj_ctx* c, double dt)
   a.Npad = fplan(c).Npad;
   a.ew = c->step_ew;
   a.nitems = c->step_ew ? c->plan3.nitems : 0;
+  a.nrow = is_fp64(c) ? c->nblk : step_rows(c->cfg.N);
   return a;
 }
 
@@ -375,7 +376,8 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
     c->plan3.items = c->n3items;
     A(dev_alloc(&c->step_ew, (size_t)2 * c->plan3.nitems));
   }
-  A(dev_alloc(&c->blk, (size_t)c->nblk * kBlkFields));
+  // rows: nblk for the per-block kernels, step_rows(N) >= nblk for the step
+  A(dev_alloc(&c->blk, (size_t)std::max(c->nblk, step_rows(N)) * kBlkFields));
   A(dev_alloc(&c->obs, 1));
   A(dev_alloc(&c->hist_base, 1));
   A(dev_alloc(&c->ev, 2));

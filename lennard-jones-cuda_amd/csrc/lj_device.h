@@ -140,11 +140,11 @@ __device__ __forceinline__ void j_range(const uint4* __restrict__ q, const float
 
 // ---------------------------------------------------------------------------
 // deterministic block reduction of NV doubles: butterfly inside the wave, then
-// the 4 wave results in wave order.  Row `out` receives the block totals.
-template <int NV>
+// the NW wave results in wave order.  Row `out` receives the block totals.
+template <int NV, int NW = kWaves>
 __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out, const int (&field)[NV])
 {
-  __shared__ double sh[kWaves][NV];
+  __shared__ double sh[NW][NV];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
@@ -156,7 +156,7 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out, co
   __syncthreads();
   if (threadIdx.x < NV) {
     double s = 0.;
-    for (int w = 0; w < kWaves; ++w) s += sh[w][threadIdx.x];
+    for (int w = 0; w < NW; ++w) s += sh[w][threadIdx.x];
     int f = 0;
 #pragma unroll
     for (int k = 0; k < NV; ++k)
@@ -202,20 +202,20 @@ struct Partial {
   double fx, fy, fz, a, b;
 };
 
-// Sum of the S partial entries of particle i in slot order.  The loads are
-// issued 16 at a time ahead of the (sequential, fixed-order) adds: one thread
-// per particle leaves few waves in flight, so the sum is bound by load latency
-// unless each thread keeps many loads outstanding.
+// Sum of the partial entries [s0, s1) of particle i in slot order.  The loads
+// are issued 16 at a time ahead of the (sequential, fixed-order) adds: few
+// threads per particle leave few waves in flight, so the sum is bound by load
+// latency unless each thread keeps many loads outstanding.
 // WITH_A = false: force-only partials (apart is not read; a = 0).
 template <bool WITH_A = true>
-__device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
-                                              const float* __restrict__ apart, int i, int S,
-                                              int Npad)
+__device__ __forceinline__ Partial sum_range(const float4* __restrict__ fpart,
+                                             const float* __restrict__ apart, int i, int s0,
+                                             int s1, int Npad)
 {
   constexpr int B = 16;
   Partial p{0., 0., 0., 0., 0.};
-  int s = 0;
-  for (; s + B <= S; s += B) {
+  int s = s0;
+  for (; s + B <= s1; s += B) {
     float4 f[B];
     float a[B];
 #pragma unroll
@@ -232,13 +232,49 @@ __device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
       p.a += a[k];
     }
   }
-  for (; s < S; ++s) {
+  for (; s < s1; ++s) {
     const float4 f = fpart[(size_t)s * Npad + i];
     p.fx += f.x;
     p.fy += f.y;
     p.fz += f.z;
     p.b += f.w;
     if (WITH_A) p.a += apart[(size_t)s * Npad + i];
+  }
+  return p;
+}
+
+// Sum of the S partial entries of particle i in slot order.
+template <bool WITH_A = true>
+__device__ __forceinline__ Partial sum_slices(const float4* __restrict__ fpart,
+                                              const float* __restrict__ apart, int i, int S,
+                                              int Npad)
+{
+  return sum_range<WITH_A>(fpart, apart, i, 0, S, Npad);
+}
+
+// Sum of the S partial entries of particle i shared by R adjacent lanes
+// (r = 0..R-1, R a power of two dividing 64): lane r sums the r-th of R
+// contiguous slot ranges in slot order, then a butterfly over the R lanes
+// adds the range sums.  a + b and b + a round alike, so every lane of the
+// particle ends with the same bits.  i < 0 marks a lane without a particle
+// (it contributes zeros); every lane of the wave must call this.
+template <int R, bool WITH_A>
+__device__ __forceinline__ Partial sum_slices_lanes(const float4* __restrict__ fpart,
+                                                    const float* __restrict__ apart, int i,
+                                                    int S, int Npad, int r)
+{
+  static_assert(R > 0 && (R & (R - 1)) == 0 && R <= 64, "R: a power of two up to 64");
+  const int C = (S + R - 1) / R;
+  const int s0 = i >= 0 ? min(S, r * C) : 0;
+  const int s1 = i >= 0 ? min(S, s0 + C) : 0;
+  Partial p = sum_range<WITH_A>(fpart, apart, i, s0, s1, Npad);
+#pragma unroll
+  for (int o = 1; o < R; o <<= 1) {
+    p.fx += __shfl_xor(p.fx, o, 64);
+    p.fy += __shfl_xor(p.fy, o, 64);
+    p.fz += __shfl_xor(p.fz, o, 64);
+    p.a += __shfl_xor(p.a, o, 64);
+    p.b += __shfl_xor(p.b, o, 64);
   }
   return p;
 }

@@ -99,7 +99,13 @@ struct StepArgs {
   // totals ew[2 k] (sum of e over ordered pairs), ew[2 k + 1] (sum of w)
   const double* ew;
   int nitems;
+  // block rows of the step's sums (K2, TV2, E, B): one per kRowParticles
+  // particles in the FP32 step kernels, one per kBlock in the FP64 ones
+  int nrow;
 };
+// particles per block row of the FP32 time-step kernels
+constexpr int kRowParticles = 64;
+inline int step_rows(int N) { return (N + kRowParticles - 1) / kRowParticles; }
 hipError_t launch_drift(hipStream_t st, const StepArgs& a);
 hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a);
 hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a);

@@ -165,14 +165,25 @@ __global__ __launch_bounds__(kBlock) void k_drift(StepArgs a)
   if constexpr (MAKEQ) a.q[i] = make_uint4(fixq(p.x, a.qscale), fixq(p.y, a.qscale), fixq(p.z, a.qscale), 0u);
 }
 
-__global__ __launch_bounds__(kBlock) void k_kick_evn(StepArgs a)
+// The kick kernels read the S partial entries of every particle, a sum bound
+// by load latency: kSumLanes lanes share one particle's sum and a block holds
+// kRowParticles particles, so that every CU takes part and many loads are in
+// flight.  The lead lane of each particle does the integrator arithmetic; each
+// block writes one block row.
+constexpr int kSumLanes = 4;
+constexpr int kSumBlock = kRowParticles * kSumLanes;
+static_assert(kSumBlock == kBlock, "the row sums assume kBlock threads");
+
+template <bool WITH_A>
+__global__ __launch_bounds__(kSumBlock) void k_kick_evn(StepArgs a)
 {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int r = threadIdx.x % kSumLanes;
+  const int i = blockIdx.x * kRowParticles + threadIdx.x / kSumLanes;
+  const Partial pp = sum_slices_lanes<kSumLanes, WITH_A>(a.fpart, a.apart, i < a.N ? i : -1,
+                                                         a.S, a.Npad, r);
   double v3[3] = {0., 0., 0.};
-  if (i < a.N) {
-    const Partial pp = a.nitems ? sum_slices<false>(a.fpart, a.apart, i, a.S, a.Npad)
-                                : sum_slices<true>(a.fpart, a.apart, i, a.S, a.Npad);
+  if (r == 0 && i < a.N) {
     const double e = 0.5 * (pp.b - pp.a);
     const float fx = (float)(a.FS * pp.fx), fy = (float)(a.FS * pp.fy), fz = (float)(a.FS * pp.fz);
     a.force[i] = make_float4(fx, fy, fz, (float)e);
@@ -182,9 +193,9 @@ __global__ __launch_bounds__(kBlock) void k_kick_evn(StepArgs a)
     v.y = (float)(v.y + dt * fy / 2.);
     v.z = (float)(v.z + dt * fz / 2.);
     if (a.bc != LJ_BC_NONE) {
-      float4 p = a.pos[i];
-      apply_bc_one(p, v, a.bc, a.L);
-      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
+      float4 q = a.pos[i];
+      apply_bc_one(q, v, a.bc, a.L);
+      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = q;
     }
     a.vel[i] = v;
     v3[0] = kin2(v);
@@ -195,14 +206,16 @@ __global__ __launch_bounds__(kBlock) void k_kick_evn(StepArgs a)
   block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
 }
 
-__global__ __launch_bounds__(kBlock) void k_tvn_a(StepArgs a)
+template <bool WITH_A>
+__global__ __launch_bounds__(kSumBlock) void k_tvn_a(StepArgs a)
 {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int r = threadIdx.x % kSumLanes;
+  const int i = blockIdx.x * kRowParticles + threadIdx.x / kSumLanes;
+  const Partial pp = sum_slices_lanes<kSumLanes, WITH_A>(a.fpart, a.apart, i < a.N ? i : -1,
+                                                         a.S, a.Npad, r);
   double v3[3] = {0., 0., 0.};
-  if (i < a.N) {
-    const Partial pp = a.nitems ? sum_slices<false>(a.fpart, a.apart, i, a.S, a.Npad)
-                                : sum_slices<true>(a.fpart, a.apart, i, a.S, a.Npad);
+  if (r == 0 && i < a.N) {
     const double e = 0.5 * (pp.b - pp.a);
     const float fx = (float)(a.FS * pp.fx), fy = (float)(a.FS * pp.fy), fz = (float)(a.FS * pp.fz);
     a.force[i] = make_float4(fx, fy, fz, (float)e);
@@ -226,14 +239,17 @@ __global__ __launch_bounds__(kBlock) void k_tvn_a(StepArgs a)
   block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
 }
 
-__global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a, int nblk)
+// one particle per thread; each wave (kRowParticles particles) writes the K2
+// field of its block row
+__global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a)
 {
 #pragma clang fp contract(off)
-  const double tv2 = blk_total(a.blk, nblk, BLK_TV2);
+  static_assert(kRowParticles == 64, "one block row per wave");
+  const double tv2 = blk_total(a.blk, a.nrow, BLK_TV2);
   const double Tkin = tv2 * (1. / 3. / a.N);  // MDSystem.cpp:370
   const double chi = sqrt(a.T0 / Tkin);      // MDSystem.cpp:499
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  double v1[1] = {0.};
+  double k2 = 0.;
   if (i < a.N) {
     float4 v = a.vel[i];
     const float4 tf = a.tF[i];
@@ -247,11 +263,12 @@ __global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a, int nblk)
       if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
     }
     a.vel[i] = v;
-    v1[0] = kin2(v);
+    k2 = kin2(v);
   }
-  __syncthreads();
-  const int fld[1] = {BLK_K2};
-  block_sum_store<1>(v1, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) k2 += __shfl_xor(k2, o, 64);
+  const int row = i / kRowParticles;
+  if ((threadIdx.x & 63) == 0 && row < a.nrow) a.blk[(size_t)row * kBlkFields + BLK_K2] = k2;
 }
 
 // observables from block partials: CalculateParameters (MDSystem.cpp:325-359)
@@ -638,25 +655,31 @@ hipError_t launch_drift(hipStream_t st, const StepArgs& a)
 
 hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a)
 {
-  hipLaunchKernelGGL(k_kick_evn, dim3(nblocks(a.N)), dim3(kBlock), 0, st, a);
+  if (a.nitems)  // force-only partials of the pair-once time-step pass
+    hipLaunchKernelGGL(k_kick_evn<false>, dim3(a.nrow), dim3(kSumBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_kick_evn<true>, dim3(a.nrow), dim3(kSumBlock), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a)
 {
-  hipLaunchKernelGGL(k_tvn_a, dim3(nblocks(a.N)), dim3(kBlock), 0, st, a);
+  if (a.nitems)
+    hipLaunchKernelGGL(k_tvn_a<false>, dim3(a.nrow), dim3(kSumBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_tvn_a<true>, dim3(a.nrow), dim3(kSumBlock), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_tvn_b(hipStream_t st, const StepArgs& a)
 {
-  hipLaunchKernelGGL(k_tvn_b, dim3(nblocks(a.N)), dim3(kBlock), 0, st, a, nblocks(a.N));
+  hipLaunchKernelGGL(k_tvn_b, dim3(nblocks(a.N)), dim3(kBlock), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_finalize_step(hipStream_t st, const StepArgs& a)
 {
-  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(kBlock), 0, st, a, nblocks(a.N));
+  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(kBlock), 0, st, a, a.nrow);
   return hipGetLastError();
 }
 

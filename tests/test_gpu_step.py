@@ -98,12 +98,18 @@ def test_time_reversibility():
     assert err < 1e-3, err
 
 
-def test_graph_replay_deterministic():
-    N, rho, T0 = 3000, 0.85, 1.0
+@pytest.mark.parametrize("N,canon", [(3000, ljamd.TVN), (10000, ljamd.TVN), (10000, ljamd.EVN)])
+def test_graph_replay_deterministic(N, canon):
+    # 20 steps run as 8 + 8 + 4 x 1 (multi-step graph, then single steps)
+    # and as 7 x 1 + (8 + 5 x 1): the same kernels, the same bits; N = 10000
+    # takes the pair-once force pass
+    rho, T0 = 0.85, 1.0
     L, pos, vel = lattice_state(N, rho, T0, seed=9)
+    print(f"\n[case] graph replay N={N} canon={canon}", flush=True)
     outs = []
     for chunks in ([20], [7, 13]):
-        with ljamd.LJSystem(N, rho, T0, canonical=ljamd.TVN) as s:
+        with ljamd.LJSystem(N, rho, T0, canonical=canon) as s:
+            assert s.force_plan()[0] == (N >= 9472)
             s.set_state(pos, vel)
             s.CalculateForces()
             hs = [s.Integrate(0.005, c) for c in chunks]
