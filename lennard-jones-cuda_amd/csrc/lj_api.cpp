@@ -54,7 +54,8 @@ void drop_graph(lj_ctx* c)
   c->graph_run = nullptr;
 }
 
-StepArgs step_args(lj_ctx* c, double dt)
+// b: which of the two per-workgroup total buffers the step's force pass fills
+StepArgs step_args(lj_ctx* c, double dt, int b = 0)
 {
   StepArgs a{};
   a.pos = c->pos;
@@ -81,7 +82,7 @@ StepArgs step_args(lj_ctx* c, double dt)
   a.canonical = c->cfg.canonical;
   a.S = fplan(c).S;
   a.Npad = fplan(c).Npad;
-  a.ew = c->step_ew;
+  a.ew = c->step_ew ? c->step_ew + (size_t)b * 2 * c->plan3.nitems : nullptr;
   a.nitems = c->step_ew ? c->plan3.nitems : 0;
   a.nrow = is_fp64(c) ? c->nblk : step_rows(c->cfg.N);
   return a;
@@ -153,22 +154,44 @@ int enqueue_force(lj_ctx* c, hipStream_t st, const float4* d_pos)
   return LJ_OK;
 }
 
-int enqueue_step(lj_ctx* c, hipStream_t st, double dt)
+// One step of a run of steps captured together.  The first step starts with
+// the drift; each later one finds it done by the previous step's last kernel
+// (drift_next).  With the pair-once pass the observables of a step are
+// computed by one extra workgroup of the next step's force pass, and only the
+// last step launches the stand-alone finaliser.  Steps alternate between the
+// two buffers of per-workgroup totals (b = step parity), so that the
+// finaliser of step k reads its totals while step k + 1 writes its own.  Every
+// kernel does the arithmetic of the one-step sequence, so a run gives the
+// bits of single steps.
+int enqueue_step_part(lj_ctx* c, hipStream_t st, double dt, bool first, bool last, int b)
 {
   if (is_fp64(c)) return enqueue_step64(c, st, dt);
-  const StepArgs a = step_args(c, dt);
-  LJ_HIP(launch_drift(st, a));
+  const StepArgs a = step_args(c, dt, b);
+  const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
+  const bool fused_fin = a.nitems > 0;  // the pair-once time-step pass
+  if (first) LJ_HIP(launch_drift(st, a));
   // the drift kernel already wrote the fixed-point copy for periodic boxes
-  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart,
-                      c->cfg.precision == LJ_PREC_COMPENSATED, false, c->step_ew));
+  double* ew = const_cast<double*>(a.ew);
+  if (fused_fin && !first) {
+    const StepArgs prev = step_args(c, dt, b ^ 1);
+    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false, ew,
+                        &prev));
+  } else {
+    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false, ew));
+  }
   if (c->cfg.canonical == LJ_EVN) {
-    LJ_HIP(launch_kick_evn(st, a));
+    LJ_HIP(launch_kick_evn(st, a, !last));
   } else {
     LJ_HIP(launch_tvn_a(st, a));
-    LJ_HIP(launch_tvn_b(st, a));
+    LJ_HIP(launch_tvn_b(st, a, !last));
   }
-  LJ_HIP(launch_finalize_step(st, a));
+  if (last || !fused_fin) LJ_HIP(launch_finalize_step(st, a));
   return LJ_OK;
+}
+
+int enqueue_step(lj_ctx* c, hipStream_t st, double dt)
+{
+  return enqueue_step_part(c, st, dt, true, true, 0);
 }
 
 int ensure_hist(lj_ctx* c, int n)
@@ -192,7 +215,8 @@ int capture_steps(lj_ctx* c, double dt, int steps, hipGraph_t* graph, hipGraphEx
   hipStream_t st = c->stream;
   LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int rc = LJ_OK;
-  for (int k = 0; k < steps && rc == LJ_OK; ++k) rc = enqueue_step(c, st, dt);
+  for (int k = 0; k < steps && rc == LJ_OK; ++k)
+    rc = enqueue_step_part(c, st, dt, k == 0, k == steps - 1, k & 1);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(st, &g);
   if (rc != LJ_OK) {
@@ -374,7 +398,7 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
         hipMemcpy(c->n3items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice) != hipSuccess)
       A(fail(LJ_ERR_DEVICE, "item table upload failed"));
     c->plan3.items = c->n3items;
-    A(dev_alloc(&c->step_ew, (size_t)2 * c->plan3.nitems));
+    A(dev_alloc(&c->step_ew, (size_t)4 * c->plan3.nitems));  // two buffers
   }
   // rows: nblk for the per-block kernels, step_rows(N) >= nblk for the step
   A(dev_alloc(&c->blk, (size_t)std::max(c->nblk, step_rows(N)) * kBlkFields));

@@ -165,11 +165,11 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out, co
   }
 }
 
-// block-wide total of NV doubles, result broadcast to every thread
-template <int NV>
+// block-wide total of NV doubles over NW waves, result broadcast to every thread
+template <int NV, int NW = kWaves>
 __device__ __forceinline__ void block_allsum(double (&v)[NV])
 {
-  __shared__ double sh[kWaves][NV];
+  __shared__ double sh[NW][NV];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
@@ -183,7 +183,7 @@ __device__ __forceinline__ void block_allsum(double (&v)[NV])
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.;
-    for (int w = 0; w < kWaves; ++w) s += sh[w][k];
+    for (int w = 0; w < NW; ++w) s += sh[w][k];
     v[k] = s;
   }
 }
@@ -313,6 +313,84 @@ __device__ __forceinline__ float kin2(const float4& v)
   return s;
 }
 
+// Observables of a step from its block rows: CalculateParameters
+// (MDSystem.cpp:325-359).  Run by one block of NW waves (the stand-alone
+// finaliser, or one extra workgroup of the next step's force pass); the
+// reduction tree depends on NW only, so both give the same bits for NW = 4.
+// force_terms: V and W from the block rows' E/B fields plus the per-workgroup
+// totals of the force pass (a.ew, a.nitems); advance: t and the step counter
+// move on and the history row of the step is written.
+template <int NW>
+__device__ __forceinline__ void step_observables(const StepArgs& a, int nrow, bool advance,
+                                                 bool force_terms)
+{
+  constexpr int NT = 64 * NW;
+  double v[3] = {0., 0., 0.};
+  for (int r = threadIdx.x; r < nrow; r += NT) {
+    v[0] += a.blk[(size_t)r * kBlkFields + BLK_K2];
+    v[1] += a.blk[(size_t)r * kBlkFields + BLK_E];
+    v[2] += a.blk[(size_t)r * kBlkFields + BLK_B];
+  }
+  if (force_terms)  // per-workgroup totals of the time step's force pass
+    for (int r = threadIdx.x; r < a.nitems; r += NT) {
+      v[1] += a.ew[2 * (size_t)r];
+      v[2] += a.ew[2 * (size_t)r + 1];
+    }
+  block_allsum<3, NW>(v);
+  if (threadIdx.x != 0) return;
+  lj_obs* o = a.obs;
+  const int N = a.N;
+  const double K = v[0] / 2.;
+  const double T = 2. * K / 3. / N;
+  if (force_terms) {
+    o->V = 2. * a.eps * v[1];
+    o->W = 12. * a.eps * v[2];
+  }
+  const double Pvir = o->W / 3.;
+  const double P = (Pvir + N * T) / (N / a.rho);
+  o->K = K;
+  o->T = T;
+  o->P = P;
+  o->U = K + o->V;
+  o->Pshear = 0.;
+  o->av_iters += 1;
+  o->av_U_tot += o->U;
+  o->av_p_tot += P;
+  o->av_T_tot += T;
+  if (advance) {
+    o->t += a.dt;
+    const int64_t k = o->step - *a.hist_base;
+    o->step += 1;
+    if (a.hist && k >= 0 && k < a.hist_cap) {
+      double* h = a.hist + 5 * k;
+      h[0] = o->U;
+      h[1] = T;
+      h[2] = K;
+      h[3] = o->V;
+      h[4] = P;
+    }
+  }
+}
+
+// First half of the velocity-Verlet step for one particle, in the reference's
+// double/float pattern: x += dt v + dt^2 f / 2 (MDSystem.cpp:447-449), then
+// EVN: v += dt f / 2 (:451-453); TVN: tF = f / 2 (:477-479).
+template <int CANON>
+__device__ __forceinline__ void drift_one(float4& p, float4& v, float4& tf, const float4& f,
+                                          double dt)
+{
+#pragma clang fp contract(off)
+  p.x = (float)(p.x + (dt * v.x + dt * dt * f.x / 2.));
+  p.y = (float)(p.y + (dt * v.y + dt * dt * f.y / 2.));
+  p.z = (float)(p.z + (dt * v.z + dt * dt * f.z / 2.));
+  if constexpr (CANON == LJ_EVN) {
+    v.x = (float)(v.x + dt * f.x / 2.);
+    v.y = (float)(v.y + dt * f.y / 2.);
+    v.z = (float)(v.z + dt * f.z / 2.);
+  } else {
+    tf = make_float4(0.5f * f.x, 0.5f * f.y, 0.5f * f.z, 0.f);
+  }
+}
 
 }  // namespace dev
 }  // namespace lj

@@ -329,15 +329,28 @@ __device__ __forceinline__ void n3_item(const uint4* __restrict__ q, const float
   }
 }
 
-template <bool PERIODIC, bool TIES, int W, bool ENERGY>
+// FIN: one workgroup besides the items computes the observables of the
+// previous step from its block rows and force totals (`fin`: that step's
+// arguments, its own ew buffer), in place of a separate finaliser launch
+// between the steps.  It is block 0, so that it is dispatched with the first
+// wave of workgroups: as the last block it would wait for a free slot and
+// finish after the items.
+template <bool PERIODIC, bool TIES, int W, bool ENERGY, bool FIN>
 __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8))) void k_force_n3(
     const uint4* __restrict__ q, const float4* __restrict__ pf, const int2* __restrict__ items,
     float4* __restrict__ fpart, float* __restrict__ apart, double* __restrict__ ew, int N, int Npad,
-    float C1)
+    float C1, StepArgs fin)
 {
-  const int2 it = items[blockIdx.x];
-  n3_item<PERIODIC, TIES, W, ENERGY>(q, pf, fpart, apart, ENERGY ? nullptr : ew + 2 * (size_t)blockIdx.x,
-                                     N, Npad, C1, it.x, it.y);
+  if constexpr (FIN) {
+    if (blockIdx.x == 0) {
+      step_observables<W * W>(fin, fin.nrow, true, true);
+      return;
+    }
+  }
+  const int k = FIN ? (int)blockIdx.x - 1 : (int)blockIdx.x;  // item
+  const int2 it = items[k];
+  n3_item<PERIODIC, TIES, W, ENERGY>(q, pf, fpart, apart, ENERGY ? nullptr : ew + 2 * (size_t)k, N,
+                                     Npad, C1, it.x, it.y);
 }
 
 // batched ensemble: item = (box, g, h) over all boxes; each box's partials at
@@ -401,22 +414,38 @@ void n3_items(int ng, int2* out)
 
 namespace {
 
-template <int W, bool ENERGY>
-hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
-                       const float4* posf, float4* fpart, float* apart, double* ew, bool ties)
+template <int W, bool ENERGY, bool FIN>
+void launch_n3_f(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
+                 const float4* posf, float4* fpart, float* apart, double* ew, bool ties,
+                 const StepArgs& fin)
 {
-  dim3 grid(pl.nitems), block(64 * W * W);
+  dim3 grid(pl.nitems + (FIN ? 1 : 0)), block(64 * W * W);
   if (ps.periodic) {
     if (ties)
-      hipLaunchKernelGGL((k_force_n3<true, true, W, ENERGY>), grid, block, 0, st, q, posf,
-                         pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+      hipLaunchKernelGGL((k_force_n3<true, true, W, ENERGY, FIN>), grid, block, 0, st, q, posf,
+                         pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1, fin);
     else
-      hipLaunchKernelGGL((k_force_n3<true, false, W, ENERGY>), grid, block, 0, st, q, posf,
-                         pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+      hipLaunchKernelGGL((k_force_n3<true, false, W, ENERGY, FIN>), grid, block, 0, st, q, posf,
+                         pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1, fin);
   } else {
-    hipLaunchKernelGGL((k_force_n3<false, false, W, ENERGY>), grid, block, 0, st, q, posf,
-                       pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+    hipLaunchKernelGGL((k_force_n3<false, false, W, ENERGY, FIN>), grid, block, 0, st, q, posf,
+                       pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1, fin);
   }
+}
+
+template <int W, bool ENERGY>
+hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
+                       const float4* posf, float4* fpart, float* apart, double* ew, bool ties,
+                       const StepArgs* fin)
+{
+  if constexpr (!ENERGY) {  // the finaliser workgroup rides on the time step's variant only
+    if (fin) {
+      launch_n3_f<W, ENERGY, true>(st, pl, ps, q, posf, fpart, apart, ew, ties, *fin);
+      return hipGetLastError();
+    }
+  }
+  if (fin) return hipErrorInvalidValue;
+  launch_n3_f<W, ENERGY, false>(st, pl, ps, q, posf, fpart, apart, ew, ties, StepArgs{});
   return hipGetLastError();
 }
 
@@ -441,15 +470,16 @@ hipError_t launch_batch_force_n3(hipStream_t st, const BatchArgs& a, bool period
 
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,
-                           bool ties, double* ew)
+                           bool ties, double* ew, const StepArgs* fin)
 {
-  // ew != null: the time step's variant (force partials + per-workgroup totals)
+  // ew != null: the time step's variant (force partials + per-workgroup totals),
+  // with the previous step's observables when fin != null
   if (ew) {
-    if (pl.n3 == 2) return launch_n3_w<2, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
-    return launch_n3_w<4, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
+    if (pl.n3 == 2) return launch_n3_w<2, false>(st, pl, ps, q, posf, fpart, apart, ew, false, fin);
+    return launch_n3_w<4, false>(st, pl, ps, q, posf, fpart, apart, ew, false, fin);
   }
-  if (pl.n3 == 2) return launch_n3_w<2, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
-  return launch_n3_w<4, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
+  if (pl.n3 == 2) return launch_n3_w<2, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties, nullptr);
+  return launch_n3_w<4, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties, nullptr);
 }
 
 }  // namespace lj

@@ -63,13 +63,17 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 // ties: resolve exact half-box separations as the reference does (single
 // evaluations of arbitrary states); the step graph uses ties = false
 // step_ew (pair-once plans only, may be null): the time step's variant, which
-// writes force-only partials and per-workgroup (E, W) totals to step_ew
+// writes force-only partials and per-workgroup (E, W) totals to step_ew;
+// fin (with step_ew only, may be null): one more workgroup computes the
+// observables of the step described by *fin (the previous one) meanwhile
+struct StepArgs;
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
-                        bool compensated, bool ties, double* step_ew = nullptr);
+                        bool compensated, bool ties, double* step_ew = nullptr,
+                        const StepArgs* fin = nullptr);
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,
-                           bool ties, double* ew);
+                           bool ties, double* ew, const StepArgs* fin);
 // sum slices -> force (w = e_i); block partials of e, b
 hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                                 const float4* fpart, const float* apart, float4* force,
@@ -107,9 +111,10 @@ struct StepArgs {
 constexpr int kRowParticles = 64;
 inline int step_rows(int N) { return (N + kRowParticles - 1) / kRowParticles; }
 hipError_t launch_drift(hipStream_t st, const StepArgs& a);
-hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a);
+// drift_next: also the next step's drift (as launch_drift) for each particle
+hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a, bool drift_next = false);
 hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a);
-hipError_t launch_tvn_b(hipStream_t st, const StepArgs& a);
+hipError_t launch_tvn_b(hipStream_t st, const StepArgs& a, bool drift_next = false);
 // observables of the step; advance t/step, write history
 hipError_t launch_finalize_step(hipStream_t st, const StepArgs& a);
 // observables of the current state (no time advance): K from vel, V/W from blk

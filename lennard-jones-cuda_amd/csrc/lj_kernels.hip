@@ -143,24 +143,14 @@ __global__ __launch_bounds__(kBlock) void k_finalize_forces(const double* __rest
 template <bool MAKEQ, int CANON>
 __global__ __launch_bounds__(kBlock) void k_drift(StepArgs a)
 {
-#pragma clang fp contract(off)
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.N) return;
   float4 p = a.pos[i];
   float4 v = a.vel[i];
-  const float4 f = a.force[i];
-  const double dt = a.dt;
-  p.x = (float)(p.x + (dt * v.x + dt * dt * f.x / 2.));  // MDSystem.cpp:447-449
-  p.y = (float)(p.y + (dt * v.y + dt * dt * f.y / 2.));
-  p.z = (float)(p.z + (dt * v.z + dt * dt * f.z / 2.));
-  if constexpr (CANON == LJ_EVN) {
-    v.x = (float)(v.x + dt * f.x / 2.);  // MDSystem.cpp:451-453
-    v.y = (float)(v.y + dt * f.y / 2.);
-    v.z = (float)(v.z + dt * f.z / 2.);
-    a.vel[i] = v;
-  } else {
-    a.tF[i] = make_float4(0.5f * f.x, 0.5f * f.y, 0.5f * f.z, 0.f);  // MDSystem.cpp:477-479
-  }
+  float4 tf;
+  drift_one<CANON>(p, v, tf, a.force[i], a.dt);
+  if constexpr (CANON == LJ_EVN) a.vel[i] = v;
+  else a.tF[i] = tf;
   a.pos[i] = p;
   if constexpr (MAKEQ) a.q[i] = make_uint4(fixq(p.x, a.qscale), fixq(p.y, a.qscale), fixq(p.z, a.qscale), 0u);
 }
@@ -170,11 +160,14 @@ __global__ __launch_bounds__(kBlock) void k_drift(StepArgs a)
 // kRowParticles particles, so that every CU takes part and many loads are in
 // flight.  The lead lane of each particle does the integrator arithmetic; each
 // block writes one block row.
+// DRIFT: the kernel that ends a step also does the next step's drift for its
+// particle (the arithmetic of k_drift on the values it just wrote), so that a
+// run of steps needs no separate drift launch; MAKEQ: periodic fixed-point copy.
 constexpr int kSumLanes = 4;
 constexpr int kSumBlock = kRowParticles * kSumLanes;
 static_assert(kSumBlock == kBlock, "the row sums assume kBlock threads");
 
-template <bool WITH_A>
+template <bool WITH_A, bool DRIFT, bool MAKEQ>
 __global__ __launch_bounds__(kSumBlock) void k_kick_evn(StepArgs a)
 {
 #pragma clang fp contract(off)
@@ -192,15 +185,23 @@ __global__ __launch_bounds__(kSumBlock) void k_kick_evn(StepArgs a)
     v.x = (float)(v.x + dt * fx / 2.);  // MDSystem.cpp:460-462
     v.y = (float)(v.y + dt * fy / 2.);
     v.z = (float)(v.z + dt * fz / 2.);
+    float4 p;
+    if (DRIFT || a.bc != LJ_BC_NONE) p = a.pos[i];
     if (a.bc != LJ_BC_NONE) {
-      float4 q = a.pos[i];
-      apply_bc_one(q, v, a.bc, a.L);
-      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = q;
+      apply_bc_one(p, v, a.bc, a.L);
+      if (!DRIFT && a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
     }
-    a.vel[i] = v;
     v3[0] = kin2(v);
     v3[1] = e;
     v3[2] = pp.b;
+    if constexpr (DRIFT) {
+      float4 tf;
+      drift_one<LJ_EVN>(p, v, tf, make_float4(fx, fy, fz, 0.f), dt);
+      a.pos[i] = p;
+      if constexpr (MAKEQ)
+        a.q[i] = make_uint4(fixq(p.x, a.qscale), fixq(p.y, a.qscale), fixq(p.z, a.qscale), 0u);
+    }
+    a.vel[i] = v;
   }
   const int fld[3] = {BLK_K2, BLK_E, BLK_B};
   block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
@@ -241,6 +242,7 @@ __global__ __launch_bounds__(kSumBlock) void k_tvn_a(StepArgs a)
 
 // one particle per thread; each wave (kRowParticles particles) writes the K2
 // field of its block row
+template <bool DRIFT, bool MAKEQ>
 __global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a)
 {
 #pragma clang fp contract(off)
@@ -257,13 +259,22 @@ __global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a)
     v.x = (float)((2. * chi - 1.) * v.x + chi * dt * tf.x);  // MDSystem.cpp:503-505
     v.y = (float)((2. * chi - 1.) * v.y + chi * dt * tf.y);
     v.z = (float)((2. * chi - 1.) * v.z + chi * dt * tf.z);
+    float4 p;
+    if (DRIFT || a.bc != LJ_BC_NONE) p = a.pos[i];
     if (a.bc != LJ_BC_NONE) {
-      float4 p = a.pos[i];
       apply_bc_one(p, v, a.bc, a.L);
-      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
+      if (!DRIFT && a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
     }
     a.vel[i] = v;
     k2 = kin2(v);
+    if constexpr (DRIFT) {
+      float4 tfn;
+      drift_one<LJ_TVN>(p, v, tfn, a.force[i], dt);
+      a.tF[i] = tfn;
+      a.pos[i] = p;
+      if constexpr (MAKEQ)
+        a.q[i] = make_uint4(fixq(p.x, a.qscale), fixq(p.y, a.qscale), fixq(p.z, a.qscale), 0u);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) k2 += __shfl_xor(k2, o, 64);
@@ -271,65 +282,14 @@ __global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a)
   if ((threadIdx.x & 63) == 0 && row < a.nrow) a.blk[(size_t)row * kBlkFields + BLK_K2] = k2;
 }
 
-// observables from block partials: CalculateParameters (MDSystem.cpp:325-359)
-__device__ __forceinline__ void observables(const StepArgs& a, int nblk, bool advance,
-                                            bool force_terms)
-{
-  double v[3] = {0., 0., 0.};
-  for (int r = threadIdx.x; r < nblk; r += kBlock) {
-    v[0] += a.blk[(size_t)r * kBlkFields + BLK_K2];
-    v[1] += a.blk[(size_t)r * kBlkFields + BLK_E];
-    v[2] += a.blk[(size_t)r * kBlkFields + BLK_B];
-  }
-  if (force_terms)  // per-workgroup totals of the time step's force pass
-    for (int r = threadIdx.x; r < a.nitems; r += kBlock) {
-      v[1] += a.ew[2 * (size_t)r];
-      v[2] += a.ew[2 * (size_t)r + 1];
-    }
-  block_allsum<3>(v);
-  if (threadIdx.x != 0) return;
-  lj_obs* o = a.obs;
-  const int N = a.N;
-  const double K = v[0] / 2.;
-  const double T = 2. * K / 3. / N;
-  if (force_terms) {
-    o->V = 2. * a.eps * v[1];
-    o->W = 12. * a.eps * v[2];
-  }
-  const double Pvir = o->W / 3.;
-  const double P = (Pvir + N * T) / (N / a.rho);
-  o->K = K;
-  o->T = T;
-  o->P = P;
-  o->U = K + o->V;
-  o->Pshear = 0.;
-  o->av_iters += 1;
-  o->av_U_tot += o->U;
-  o->av_p_tot += P;
-  o->av_T_tot += T;
-  if (advance) {
-    o->t += a.dt;
-    const int64_t k = o->step - *a.hist_base;
-    o->step += 1;
-    if (a.hist && k >= 0 && k < a.hist_cap) {
-      double* h = a.hist + 5 * k;
-      h[0] = o->U;
-      h[1] = T;
-      h[2] = K;
-      h[3] = o->V;
-      h[4] = P;
-    }
-  }
-}
-
 __global__ __launch_bounds__(kBlock) void k_finalize_step(StepArgs a, int nblk)
 {
-  observables(a, nblk, true, true);
+  step_observables<kWaves>(a, nblk, true, true);
 }
 
 __global__ __launch_bounds__(kBlock) void k_params(StepArgs a, int nblk, int force_terms)
 {
-  observables(a, nblk, false, force_terms != 0);
+  step_observables<kWaves>(a, nblk, false, force_terms != 0);
 }
 
 __global__ void k_set_hist_base(lj_obs* obs, int64_t* base) { *base = obs->step; }
@@ -606,11 +566,12 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
-                        bool compensated, bool ties, double* step_ew)
+                        bool compensated, bool ties, double* step_ew, const StepArgs* fin)
 {
   const bool ga = ps.thr > 0.f;
   if (pl.n3 && !compensated && !ga)
-    return launch_force_n3(st, pl, ps, q, posf, fpart, apart, ties, step_ew);
+    return launch_force_n3(st, pl, ps, q, posf, fpart, apart, ties, step_ew, step_ew ? fin : nullptr);
+  if (fin) return hipErrorInvalidValue;  // the finaliser workgroup exists in the pair-once pass only
   if (ps.periodic) {
     if (ties)
       return ga ? launch_force_c<true, true, true>(st, pl, ps, q, posf, fpart, apart, compensated)
@@ -653,12 +614,21 @@ hipError_t launch_drift(hipStream_t st, const StepArgs& a)
   return hipGetLastError();
 }
 
-hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a)
+template <bool WITH_A>
+void kick_evn_w(hipStream_t st, const StepArgs& a, bool drift_next)
+{
+  const dim3 g(a.nrow), b(kSumBlock);
+  if (!drift_next) hipLaunchKernelGGL((k_kick_evn<WITH_A, false, false>), g, b, 0, st, a);
+  else if (a.bc == LJ_BC_PERIODIC) hipLaunchKernelGGL((k_kick_evn<WITH_A, true, true>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((k_kick_evn<WITH_A, true, false>), g, b, 0, st, a);
+}
+
+hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a, bool drift_next)
 {
   if (a.nitems)  // force-only partials of the pair-once time-step pass
-    hipLaunchKernelGGL(k_kick_evn<false>, dim3(a.nrow), dim3(kSumBlock), 0, st, a);
+    kick_evn_w<false>(st, a, drift_next);
   else
-    hipLaunchKernelGGL(k_kick_evn<true>, dim3(a.nrow), dim3(kSumBlock), 0, st, a);
+    kick_evn_w<true>(st, a, drift_next);
   return hipGetLastError();
 }
 
@@ -671,9 +641,12 @@ hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a)
   return hipGetLastError();
 }
 
-hipError_t launch_tvn_b(hipStream_t st, const StepArgs& a)
+hipError_t launch_tvn_b(hipStream_t st, const StepArgs& a, bool drift_next)
 {
-  hipLaunchKernelGGL(k_tvn_b, dim3(nblocks(a.N)), dim3(kBlock), 0, st, a);
+  const dim3 g(nblocks(a.N)), b(kBlock);
+  if (!drift_next) hipLaunchKernelGGL((k_tvn_b<false, false>), g, b, 0, st, a);
+  else if (a.bc == LJ_BC_PERIODIC) hipLaunchKernelGGL((k_tvn_b<true, true>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((k_tvn_b<true, false>), g, b, 0, st, a);
   return hipGetLastError();
 }
 

@@ -98,24 +98,32 @@ def test_time_reversibility():
     assert err < 1e-3, err
 
 
-@pytest.mark.parametrize("N,canon", [(3000, ljamd.TVN), (10000, ljamd.TVN), (10000, ljamd.EVN)])
-def test_graph_replay_deterministic(N, canon):
-    # 20 steps run as 8 + 8 + 4 x 1 (multi-step graph, then single steps)
-    # and as 7 x 1 + (8 + 5 x 1): the same kernels, the same bits; N = 10000
+@pytest.mark.parametrize("N,canon,bc", [
+    (3000, ljamd.TVN, ljamd.BC_PERIODIC), (3000, ljamd.EVN, ljamd.BC_WALL),
+    (10000, ljamd.TVN, ljamd.BC_PERIODIC), (10000, ljamd.EVN, ljamd.BC_PERIODIC),
+    (10000, ljamd.TVN, ljamd.BC_WALL), (10000, ljamd.EVN, ljamd.BC_NONE)])
+def test_graph_replay_deterministic(N, canon, bc):
+    # 20 steps run as 8 + 8 + 4 x 1 (8-step graphs, where each step's kick
+    # kernel does the next drift and, with the pair-once pass, the next force
+    # pass finalises the step; then single steps) and as 7 x 1 + (8 + 5 x 1):
+    # the same arithmetic, the same bits, history rows included; N = 10000
     # takes the pair-once force pass
     rho, T0 = 0.85, 1.0
     L, pos, vel = lattice_state(N, rho, T0, seed=9)
-    print(f"\n[case] graph replay N={N} canon={canon}", flush=True)
+    print(f"\n[case] graph replay N={N} canon={canon} bc={bc}", flush=True)
     outs = []
     for chunks in ([20], [7, 13]):
-        with ljamd.LJSystem(N, rho, T0, canonical=canon) as s:
+        with ljamd.LJSystem(N, rho, T0, canonical=canon, boundary=bc) as s:
             assert s.force_plan()[0] == (N >= 9472)
             s.set_state(pos, vel)
             s.CalculateForces()
             hs = [s.Integrate(0.005, c) for c in chunks]
-            outs.append((s.pos, s.vel, np.concatenate(hs)))
-    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
-    assert np.array_equal(outs[0][2], outs[1][2])
+            obs = s.observables()
+            outs.append((s.pos, s.vel, np.concatenate(hs), s.force,
+                         [obs[k] for k in ("U", "T", "K", "V", "P", "t", "step", "av_iters")]))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    assert outs[0][4][6] == 20
 
 
 def test_renormalize_and_modes():
