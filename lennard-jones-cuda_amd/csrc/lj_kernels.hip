@@ -173,20 +173,24 @@ __global__ __launch_bounds__(kSumBlock) void k_kick_evn(StepArgs a)
 #pragma clang fp contract(off)
   const int r = threadIdx.x % kSumLanes;
   const int i = blockIdx.x * kRowParticles + threadIdx.x / kSumLanes;
+  const bool lead = r == 0 && i < a.N;
+  // the particle's own loads go out ahead of the slot sums (their latencies overlap)
+  float4 v = {}, p = {};
+  if (lead) {
+    v = a.vel[i];
+    if (DRIFT || a.bc != LJ_BC_NONE) p = a.pos[i];
+  }
   const Partial pp = sum_slices_lanes<kSumLanes, WITH_A>(a.fpart, a.apart, i < a.N ? i : -1,
                                                          a.S, a.Npad, r);
   double v3[3] = {0., 0., 0.};
-  if (r == 0 && i < a.N) {
+  if (lead) {
     const double e = 0.5 * (pp.b - pp.a);
     const float fx = (float)(a.FS * pp.fx), fy = (float)(a.FS * pp.fy), fz = (float)(a.FS * pp.fz);
     a.force[i] = make_float4(fx, fy, fz, (float)e);
-    float4 v = a.vel[i];
     const double dt = a.dt;
     v.x = (float)(v.x + dt * fx / 2.);  // MDSystem.cpp:460-462
     v.y = (float)(v.y + dt * fy / 2.);
     v.z = (float)(v.z + dt * fz / 2.);
-    float4 p;
-    if (DRIFT || a.bc != LJ_BC_NONE) p = a.pos[i];
     if (a.bc != LJ_BC_NONE) {
       apply_bc_one(p, v, a.bc, a.L);
       if (!DRIFT && a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
@@ -213,19 +217,24 @@ __global__ __launch_bounds__(kSumBlock) void k_tvn_a(StepArgs a)
 #pragma clang fp contract(off)
   const int r = threadIdx.x % kSumLanes;
   const int i = blockIdx.x * kRowParticles + threadIdx.x / kSumLanes;
+  const bool lead = r == 0 && i < a.N;
+  // the particle's own loads go out ahead of the slot sums (their latencies overlap)
+  float4 tf = {}, v = {};
+  if (lead) {
+    tf = a.tF[i];
+    v = a.vel[i];
+  }
   const Partial pp = sum_slices_lanes<kSumLanes, WITH_A>(a.fpart, a.apart, i < a.N ? i : -1,
                                                          a.S, a.Npad, r);
   double v3[3] = {0., 0., 0.};
-  if (r == 0 && i < a.N) {
+  if (lead) {
     const double e = 0.5 * (pp.b - pp.a);
     const float fx = (float)(a.FS * pp.fx), fy = (float)(a.FS * pp.fy), fz = (float)(a.FS * pp.fz);
     a.force[i] = make_float4(fx, fy, fz, (float)e);
-    float4 tf = a.tF[i];
     tf.x += 0.5f * fx;  // MDSystem.cpp:486-488
     tf.y += 0.5f * fy;
     tf.z += 0.5f * fz;
     a.tF[i] = tf;
-    const float4 v = a.vel[i];
     const double dt = a.dt;
     float4 tv;
     tv.x = (float)(v.x + dt * tf.x / 2.);  // MDSystem.cpp:493-495
@@ -247,20 +256,24 @@ __global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a)
 {
 #pragma clang fp contract(off)
   static_assert(kRowParticles == 64, "one block row per wave");
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  // the particle's loads go out ahead of the row total (their latencies overlap)
+  float4 v = {}, tf = {}, p = {}, f = {};
+  if (i < a.N) {
+    v = a.vel[i];
+    tf = a.tF[i];
+    if (DRIFT || a.bc != LJ_BC_NONE) p = a.pos[i];
+    if (DRIFT) f = a.force[i];
+  }
   const double tv2 = blk_total(a.blk, a.nrow, BLK_TV2);
   const double Tkin = tv2 * (1. / 3. / a.N);  // MDSystem.cpp:370
   const double chi = sqrt(a.T0 / Tkin);      // MDSystem.cpp:499
-  const int i = blockIdx.x * kBlock + threadIdx.x;
   double k2 = 0.;
   if (i < a.N) {
-    float4 v = a.vel[i];
-    const float4 tf = a.tF[i];
     const double dt = a.dt;
     v.x = (float)((2. * chi - 1.) * v.x + chi * dt * tf.x);  // MDSystem.cpp:503-505
     v.y = (float)((2. * chi - 1.) * v.y + chi * dt * tf.y);
     v.z = (float)((2. * chi - 1.) * v.z + chi * dt * tf.z);
-    float4 p;
-    if (DRIFT || a.bc != LJ_BC_NONE) p = a.pos[i];
     if (a.bc != LJ_BC_NONE) {
       apply_bc_one(p, v, a.bc, a.L);
       if (!DRIFT && a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
@@ -269,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_tvn_b(StepArgs a)
     k2 = kin2(v);
     if constexpr (DRIFT) {
       float4 tfn;
-      drift_one<LJ_TVN>(p, v, tfn, a.force[i], dt);
+      drift_one<LJ_TVN>(p, v, tfn, f, dt);
       a.tF[i] = tfn;
       a.pos[i] = p;
       if constexpr (MAKEQ)
