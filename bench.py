@@ -178,10 +178,61 @@ def cpu_leg(budget_s=15.0):
             "steps_per_s": steps / tt}
 
 
-def profile_force(kernel, reps=20, copy_mib=256):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_parallel_leg(threads=16, budget_s=5.0):
+    """The reference CPU path restated in the oracle (bit-identical to it: each
+    particle's j order is unchanged) with the i loop split over `threads` host
+    threads, on the same workload; a reported baseline, not a target."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleSystem, lattice_state
+
+    L, pos, vel = lattice_state(N, RHO, T0, SEED)
+    o = OracleSystem(N, T0, RHO, 1, 0, nthreads=threads)
+    o.set_state(pos, vel)
+    o.forces()
+    o.params()
+    t0 = time.perf_counter()
+    o.integrate(DT, 1)
+    t1 = time.perf_counter() - t0
+    k = max(1, min(40, int(budget_s / max(t1, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    o.integrate(DT, k)
+    tk = time.perf_counter() - t0
+    steps, tt = k + 1, t1 + tk
+    return {"value": N * N * steps / tt, "unit": "pair interactions/s", "cores": threads,
+            "kind": "port", "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": f"{steps} TVN steps of the oracle's restatement of MDSystem::Integrate "
+                      f"at N={N}, rho*={RHO}, {threads} host threads, {tt:.1f} s",
+            "steps_per_s": steps / tt}
+
+
+def profiled_sclk_ghz(kernel):
+    """Shader clock measured in the committed profile of the force kernel
+    (GRBM_GUI_ACTIVE cycles per XCD over the kernel's duration)."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"{kernel}_n16384_periodic.md")
+    try:
+        m = re.search(r"sclk_ghz_measured:\s*([0-9.]+)", open(path).read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def profile_force(kernel, reps=20, warm=600, copy_mib=256):
     """Launch pattern for counter collection: a device-to-device copy of a
     known size (calibrates FETCH_SIZE / WRITE_SIZE per byte moved), then the
-    force kernel alone `reps` times at the headline workload."""
+    force kernel alone `warm` times (the clocks settle over the first ~0.1 s
+    of load) and `reps` times more, timed, at the headline workload."""
     import ctypes as C
 
     import ljamd
@@ -199,9 +250,10 @@ def profile_force(kernel, reps=20, copy_mib=256):
     hip.hipFree(dst)
     jsl = 32 if kernel == "sliced" else 0
     with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED, jslices=jsl) as s:
+        s.time_force_kernel(warm)
         us = s.time_force_kernel(reps)
         pair_once, nslots = s.force_plan()
-    print(json.dumps({"profile": "force", "N": N, "reps": reps, "kernel_us": us,
+    print(json.dumps({"profile": "force", "N": N, "warm": warm, "reps": reps, "kernel_us": us,
                       "pair_once": pair_once, "nslots": nslots, "copy_bytes": nbytes}))
 
 
@@ -244,6 +296,12 @@ def main():
     batched = None if a.no_batch else batch_leg(a.batch_steps)
     adjoint = adjoint_leg()
     cpu = None if a.no_cpu else cpu_leg()
+    if cpu is not None:
+        cpu["all_cores"] = cpu_parallel_leg()
+    kname = "k_force_n3" if g["pair_once"] else "k_force"
+    sclk = profiled_sclk_ghz(kname)
+    # 256 CU x 64 FP32 lanes x 2 FLOP (FMA) per clock, non-packed
+    peak_sclk = 256 * 64 * 2 * sclk / 1e3 if sclk else None
     line = {
         "metric": METRIC,
         "value": value,
@@ -265,14 +323,18 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "frac_scalar_fp32": achieved / PEAK_FP32_SCALAR_TFLOPS,
+                     "sclk_ghz_measured": sclk,
+                     "peak_scalar_at_measured_sclk": peak_sclk,
+                     "frac_scalar_at_measured_sclk": achieved / peak_sclk if peak_sclk else None,
                      "kernel": ("k_force_n3 (each unordered pair once, periodic fixed point)"
                                 if g["pair_once"] else "k_force (sliced, periodic fixed point)"),
                      "kernel_us": g["force_us"],
-                     "traffic": profiled_traffic("k_force_n3" if g["pair_once"] else "k_force"),
+                     "traffic": profiled_traffic(kname),
                      "newton3": g["pair_once"],
                      "pair_evaluations_per_launch": (N * (N - 1) // 2 if g["pair_once"]
                                                      else N * N),
-                     "issue_slots_per_ordered_pair": 14.4 if g["pair_once"] else 22,
+                     # SQ_INSTS_VALU per ordered pair (13.19 measured) + half-rate v_rcp_f32
+                     "issue_slots_per_ordered_pair": 13.7 if g["pair_once"] else 22,
                      "sliced_kernel_us": sliced_us,
                      "sliced_kernel_tflops": FLOP_PER_PAIR * pairs / (sliced_us * 1e-6) / 1e12},
         "cpu_baseline": cpu,
