@@ -144,9 +144,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_pass(const uint4* __restrict__ q
   }
   if (i < N) {
     const size_t o = 3 * ((size_t)s * Npad + i);
-    part[o] = make_float4(acc.hx, acc.hy, acc.hz, acc.a);
-    part[o + 1] = make_float4(acc.fx, acc.fy, acc.fz, acc.c);
-    part[o + 2] = make_float4(acc.gx, acc.gy, acc.gz, acc.s);
+    store_part(part + o, acc.hx, acc.hy, acc.hz, acc.a);
+    store_part(part + o + 1, acc.fx, acc.fy, acc.fz, acc.c);
+    store_part(part + o + 2, acc.gx, acc.gy, acc.gz, acc.s);
   }
 }
 

@@ -44,8 +44,8 @@ __device__ __forceinline__ void batch_item(const uint4* __restrict__ q, const fl
   for (int k = 0; k < IPL; ++k) {
     if (ii[k] < N) {
       const size_t o = (size_t)bp.pbase + (size_t)s * bp.Npad + ii[k];
-      fpart[o] = make_float4(hi[k].fx, hi[k].fy, hi[k].fz, hi[k].b);
-      apart[o] = hi[k].a;
+      store_part(fpart + o, hi[k].fx, hi[k].fy, hi[k].fz, hi[k].b);
+      store_part(apart + o, hi[k].a);
     }
   }
 }

@@ -320,14 +320,14 @@ __device__ __forceinline__ void n3_item(const uint4* __restrict__ q, const float
     }
     const int i = kG * g + T;
     if (i < N) {
-      fpart[(size_t)h * Npad + i] = make_float4(v[0], v[1], v[2], ENERGY ? v[3] : 0.f);
-      if constexpr (ENERGY) apart[(size_t)h * Npad + i] = v[4];
+      store_part(fpart + (size_t)h * Npad + i, v[0], v[1], v[2], ENERGY ? v[3] : 0.f);
+      if constexpr (ENERGY) store_part(apart + (size_t)h * Npad + i, v[4]);
     }
     if (!diag) {
       const int j = kG * h + T;
       if (j < N) {
-        fpart[(size_t)g * Npad + j] = make_float4(jf[0], jf[1], jf[2], 0.f);
-        if constexpr (ENERGY) apart[(size_t)g * Npad + j] = jf[3];
+        store_part(fpart + (size_t)g * Npad + j, jf[0], jf[1], jf[2], 0.f);
+        if constexpr (ENERGY) store_part(apart + (size_t)g * Npad + j, jf[3]);
       }
     }
   }

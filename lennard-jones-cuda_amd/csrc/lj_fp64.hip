@@ -62,8 +62,8 @@ __global__ __launch_bounds__(kBlock) void k_force64(const double4* __restrict__ 
   }
   if (i < N) {
     const size_t o = (size_t)s * Npad + i;
-    part[o] = make_double4(fx, fy, fz, b);
-    apart[o] = a;
+    store_part(part + o, fx, fy, fz, b);
+    store_part(apart + o, a);
   }
 }
 

@@ -76,12 +76,12 @@ __global__ __launch_bounds__(kBlock) void k_force(const uint4* __restrict__ q,
     if (ii[k] < N) {
       const size_t o = (size_t)s * Npad + ii[k];
       if constexpr (COMP) {
-        fpart[o] = make_float4(hi[k].fx + lo[k].fx, hi[k].fy + lo[k].fy, hi[k].fz + lo[k].fz,
-                               hi[k].b + lo[k].b);
-        apart[o] = hi[k].a + lo[k].a;
+        store_part(fpart + o, hi[k].fx + lo[k].fx, hi[k].fy + lo[k].fy, hi[k].fz + lo[k].fz,
+                   hi[k].b + lo[k].b);
+        store_part(apart + o, hi[k].a + lo[k].a);
       } else {
-        fpart[o] = make_float4(hi[k].fx, hi[k].fy, hi[k].fz, hi[k].b);
-        apart[o] = hi[k].a;
+        store_part(fpart + o, hi[k].fx, hi[k].fy, hi[k].fz, hi[k].b);
+        store_part(apart + o, hi[k].a);
       }
     }
   }
