@@ -18,7 +18,7 @@ def _pair(N, rho, T0, canon, bc, pos, vel):
     g.set_state(pos, vel)
     g.CalculateForces()
     g.CalculateParameters()
-    o = OracleSystem(N, T0, rho, canon, bc, nthreads=8)
+    o = OracleSystem(N, T0, rho, canon, bc, nthreads=16)
     o.set_state(pos, vel)
     o.forces()
     o.params()
@@ -46,11 +46,15 @@ def test_initial_state_matches_oracle():
     (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_WALL, 60, 2e-4),       # config 2 (hard wall)
     (1000, 0.3, 1.0, ljamd.EVN, ljamd.BC_WALL, 100, 1e-4),
     (600, 0.2, 1.0, ljamd.EVN, ljamd.BC_NONE, 60, 1e-4),
+    # the pair-once pass with a partial last group of 256 (16 and 201 particles)
+    (10000, 0.85, 1.0, ljamd.TVN, ljamd.BC_PERIODIC, 10, 2e-6),
+    (13001, 0.85, 1.0, ljamd.EVN, ljamd.BC_WALL, 10, 2e-6),
 ])
 def test_trajectory_vs_reference(N, rho, T0, canon, bc, nsteps, tolU):
     L, pos, vel = lattice_state(N, rho, T0, seed=77)
     print(f"\n[case] traj N={N} canon={canon} bc={bc}", flush=True)
     g, o = _pair(N, rho, T0, canon, bc, pos, vel)
+    assert g.force_plan()[0] == (N >= 9472)
     try:
         hg = g.Integrate(0.005, nsteps)
         ho = o.integrate(0.005, nsteps)
