@@ -329,7 +329,10 @@ int lj_batch_get_stats(lj_batch* b, double* out /* nboxes * LJ_BATCH_NSTAT */);
 int lj_batch_reset_stats(lj_batch* b);
 int lj_batch_set_tuning(lj_batch* b, const lj_batch_tuning* t);
 int lj_batch_get_tuning(lj_batch* b, lj_batch_tuning* t);
-/* average time of one batched force pass (HIP events), microseconds */
+/* average time of one batched force pass (HIP events), microseconds: the
+ * time step's pass (with pair_once: force-only partials and per-box energy and
+ * virial totals; after lj_batch_integrate with pair_once the force array's w
+ * is 0 until the next lj_batch_compute_forces) */
 int lj_batch_time_force(lj_batch* b, int reps, double* us);
 /* Times every candidate tiling on this batch (reps force passes each), keeps
  * the fastest; cached in the file named by LJ_AUTOTUNE_CACHE when set. */

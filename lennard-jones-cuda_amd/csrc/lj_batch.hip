@@ -141,7 +141,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_drift(BatchArgs a)
 
 __device__ __forceinline__ Partial box_slices(const BatchArgs& a, const BoxParam& bp, int i)
 {
-  return sum_slices(a.fpart + bp.pbase, a.apart + bp.pbase, i, bp.S, bp.Npad);
+  if (a.lean)  // force-only partials of the time step's pair-once pass (a = b = 0)
+    return sum_slices<false>(a.fpart + bp.pbase, a.apart + bp.pbase, i, bp.S, bp.Npad);
+  return sum_slices<true>(a.fpart + bp.pbase, a.apart + bp.pbase, i, bp.S, bp.Npad);
 }
 
 __device__ __forceinline__ float inside(const float4& p, const BoxParam& bp)
@@ -291,6 +293,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_finalize(BatchArgs a, int adva
     v[2] += rows[(size_t)r * kBlkFields + BLK_B];
     v[3] += rows[(size_t)r * kBlkFields + BLK_NSUB];
   }
+  if (a.lean)  // the box's per-item totals of the step's force pass, in slot order
+    for (int k = threadIdx.x; k < bp.nitem; k += kBlock) {
+      v[1] += a.ew[2 * (size_t)(bp.ew0 + k)];
+      v[2] += a.ew[2 * (size_t)(bp.ew0 + k) + 1];
+    }
   block_allsum<4>(v);
   if (threadIdx.x != 0) return;
   lj_obs* o = a.obs + b;

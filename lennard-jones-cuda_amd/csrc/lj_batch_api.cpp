@@ -42,6 +42,8 @@ struct lj_batch {
   int2* pblock = nullptr;
   BatchItem* items = nullptr;
   size_t items_cap = 0;
+  double* ew = nullptr;  // pair-once: (E, W) per item, boxes' runs contiguous
+  size_t ew_cap = 0;
   unsigned* counter = nullptr;
   int nblocks = 0, nitems = 0;
   std::vector<BoxParam> hbox;
@@ -68,7 +70,9 @@ int bsetdev(lj_batch* b)
   return LJ_OK;
 }
 
-BatchArgs bargs(lj_batch* b, double dt)
+// lean: the time step's force pass (force-only partials and per-item totals
+// when the pair-once decomposition is tuned in)
+BatchArgs bargs(lj_batch* b, double dt, bool lean = false)
 {
   BatchArgs a{};
   a.pos = b->pos;
@@ -96,6 +100,8 @@ BatchArgs bargs(lj_batch* b, double dt)
   a.persistent_grid = b->tune.blocks_per_cu > 0 ? b->tune.blocks_per_cu * b->ncu : 0;
   a.pair_once = b->tune.pair_once;
   if (a.pair_once) a.persistent_grid = 0;
+  a.lean = lean && a.pair_once ? 1 : 0;
+  a.ew = b->ew;
   return a;
 }
 
@@ -139,8 +145,10 @@ int build_tables(lj_batch* b)
     p.sub_lo = 0.5 * b->L[k] - 0.5 * side;
     p.sub_hi = 0.5 * b->L[k] + 0.5 * side;
     if (b->tune.pair_once) {
+      p.ew0 = (int)items.size();
       for (int g = 0; g < p.S; ++g)
-        for (int h = g; h < p.S; ++h) items.push_back(BatchItem{k, g, h, 0});
+        for (int h = g; h < p.S; ++h) items.push_back(BatchItem{k, g, h, (int)items.size()});
+      p.nitem = (int)items.size() - p.ew0;
     } else {
       for (int t = 0; t < (N + tile - 1) / tile; ++t)
         for (int s = 0; s < p.S; ++s) items.push_back(BatchItem{k, t * tile, s, 0});
@@ -180,6 +188,12 @@ int build_tables(lj_batch* b)
     LJ_TRY(dev_alloc(&b->items, items.size()));
     b->items_cap = items.size();
   }
+  if (b->tune.pair_once && items.size() > b->ew_cap) {
+    dev_free(b->ew);
+    b->ew_cap = 0;
+    LJ_TRY(dev_alloc(&b->ew, 2 * items.size()));
+    b->ew_cap = items.size();
+  }
   if (!b->box) LJ_TRY(dev_alloc(&b->box, b->nboxes));
   if (!b->pblock) LJ_TRY(dev_alloc(&b->pblock, pblock.size()));
   if (!b->blk) LJ_TRY(dev_alloc(&b->blk, (size_t)blk0 * kBlkFields));
@@ -195,7 +209,7 @@ int build_tables(lj_batch* b)
 
 int enqueue_batch_step(lj_batch* b, hipStream_t st, double dt)
 {
-  const BatchArgs a = bargs(b, dt);
+  const BatchArgs a = bargs(b, dt, true);
   LJ_HIP(launch_batch_drift(st, a));
   LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC, false));
   if (b->canonical == LJ_EVN) {
@@ -208,9 +222,11 @@ int enqueue_batch_step(lj_batch* b, hipStream_t st, double dt)
   return LJ_OK;
 }
 
-int enqueue_batch_forces(lj_batch* b, hipStream_t st, bool ties)
+// lean: the time step's pass (with pair_once: force-only partials and
+// per-item totals), as timed by lj_batch_time_force and the autotuner
+int enqueue_batch_forces(lj_batch* b, hipStream_t st, bool ties, bool lean = false)
 {
-  const BatchArgs a = bargs(b, 0.);
+  const BatchArgs a = bargs(b, 0., lean);
   LJ_HIP(launch_batch_prep_q(st, a));  // also resets the queue head
   LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC, ties));
   return LJ_OK;
@@ -227,7 +243,7 @@ std::string tune_key(lj_batch* b)
     h *= 1099511628211ull;
   }
   std::ostringstream os;
-  os << "v2:" << dev << ":" << b->ncu << ":" << b->nboxes << ":" << b->boundary << ":" << std::hex << h;
+  os << "v3:" << dev << ":" << b->ncu << ":" << b->nboxes << ":" << b->boundary << ":" << std::hex << h;
   return os.str();
 }
 
@@ -330,6 +346,7 @@ int lj_batch_destroy(lj_batch* b)
   dev_free(b->box);
   dev_free(b->pblock);
   dev_free(b->items);
+  dev_free(b->ew);
   dev_free(b->counter);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
@@ -450,9 +467,9 @@ int lj_batch_time_force(lj_batch* b, int reps, double* us)
   LJ_TRY(bsetdev(b));
   if (reps <= 0 || !us) return fail(LJ_ERR_ARG, "lj_batch_time_force: bad arguments");
   hipStream_t st = b->stream;
-  LJ_TRY(enqueue_batch_forces(b, st, false));  // warm
+  LJ_TRY(enqueue_batch_forces(b, st, false, true));  // warm
   LJ_HIP(hipEventRecord(b->e0, st));
-  for (int k = 0; k < reps; ++k) LJ_TRY(enqueue_batch_forces(b, st, false));
+  for (int k = 0; k < reps; ++k) LJ_TRY(enqueue_batch_forces(b, st, false, true));
   LJ_HIP(hipEventRecord(b->e1, st));
   LJ_HIP(hipEventSynchronize(b->e1));
   float ms = 0.f;

@@ -358,16 +358,20 @@ __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8)))
 }
 
 // batched ensemble: item = (box, g, h) over all boxes; each box's partials at
-// its own rows (pbase, S = groups of 256, Npad), per-particle energies kept
-template <bool PERIODIC, bool TIES>
+// its own rows (pbase, S = groups of 256, Npad); ENERGY: per-particle energies
+// kept (single evaluations), else the time step's variant with the item's
+// (E, W) totals at ew[2 e]
+template <bool PERIODIC, bool TIES, bool ENERGY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_batch_force_n3(
     const uint4* __restrict__ q, const float4* __restrict__ pf, float4* __restrict__ fpart,
-    float* __restrict__ apart, const BoxParam* __restrict__ box, const BatchItem* __restrict__ items)
+    float* __restrict__ apart, const BoxParam* __restrict__ box, const BatchItem* __restrict__ items,
+    double* __restrict__ ew)
 {
   const BatchItem w = items[blockIdx.x];
   const BoxParam bp = box[w.box];
-  n3_item<PERIODIC, TIES, 2, true>(q + bp.off, pf + bp.off, fpart + bp.pbase, apart + bp.pbase,
-                                   nullptr, bp.N, bp.Npad, bp.C1, w.i0, w.s);
+  n3_item<PERIODIC, TIES, 2, ENERGY>(q + bp.off, pf + bp.off, fpart + bp.pbase, apart + bp.pbase,
+                                     ENERGY ? nullptr : ew + 2 * (size_t)w.e, bp.N, bp.Npad, bp.C1,
+                                     w.i0, w.s);
 }
 
 }  // namespace
@@ -458,16 +462,26 @@ hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& p
 hipError_t launch_batch_force_n3(hipStream_t st, const BatchArgs& a, bool periodic, bool ties)
 {
   dim3 grid(a.nitems), block(256);
+  if (a.lean) {  // the time step (no exact-tie rule there)
+    if (ties || !a.ew) return hipErrorInvalidValue;
+    if (periodic)
+      hipLaunchKernelGGL((k_batch_force_n3<true, false, false>), grid, block, 0, st, a.q, a.pos,
+                         a.fpart, a.apart, a.box, a.items, a.ew);
+    else
+      hipLaunchKernelGGL((k_batch_force_n3<false, false, false>), grid, block, 0, st, a.q, a.pos,
+                         a.fpart, a.apart, a.box, a.items, a.ew);
+    return hipGetLastError();
+  }
   if (periodic) {
     if (ties)
-      hipLaunchKernelGGL((k_batch_force_n3<true, true>), grid, block, 0, st, a.q, a.pos, a.fpart,
-                         a.apart, a.box, a.items);
+      hipLaunchKernelGGL((k_batch_force_n3<true, true, true>), grid, block, 0, st, a.q, a.pos,
+                         a.fpart, a.apart, a.box, a.items, a.ew);
     else
-      hipLaunchKernelGGL((k_batch_force_n3<true, false>), grid, block, 0, st, a.q, a.pos, a.fpart,
-                         a.apart, a.box, a.items);
+      hipLaunchKernelGGL((k_batch_force_n3<true, false, true>), grid, block, 0, st, a.q, a.pos,
+                         a.fpart, a.apart, a.box, a.items, a.ew);
   } else {
-    hipLaunchKernelGGL((k_batch_force_n3<false, false>), grid, block, 0, st, a.q, a.pos, a.fpart,
-                       a.apart, a.box, a.items);
+    hipLaunchKernelGGL((k_batch_force_n3<false, false, true>), grid, block, 0, st, a.q, a.pos,
+                       a.fpart, a.apart, a.box, a.items, a.ew);
   }
   return hipGetLastError();
 }

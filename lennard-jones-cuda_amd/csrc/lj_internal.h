@@ -161,6 +161,7 @@ struct BoxParam {
   int Npad;          // row pitch of the box's partial buffers
   int pbase;         // offset of the box's partial rows (elements)
   int blk0, nblk;    // first block row and number of block rows
+  int ew0, nitem;    // pair-once items of the box: their (E, W) totals at ew[2 (ew0 + k)]
   double L, qscale, FS, rho, T0;
   float C1, pad;
   double sub_lo, sub_hi;  // sub-volume cube [lo, hi]^3 of the fluctuation counts
@@ -171,7 +172,8 @@ struct BoxParam {
 constexpr int kPairOnceGroup = 256;  // group size of the batched pair-once pass
 
 struct BatchItem {
-  int box, i0, s, pad;
+  int box, i0, s;
+  int e;  // pair-once: the item's slot in the per-box runs of (E, W) totals
 };
 
 // per-box accumulated fluctuation statistics (LJ_BATCH_NSTAT doubles per box)
@@ -200,6 +202,12 @@ struct BatchArgs {
   int bc, canonical;
   int ipl, persistent_grid;
   int pair_once;        // items are (box, g, h) group pairs of 256 (k_batch_force_n3)
+  // lean (the time step with pair_once): the force pass writes force-only
+  // partials and per-item totals ew[2 e] = sum of e, ew[2 e + 1] = sum of w over
+  // ordered pairs; the kick kernels then read no apart, and the finaliser adds
+  // each box's item totals
+  int lean;
+  double* ew;
 };
 hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic, bool ties);

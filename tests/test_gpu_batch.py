@@ -86,6 +86,41 @@ def test_batch_trajectory_matches_single_box(canon):
         assert du <= 1e-4 and dx <= 1e-3, (k, du, dx)
 
 
+@pytest.mark.parametrize("canon,bc", [(ljamd.EVN, ljamd.BC_PERIODIC), (ljamd.TVN, ljamd.BC_WALL)])
+def test_batch_pair_once_step_totals(canon, bc):
+    """With the pair-once tuning the step's force pass keeps no per-particle
+    energies: each item writes its E and W totals and the box's finaliser adds
+    them.  V and W of the step equal a full evaluation (per-particle energies)
+    at the same positions; Force.w reads 0 after the step; the trajectory
+    follows the sliced tuning's."""
+    N, rho, T0 = _small_batch(nb=6, seed=17)
+    print(f"\n[case] batch pair-once step canon={canon} bc={bc}", flush=True)
+    runs = {}
+    for tuning in [(1, 512, 0, 1), (1, 512, 0, 0)]:
+        with ljamd.LJBatch(N, rho, T0, canonical=canon, boundary=bc, seed=5) as b:
+            b.set_tuning(*tuning)
+            b.integrate(0.005, 20)
+            o_step = b.observables()
+            f_step = [b.get_array(ljamd.ARRAY_FORCE, box=k) for k in range(len(N))]
+            p_step = [b.get_array(ljamd.ARRAY_POS, box=k) for k in range(len(N))]
+            b.compute_forces()  # full evaluation at the positions the step ended with
+            o_full = b.observables()
+            f_full = [b.get_array(ljamd.ARRAY_FORCE, box=k) for k in range(len(N))]
+        runs[tuning] = (o_step, p_step)
+        for k in range(len(N)):
+            # the step's force pass saw these positions: the kick after it only
+            # wraps them (periodic) or flips velocities (wall)
+            assert rel(o_step[k]["V"], o_full[k]["V"]) <= 1e-6, (tuning, k)
+            assert rel(o_step[k]["W"], o_full[k]["W"]) <= 1e-6, (tuning, k)
+            if tuning[3]:
+                assert np.all(f_step[k].reshape(-1, 4)[:, 3] == 0.0)
+                assert np.any(f_full[k].reshape(-1, 4)[:, 3] != 0.0)
+    for k in range(len(N)):
+        a, c = runs[(1, 512, 0, 1)], runs[(1, 512, 0, 0)]
+        assert abs(a[0][k]["U"] - c[0][k]["U"]) / N[k] <= 1e-5, k
+        assert np.max(np.abs(xyz(a[1][k]) - xyz(c[1][k]))) <= 1e-4, k
+
+
 def test_batch_fluctuation_stats():
     N, rho, T0 = _small_batch(nb=8, seed=13)
     frac = 0.3
