@@ -178,6 +178,24 @@ def cpu_leg(budget_s=15.0):
             "steps_per_s": steps / tt}
 
 
+def rdf_leg(reps=20):
+    """The RDF pass (integer work, reported apart from the FLOP count) at the
+    headline size: 500 bins uniform in r to L/2 (the workload's sampling) and
+    the reference's 256 bins in r^2 with its float binning."""
+    import ljamd
+
+    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED) as s:
+        s.Integrate(DT, 20)
+        s.synchronize()
+        us_r = s.time_rdf(ljamd.RDF_RBINS, 500, reps=reps)
+        us_g = s.time_rdf(ljamd.RDF_REF_GPU, 256, reps=reps)
+        us_c = s.time_rdf(ljamd.RDF_REF_CPU, 256, reps=reps)
+    pairs = float(N) * N
+    return {"rbins_500_to_half_box_us": us_r, "rbins_pairs_per_s": pairs / (us_r * 1e-6),
+            "ref_gpu_256_r2_bins_us": us_g, "ref_gpu_pairs_per_s": pairs / (us_g * 1e-6),
+            "ref_cpu_256_r2_bins_us": us_c, "ref_cpu_pairs_per_s": pairs / (us_c * 1e-6)}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -295,6 +313,7 @@ def main():
             "force_tflops": FLOP_PER_PAIR * pairs / (gw["force_us"] * 1e-6) / 1e12}
     batched = None if a.no_batch else batch_leg(a.batch_steps)
     adjoint = adjoint_leg()
+    rdf = rdf_leg()
     cpu = None if a.no_cpu else cpu_leg()
     if cpu is not None:
         cpu["all_cores"] = cpu_parallel_leg()
@@ -341,6 +360,7 @@ def main():
         "hard_wall": wall,
         "batched": batched,
         "adjoint": adjoint,
+        "rdf": rdf,
         "final_T": g["obs"]["T"],
         "final_u": g["obs"]["U"] / N,
     }

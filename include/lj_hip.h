@@ -204,6 +204,11 @@ int lj_integrate_async(lj_ctx* ctx, double dt, int nsteps);
 int lj_rdf(lj_ctx* ctx, int rule, int nbins, float dr2, double rmax, uint64_t* counts,
            int accumulate);
 
+/* Average time of one RDF pass over the current positions (HIP events,
+ * microseconds), arguments as lj_rdf; the device histogram is left holding
+ * the timed passes, so the next lj_rdf call starts from zero. */
+int lj_time_rdf(lj_ctx* ctx, int rule, int nbins, float dr2, double rmax, int reps, double* us);
+
 /* Speed histogram of the current velocities as MDSystem::initvelo /
  * updatevelo bin them (MDSystem.cpp:651-694): counts[b] = number of particles
  * with (int)(sqrt((double)fl32(vx*vx + vy*vy + vz*vz)) / step) == b, for

@@ -593,13 +593,17 @@ int lj_integrate_async(lj_ctx* c, double dt, int nsteps)
   return LJ_OK;
 }
 
-int lj_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, uint64_t* counts,
-           int accumulate)
+}  // extern "C"
+
+namespace lj {
+// validates an RDF request, sizes the context's histogram and edge table and
+// fills the kernel arguments; the histogram is cleared unless accumulating
+// into the same binning
+int prepare_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, int accumulate,
+                const char* who, RdfArgs* out)
 {
-  LJ_TRY(ctx_setdev(c));
-  if (rule < LJ_RDF_REF_CPU || rule > LJ_RDF_RBINS) return fail(LJ_ERR_ARG, "lj_rdf: unknown rule %d", rule);
-  if (nbins <= 0 || nbins > 8000) return fail(LJ_ERR_ARG, "lj_rdf: nbins must be in [1, 8000]");
-  if (!counts) return fail(LJ_ERR_ARG, "lj_rdf: null counts");
+  if (rule < LJ_RDF_REF_CPU || rule > LJ_RDF_RBINS) return fail(LJ_ERR_ARG, "%s: unknown rule %d", who, rule);
+  if (nbins <= 0 || nbins > 8000) return fail(LJ_ERR_ARG, "%s: nbins must be in [1, 8000]", who);
   if (!(dr2 > 0.f)) dr2 = rdf_dr2_rule(c->cfg.N);
   if (!(rmax > 0.)) rmax = c->L / 2.;
   hipStream_t st = c->stream;
@@ -624,26 +628,62 @@ int lj_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, uint64_t* cou
   a.L = c->L;
   a.rule = rule;
   a.nbins = nbins;
-  a.dr2 = dr2;
   a.hist = c->rdf;
-  cpu_minimage_thresholds(c->L, a.thr);
-  if (rule == LJ_RDF_RBINS) {
-    std::vector<float> e(nbins + 1);
-    rdf_edges2(nbins, rmax, e.data());
-    if (nbins + 1 > c->edges_cap) {
-      LJ_HIP(hipStreamSynchronize(st));
-      dev_free(c->edges2);
-      c->edges_cap = 0;
-      LJ_TRY(dev_alloc(&c->edges2, nbins + 1));
-      c->edges_cap = nbins + 1;
-    }
-    LJ_HIP(hipMemcpyAsync(c->edges2, e.data(), sizeof(float) * (nbins + 1), hipMemcpyHostToDevice, st));
-    a.edges2 = c->edges2;
-    a.rbin_inv = (float)(nbins / rmax);
+  if (rule == LJ_RDF_REF_GPU) gpu_minimage_thresholds(c->L, a.thr);
+  else cpu_minimage_thresholds(c->L, a.thr);
+  a.est_scale = rule == LJ_RDF_RBINS ? (float)(nbins / rmax) : (float)(1.0 / dr2);
+  std::vector<float> e(nbins + 1);
+  rdf_bin_thresholds(rule, nbins, dr2, rmax, e.data());
+  a.far = rdf_far_test(a.periodic != 0, c->L, rule == LJ_RDF_REF_GPU ? e[nbins - 1] : e[nbins]);
+  if (nbins + 1 > c->edges_cap) {
+    LJ_HIP(hipStreamSynchronize(st));
+    dev_free(c->edges2);
+    c->edges_cap = 0;
+    LJ_TRY(dev_alloc(&c->edges2, nbins + 1));
+    c->edges_cap = nbins + 1;
   }
-  LJ_HIP(launch_rdf(st, a));
-  LJ_HIP(hipMemcpyAsync(counts, c->rdf, sizeof(uint64_t) * nbins, hipMemcpyDeviceToHost, st));
+  // synchronous: the host table goes out of scope on return
+  LJ_HIP(hipMemcpyAsync(c->edges2, e.data(), sizeof(float) * (nbins + 1), hipMemcpyHostToDevice, st));
   LJ_HIP(hipStreamSynchronize(st));
+  a.tab = c->edges2;
+  *out = a;
+  return LJ_OK;
+}
+}  // namespace lj
+
+extern "C" {
+
+int lj_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, uint64_t* counts,
+           int accumulate)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!counts) return fail(LJ_ERR_ARG, "lj_rdf: null counts");
+  RdfArgs a{};
+  LJ_TRY(prepare_rdf(c, rule, nbins, dr2, rmax, accumulate, "lj_rdf", &a));
+  hipStream_t st = c->stream;
+  LJ_HIP(launch_rdf(st, a));
+  LJ_HIP(hipMemcpyAsync(counts, c->rdf, sizeof(uint64_t) * a.nbins, hipMemcpyDeviceToHost, st));
+  LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_time_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, int reps, double* us)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (reps <= 0 || !us) return fail(LJ_ERR_ARG, "lj_time_rdf: bad arguments");
+  RdfArgs a{};
+  LJ_TRY(prepare_rdf(c, rule, nbins, dr2, rmax, 0, "lj_time_rdf", &a));
+  hipStream_t st = c->stream;
+  LJ_HIP(launch_rdf(st, a));  // warm
+  LJ_HIP(hipEventRecord(c->e0, st));
+  for (int k = 0; k < reps; ++k) LJ_HIP(launch_rdf(st, a));
+  LJ_HIP(hipEventRecord(c->e1, st));
+  LJ_HIP(hipEventSynchronize(c->e1));
+  float ms = 0.f;
+  LJ_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+  *us = 1000.0 * ms / reps;
+  // the histogram now holds reps + 1 passes: the next lj_rdf starts afresh
+  c->rdf_rule = -1;
   return LJ_OK;
 }
 
@@ -827,6 +867,8 @@ struct CompatScratch {
   double* blk = nullptr;
   unsigned long long* rdf = nullptr;
   float* pressure = nullptr;
+  float* rdf_tab = nullptr;  // bin thresholds of the GPU rule for tab_dr2
+  float tab_dr2 = 0.f, tab_last = 0.f;
 };
 
 std::mutex g_compat_mu;
@@ -850,6 +892,7 @@ void compat_release(CompatScratch& s)
   dev_free(s.blk);
   dev_free(s.rdf);
   dev_free(s.pressure);
+  dev_free(s.rdf_tab);
   if (s.st) (void)hipStreamDestroy(s.st);
   s = CompatScratch();
 }
@@ -878,6 +921,7 @@ int compat_scratch(int N, const ForcePlan& pl, CompatScratch** out)
     s.capPart = part;
   }
   if (!s.rdf) LJ_TRY(dev_alloc(&s.rdf, 256));
+  if (!s.rdf_tab) LJ_TRY(dev_alloc(&s.rdf_tab, 257));
   if (!s.pressure) LJ_TRY(dev_alloc(&s.pressure, 1));
   *out = &s;
   return LJ_OK;
@@ -903,6 +947,14 @@ int compat_forces(float* Pos, float* Force, float* host_pressure, int N, float h
   LJ_HIP(launch_finalize_forces(st, s->blk, (N + kBlock - 1) / kBlock, 1., nullptr, s->pressure));
   if (host_RDF) {
     LJ_HIP(launch_fill_u64(st, s->rdf, 256));
+    if (s->tab_dr2 != host_dr2) {
+      float tab[257];
+      rdf_bin_thresholds(LJ_RDF_REF_GPU, 256, host_dr2, 0., tab);
+      LJ_HIP(hipMemcpyAsync(s->rdf_tab, tab, sizeof(tab), hipMemcpyHostToDevice, st));
+      LJ_HIP(hipStreamSynchronize(st));  // the host table goes out of scope
+      s->tab_dr2 = host_dr2;
+      s->tab_last = tab[255];
+    }
     RdfArgs a{};
     a.pos = p;
     a.N = N;
@@ -910,7 +962,10 @@ int compat_forces(float* Pos, float* Force, float* host_pressure, int N, float h
     a.L = (double)host_L;
     a.rule = LJ_RDF_REF_GPU;
     a.nbins = 256;
-    a.dr2 = host_dr2;
+    a.tab = s->rdf_tab;
+    a.est_scale = 1.f / host_dr2;
+    gpu_minimage_thresholds((double)host_L, a.thr);
+    a.far = rdf_far_test(a.periodic != 0, (double)host_L, s->tab_last);
     a.hist = s->rdf;
     LJ_HIP(launch_rdf(st, a));
   }

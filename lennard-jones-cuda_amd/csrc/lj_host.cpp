@@ -1,6 +1,7 @@
 // lj_host.cpp -- host-side helpers of libljhip.so: initial states of the
 // benchmark configurations, box rules and the exact binning tables of the RDF.
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -133,6 +134,61 @@ void rdf_edges2(int nbins, double rmax, float* edges2)
     const double e = (double)k * dr;
     edges2[k] = (float)(e * e);
   }
+}
+
+namespace {
+// the least float x >= 0 with pred(x), for a predicate non-decreasing in x
+// (bisection on the ordered bit patterns of non-negative floats)
+template <class P>
+float first_float(float guess, P pred)
+{
+  if (pred(0.f)) return 0.f;
+  float h = guess > 0.f ? guess : 1.f;
+  while (!pred(h) && h < 3.0e38f) h *= 2.f;
+  if (!pred(h)) return std::numeric_limits<float>::infinity();
+  uint32_t lo = fbits(0.f), hi = fbits(h);
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (pred(bitsf(mid))) hi = mid;
+    else lo = mid;
+  }
+  return bitsf(hi);
+}
+}  // namespace
+
+bool rdf_far_test(bool periodic, double L, double rmax2)
+{
+  if (!periodic || !(L > 0.) || !(rmax2 >= 0.)) return false;
+  const double r = std::sqrt(rmax2);
+  if (r >= 0.5 * L) return false;
+  return 4.18879020478639098 * r * r * r < 0.3 * L * L * L;  // (4/3) pi r^3 < 0.3 L^3
+}
+
+void gpu_minimage_thresholds(double L, float thr[4])
+{
+  // k(x) = lround(x / L) in float with a correctly rounded division
+  // (MDSystem.cu:31-33): non-decreasing, odd in x
+  const float Lf = (float)L;
+  for (int m = 1; m <= 4; ++m)
+    thr[m - 1] = first_float((float)(m + 1) * Lf, [&](float x) { return std::round(x / Lf) >= (float)m; });
+}
+
+void rdf_bin_thresholds(int rule, int nbins, float dr2, double rmax, float* T)
+{
+  if (rule == LJ_RDF_RBINS) {
+    rdf_edges2(nbins, rmax, T);
+    return;
+  }
+  T[0] = 0.f;
+  const double d2 = dr2;
+  for (int k = 1; k <= nbins; ++k) {
+    if (rule == LJ_RDF_REF_CPU)  // floor((double)r2 / (double)dr2) (MDSystem.cpp:282)
+      T[k] = first_float((float)((k + 1) * d2),
+                         [&](float x) { return std::floor((double)x / d2) >= (double)k; });
+    else  // floor of the float quotient (MDSystem.cu:39)
+      T[k] = first_float((float)(k + 1) * dr2, [&](float x) { return std::floor(x / dr2) >= (float)k; });
+  }
+  if (rule == LJ_RDF_REF_GPU) T[nbins] = std::numeric_limits<float>::infinity();  // clamped, never dropped
 }
 
 }  // namespace lj

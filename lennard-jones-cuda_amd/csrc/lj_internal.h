@@ -139,12 +139,16 @@ struct RdfArgs {
   double L;
   int rule;
   int nbins;
-  float dr2;
-  const float* edges2;  // RBINS
-  float rbin_inv;       // nbins / rmax
-  float thr[4];         // REF_CPU minimum-image thresholds
+  const float* tab;  // rdf_bin_thresholds (nbins + 1)
+  float est_scale;   // bin estimate: r^2 est_scale (REF rules), sqrt(r^2) est_scale (RBINS)
+  float thr[4];      // minimum-image thresholds of the rule
+  int far;           // reject far pairs first (rdf_far_test)
   unsigned long long* hist;
 };
+// whether the RDF pass should reject far pairs by a fixed-point test first:
+// periodic boxes whose counted range (r^2 < rmax2) holds under 30% of the
+// pairs (the sphere of radius sqrt(rmax2) against the box volume)
+bool rdf_far_test(bool periodic, double L, double rmax2);
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);
 // speed histogram (MDSystem.cpp:651-694); hist is added to (clear it first)
@@ -228,6 +232,12 @@ double box_length(int N, double rho);
 float rdf_dr2_rule(int N);
 // smallest positive float x with fast_round((float)(x / L)) >= m, m = 1..4
 void cpu_minimage_thresholds(double L, float thr[4]);
+// the same for the GPU rule: thr[m-1] = least x >= 0 with lround(x / L) >= m (float)
+void gpu_minimage_thresholds(double L, float thr[4]);
 void rdf_edges2(int nbins, double rmax, float* edges2);
+// per-bin r^2 thresholds of an RDF rule: T[k] = the least float r^2 whose bin
+// is >= k (k = 0..nbins, T[0] = 0); r^2 >= T[nbins] is dropped (REF_CPU,
+// RBINS) or, REF_GPU, T[nbins] = +inf (clamped into the last bin)
+void rdf_bin_thresholds(int rule, int nbins, float dr2, double rmax, float* T);
 
 }  // namespace lj

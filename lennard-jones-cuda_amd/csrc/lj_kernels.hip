@@ -393,75 +393,165 @@ __device__ __forceinline__ float mi_cpu(float r, double L, const float (&t)[4])
   return (float)((double)r - (double)k * L);
 }
 
-__device__ __forceinline__ float mi_gpu(float d, float Lf)
+__device__ __forceinline__ float mi_gpu(float d, float Lf, const float (&t)[4])
 {
 #pragma clang fp contract(off)
-  // MDSystem.cu:31-33: r += -L * lround(r / L) with a correctly rounded division
-  const float k = __builtin_roundf(d / Lf);
-  return d - Lf * k;
+  // MDSystem.cu:31-33: r += -L * lround(r / L) with a correctly rounded
+  // division; k = lround(r / L) as a step function of r with host-computed
+  // thresholds t[m-1] = min{x : k(x) >= m}
+  int k = (d >= t[0]) + (d >= t[1]) + (d >= t[2]) + (d >= t[3]);
+  k -= (d <= -t[0]) + (d <= -t[1]) + (d <= -t[2]) + (d <= -t[3]);
+  return d - Lf * (float)k;
 }
 
-template <int RULE>
-__global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, int N,
-                                                int jchunk, int periodic, double L, float dr2,
-                                                int nbins, const float* __restrict__ edges2,
-                                                float rbin_inv, float t0, float t1, float t2,
-                                                float t3, unsigned long long* __restrict__ hist)
+// bin of r2 in the threshold table T (T[k] = least r^2 of bin >= k): the
+// largest k < nbins with T[k] <= r2, from an estimate corrected on the table
+__device__ __forceinline__ int rdf_bin(float r2, const float* T, int nbins, int est)
+{
+  int k = min(max(est, 0), nbins - 1);
+  while (k > 0 && T[k] > r2) --k;
+  while (k + 1 < nbins && T[k + 1] <= r2) ++k;
+  return k;
+}
+
+// RDF pass.  Every binning rule is symmetric in i and j (the coordinate
+// differences of the two orders are exact negatives, the image shift rounds
+// half away from zero, r^2 squares them), so each unordered pair is binned
+// once and counted twice: the ordered-pair counts of the reference, exactly.
+// Work item (g, h), h >= g, over groups of kBlock particles (grid ng x ng;
+// blocks with h < g exit at once); one i per thread.  SMALL (nbins <=
+// kRdfWaveCopy): one LDS histogram per wave and group h staged in LDS; else one
+// histogram per block and the j particles read from global memory, so that
+// the LDS stays within 64 KiB at the largest nbins.  Bin divisions and the
+// image rounding are table lookups (host-bisected thresholds), with the
+// reference's float/double arithmetic reproduced.
+constexpr int kRdfWaveCopy = 1024;
+
+// FAR (periodic, SMALL): a pair whose squared distance from the exact
+// fixed-point minimum image (the force pass's, q = x 2^32 / L) lies beyond the
+// counted range by a margin is dropped (REF_CPU, RBINS) or put in the last bin
+// (REF_GPU) without the reference arithmetic.  The two squared distances agree
+// to about 1e-6 relative plus 1e-6 L^2 (float positions, one fixed-point unit
+// L / 2^32 per axis); the margin is 1e-4 relative plus 1e-5 L^2, so no pair
+// changes bin; pairs inside or near the range take the exact path.
+// The lanes of a wave pair different i with one j, so the exact path runs
+// whenever one lane needs it: FAR pays off only when few pairs are in range
+// (the launcher enables it below a 30% in-range volume fraction).
+template <int RULE, bool PERIODIC, bool SMALL, bool FAR>
+__global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, int N, double L,
+                                                int nbins, const float* __restrict__ tab,
+                                                float est_scale, float t0, float t1, float t2,
+                                                float t3, double qscale,
+                                                unsigned long long* __restrict__ hist)
 {
 #pragma clang fp contract(off)
-  extern __shared__ unsigned int sh_rdf[];
-  unsigned int* h = sh_rdf;
-  float* e2 = reinterpret_cast<float*>(sh_rdf + nbins);
-  for (int b = threadIdx.x; b < nbins; b += kBlock) h[b] = 0u;
-  if constexpr (RULE == LJ_RDF_RBINS)
-    for (int b = threadIdx.x; b <= nbins; b += kBlock) e2[b] = edges2[b];
-  __syncthreads();
-  const float Lf = (float)L;
-  const float thr[4] = {t0, t1, t2, t3};
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = i < N;
-  const float4 pi = pos[min(i, N - 1)];
-  const int j0 = blockIdx.y * jchunk;
-  const int j1 = min(N, j0 + jchunk);
-  for (int j = j0; j < j1; ++j) {
-    const float4 pj = pos[j];
-    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-    if constexpr (RULE == LJ_RDF_REF_GPU) {
-      if (periodic) {
-        rx = mi_gpu(rx, Lf);
-        ry = mi_gpu(ry, Lf);
-        rz = mi_gpu(rz, Lf);
-      }
-      const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-      if (valid && r2 > 1e-6f) {
-        int b = (int)__builtin_floorf(r2 / dr2);
-        b = min(b, nbins - 1);
-        atomicAdd(&h[b], 1u);
-      }
-    } else {
-      if (periodic) {
-        rx = mi_cpu(rx, L, thr);
-        ry = mi_cpu(ry, L, thr);
-        rz = mi_cpu(rz, L, thr);
-      }
-      const float r2 = rx * rx + ry * ry + rz * rz;  // float, no contraction (MDSystem.cpp:279)
-      if (!valid || j == i) continue;
-      if constexpr (RULE == LJ_RDF_REF_CPU) {
-        const double qq = __builtin_floor((double)r2 / (double)dr2);  // MDSystem.cpp:282
-        if (qq < (double)nbins) atomicAdd(&h[(int)qq], 1u);
-      } else {
-        if (!(r2 < e2[nbins])) continue;
-        int k = (int)(__builtin_sqrtf(r2) * rbin_inv);
-        k = min(max(k, 0), nbins - 1);
-        while (k > 0 && e2[k] > r2) --k;
-        while (k + 1 < nbins && e2[k + 1] <= r2) ++k;
-        atomicAdd(&h[k], 1u);
-      }
+  static_assert(!FAR || (PERIODIC && SMALL), "FAR: periodic boxes, small histograms");
+  const int g = blockIdx.x, hgrp = blockIdx.y;
+  if (hgrp < g) return;  // uniform, before any barrier
+  extern __shared__ float4 sh_rdf4[];
+  constexpr int ncopy = SMALL ? kWaves : 1;
+  float4* pj_s = sh_rdf4;                                   // [kBlock] when SMALL
+  uint4* qj_s = reinterpret_cast<uint4*>(sh_rdf4 + kBlock);  // [kBlock] when FAR
+  unsigned int* hs = reinterpret_cast<unsigned int*>(sh_rdf4 + (SMALL ? kBlock : 0) + (FAR ? kBlock : 0));
+  float* T = reinterpret_cast<float*>(hs + ncopy * nbins);
+  unsigned int* hw = hs + (SMALL ? (int)(threadIdx.x >> 6) * nbins : 0);
+  for (int b = threadIdx.x; b < ncopy * nbins; b += kBlock) hs[b] = 0u;
+  for (int b = threadIdx.x; b <= nbins; b += kBlock) T[b] = tab[b];
+  const int j0 = hgrp * kBlock;
+  const int nj = min(kBlock, N - j0);
+  if constexpr (SMALL) {
+    if ((int)threadIdx.x < nj) {
+      const float4 p = pos[j0 + threadIdx.x];
+      pj_s[threadIdx.x] = p;
+      if constexpr (FAR)
+        qj_s[threadIdx.x] = make_uint4(fixq(p.x, qscale), fixq(p.y, qscale), fixq(p.z, qscale), 0u);
     }
   }
+  const float4* PJ = SMALL ? pj_s : pos + j0;
   __syncthreads();
-  for (int b = threadIdx.x; b < nbins; b += kBlock)
-    if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
+
+  const int i = g * kBlock + threadIdx.x;
+  const bool valid = i < N;
+  const float4 pi = pos[min(i, N - 1)];
+  const bool diag = g == hgrp;
+  const float Lf = (float)L;
+  const float thr[4] = {t0, t1, t2, t3};
+  const float Tmax = T[nbins];
+  const float Tlast = T[nbins - 1];
+  uint4 qi = make_uint4(0u, 0u, 0u, 0u);
+  if constexpr (FAR) qi = make_uint4(fixq(pi.x, qscale), fixq(pi.y, qscale), fixq(pi.z, qscale), 0u);
+  const float qs2 = (float)((L / 4294967296.) * (L / 4294967296.));  // (fixed-point unit)^2
+  const float far = (float)((double)(RULE == LJ_RDF_REF_GPU ? Tlast : Tmax) * (1. + 1e-4) + 1e-5 * L * L);
+  // pairs of the last bin are counted in a register: with the GPU rule every
+  // pair beyond the range is clamped into it, which would otherwise put most
+  // lanes of a wave on one LDS address
+  unsigned int last = 0u;
+  for (int jj = 0; jj < nj; ++jj) {
+    if (!valid || (diag && jj <= (int)threadIdx.x)) continue;
+    if constexpr (FAR) {
+      const uint4 qj = qj_s[jj];
+      const float dx = (float)(int)(qi.x - qj.x), dy = (float)(int)(qi.y - qj.y),
+                  dz = (float)(int)(qi.z - qj.z);
+      const float r2q = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) * qs2;
+      if (r2q > far) {
+        if constexpr (RULE == LJ_RDF_REF_GPU) ++last;
+        continue;
+      }
+    }
+    const float4 pj = PJ[jj];
+    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    if constexpr (PERIODIC) {
+      // |r| < thr[1] on every axis (any position within half a box of the box):
+      // the image index is -1, 0 or 1, from the first threshold alone
+      const float am = fmaxf(fabsf(rx), fmaxf(fabsf(ry), fabsf(rz)));
+      const float one[4] = {thr[0], 3.4e38f, 3.4e38f, 3.4e38f};
+      if constexpr (RULE == LJ_RDF_REF_GPU) {
+        if (am < thr[1]) {
+          rx = mi_gpu(rx, Lf, one);
+          ry = mi_gpu(ry, Lf, one);
+          rz = mi_gpu(rz, Lf, one);
+        } else {
+          rx = mi_gpu(rx, Lf, thr);
+          ry = mi_gpu(ry, Lf, thr);
+          rz = mi_gpu(rz, Lf, thr);
+        }
+      } else {
+        if (am < thr[1]) {
+          rx = mi_cpu(rx, L, one);
+          ry = mi_cpu(ry, L, one);
+          rz = mi_cpu(rz, L, one);
+        } else {
+          rx = mi_cpu(rx, L, thr);
+          ry = mi_cpu(ry, L, thr);
+          rz = mi_cpu(rz, L, thr);
+        }
+      }
+    }
+    float r2;
+    if constexpr (RULE == LJ_RDF_REF_GPU) {
+      r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));  // MDSystem.cu:35
+      if (!(r2 > 1e-6f)) continue;                                     // MDSystem.cu:37
+    } else {
+      r2 = rx * rx + ry * ry + rz * rz;  // float, no contraction (MDSystem.cpp:279)
+      if (!(r2 < Tmax)) continue;        // beyond the last bin: dropped
+    }
+    if (r2 >= Tlast) {
+      ++last;
+      continue;
+    }
+    const float e = RULE == LJ_RDF_RBINS ? __builtin_sqrtf(r2) * est_scale : r2 * est_scale;
+    atomicAdd(&hw[rdf_bin(r2, T, nbins, (int)e)], 2u);
+  }
+  // the wave's last-bin count: one LDS atomic per wave
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) last += __shfl_xor(last, o, 64);
+  if ((threadIdx.x & 63) == 0 && last) atomicAdd(&hw[nbins - 1], 2u * last);
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += kBlock) {
+    unsigned int c = 0u;
+    for (int k = 0; k < ncopy; ++k) c += hs[k * nbins + b];
+    if (c) atomicAdd(&hist[b], (unsigned long long)c);
+  }
 }
 
 // speed histogram of MDSystem::initvelo / updatevelo (MDSystem.cpp:651-694):
@@ -722,20 +812,30 @@ hipError_t launch_reset_clock(hipStream_t st, lj_obs* obs)
 
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a)
 {
-  const int gx = nblocks(a.N);
-  int S = max(1, min(2048 / max(gx, 1), (a.N + 255) / 256));
-  int jc = (a.N + S - 1) / S;
-  S = (a.N + jc - 1) / jc;
-  const size_t lds = sizeof(unsigned) * a.nbins + (a.rule == LJ_RDF_RBINS ? sizeof(float) * (a.nbins + 1) : 0);
-  dim3 grid(gx, S), blk(kBlock);
-#define LJ_RDF_ARGS a.pos, a.N, jc, a.periodic, a.L, a.dr2, a.nbins, a.edges2, a.rbin_inv, \
-                    a.thr[0], a.thr[1], a.thr[2], a.thr[3], a.hist
-  if (a.rule == LJ_RDF_REF_CPU)
-    hipLaunchKernelGGL(k_rdf<LJ_RDF_REF_CPU>, grid, blk, lds, st, LJ_RDF_ARGS);
-  else if (a.rule == LJ_RDF_REF_GPU)
-    hipLaunchKernelGGL(k_rdf<LJ_RDF_REF_GPU>, grid, blk, lds, st, LJ_RDF_ARGS);
-  else
-    hipLaunchKernelGGL(k_rdf<LJ_RDF_RBINS>, grid, blk, lds, st, LJ_RDF_ARGS);
+  const int ng = nblocks(a.N);
+  const bool small = a.nbins <= kRdfWaveCopy;
+  const bool far = small && a.periodic && a.far;
+  const size_t lds = (small ? sizeof(float4) * kBlock + sizeof(unsigned) * kWaves * a.nbins
+                            : sizeof(unsigned) * a.nbins) +
+                     (far ? sizeof(uint4) * kBlock : 0) + sizeof(float) * (a.nbins + 1);
+  const dim3 grid(ng, ng), blk(kBlock);
+#define LJ_RDF_ARGS a.pos, a.N, a.L, a.nbins, a.tab, a.est_scale, a.thr[0], a.thr[1], a.thr[2], \
+                    a.thr[3], 4294967296. / a.L, a.hist
+#define LJ_RDF_LAUNCH(R)                                                                             \
+  do {                                                                                               \
+    if (far) hipLaunchKernelGGL((k_rdf<R, true, true, true>), grid, blk, lds, st, LJ_RDF_ARGS);     \
+    else if (small) {                                                                                \
+      if (a.periodic) hipLaunchKernelGGL((k_rdf<R, true, true, false>), grid, blk, lds, st, LJ_RDF_ARGS); \
+      else hipLaunchKernelGGL((k_rdf<R, false, true, false>), grid, blk, lds, st, LJ_RDF_ARGS);     \
+    } else {                                                                                         \
+      if (a.periodic) hipLaunchKernelGGL((k_rdf<R, true, false, false>), grid, blk, lds, st, LJ_RDF_ARGS); \
+      else hipLaunchKernelGGL((k_rdf<R, false, false, false>), grid, blk, lds, st, LJ_RDF_ARGS);    \
+    }                                                                                                \
+  } while (0)
+  if (a.rule == LJ_RDF_REF_CPU) LJ_RDF_LAUNCH(LJ_RDF_REF_CPU);
+  else if (a.rule == LJ_RDF_REF_GPU) LJ_RDF_LAUNCH(LJ_RDF_REF_GPU);
+  else LJ_RDF_LAUNCH(LJ_RDF_RBINS);
+#undef LJ_RDF_LAUNCH
 #undef LJ_RDF_ARGS
   return hipGetLastError();
 }

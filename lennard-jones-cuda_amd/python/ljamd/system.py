@@ -151,6 +151,13 @@ class LJSystem:
         check(lib().lj_device_arrays(self.h, C.byref(p), C.byref(v), C.byref(f)))
         return p.value, v.value, f.value
 
+    def time_rdf(self, rule=_L.RDF_REF_GPU, nbins=256, dr2=0.0, rmax=0.0, reps=20):
+        """Average microseconds of one RDF pass (lj_time_rdf)."""
+        us = C.c_double()
+        check(lib().lj_time_rdf(self.h, int(rule), int(nbins), float(dr2), float(rmax), int(reps),
+                                C.byref(us)))
+        return us.value
+
     def time_force_kernel(self, reps=20):
         us = C.c_double()
         check(lib().lj_time_force_kernel(self.h, int(reps), C.byref(us)))

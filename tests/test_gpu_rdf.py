@@ -62,6 +62,21 @@ def test_rdf_accumulate():
     assert np.array_equal(b, 2 * a) and np.array_equal(c, a)
 
 
+def test_rdf_timing_leaves_rdf_exact():
+    """lj_time_rdf runs reps + 1 passes into the device histogram; the next
+    lj_rdf (even asked to accumulate) starts from zero and stays exact."""
+    N, rho = 800, 0.5
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=12)
+    with ljamd.LJSystem(N, rho, 1.0) as s:
+        s.set_state(pos)
+        a = s.rdf(ljamd.RDF_RBINS, 100, rmax=L / 2)
+        us = s.time_rdf(ljamd.RDF_RBINS, 100, rmax=L / 2, reps=3)
+        b = s.rdf(ljamd.RDF_RBINS, 100, rmax=L / 2, accumulate=True)
+        with pytest.raises(ljamd.LJError):
+            s.time_rdf(ljamd.RDF_RBINS, 100, reps=0)
+    assert us > 0 and np.array_equal(a, b)
+
+
 def test_rdf_bad_args():
     with ljamd.LJSystem(100, 0.5, 1.0) as s:
         with pytest.raises(ljamd.LJError):
