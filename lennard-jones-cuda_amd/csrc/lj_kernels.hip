@@ -404,14 +404,34 @@ __device__ __forceinline__ float mi_gpu(float d, float Lf, const float (&t)[4])
   return d - Lf * (float)k;
 }
 
-// bin of r2 in the threshold table T (T[k] = least r^2 of bin >= k): the
-// largest k < nbins with T[k] <= r2, from an estimate corrected on the table
+// bin of r2 in the threshold table T (T[k] = least r^2 of bin >= k, T[0] = 0
+// <= r2, and r2 < T[nbins - 1] here): the largest k < nbins with T[k] <= r2.
+// The estimate (r2 / dr2, or r / dr, in float with an approximate square
+// root) is off by at most one bin: its relative error is below 2^-21 on bin
+// indices below 2^13, so one step down and one step up on the table give the
+// bin exactly.
 __device__ __forceinline__ int rdf_bin(float r2, const float* T, int nbins, int est)
 {
   int k = min(max(est, 0), nbins - 1);
-  while (k > 0 && T[k] > r2) --k;
-  while (k + 1 < nbins && T[k + 1] <= r2) ++k;
+  k -= T[k] > r2 ? 1 : 0;
+  k += T[k + 1] <= r2 ? 1 : 0;  // k + 1 <= nbins - 1 < nbins + 1 entries
   return k;
+}
+
+// minimum image for |r| < the second threshold (image index -1, 0 or 1):
+// k L is exact, so one fused rounding of r - k L in double equals the
+// reference's product-then-difference
+__device__ __forceinline__ float mi_cpu1(float r, double L, float t0)
+{
+  const int k = (r >= t0 ? 1 : 0) - (r <= -t0 ? 1 : 0);
+  return (float)__builtin_fma(-(double)k, L, (double)r);
+}
+
+__device__ __forceinline__ float mi_gpu1(float d, float Lf, float t0)
+{
+#pragma clang fp contract(off)
+  const float k = (d >= t0 ? 1.f : 0.f) - (d <= -t0 ? 1.f : 0.f);
+  return d - Lf * k;
 }
 
 // RDF pass.  Every binning rule is symmetric in i and j (the coordinate
@@ -504,12 +524,11 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
       // |r| < thr[1] on every axis (any position within half a box of the box):
       // the image index is -1, 0 or 1, from the first threshold alone
       const float am = fmaxf(fabsf(rx), fmaxf(fabsf(ry), fabsf(rz)));
-      const float one[4] = {thr[0], 3.4e38f, 3.4e38f, 3.4e38f};
       if constexpr (RULE == LJ_RDF_REF_GPU) {
         if (am < thr[1]) {
-          rx = mi_gpu(rx, Lf, one);
-          ry = mi_gpu(ry, Lf, one);
-          rz = mi_gpu(rz, Lf, one);
+          rx = mi_gpu1(rx, Lf, thr[0]);
+          ry = mi_gpu1(ry, Lf, thr[0]);
+          rz = mi_gpu1(rz, Lf, thr[0]);
         } else {
           rx = mi_gpu(rx, Lf, thr);
           ry = mi_gpu(ry, Lf, thr);
@@ -517,9 +536,9 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
         }
       } else {
         if (am < thr[1]) {
-          rx = mi_cpu(rx, L, one);
-          ry = mi_cpu(ry, L, one);
-          rz = mi_cpu(rz, L, one);
+          rx = mi_cpu1(rx, L, thr[0]);
+          ry = mi_cpu1(ry, L, thr[0]);
+          rz = mi_cpu1(rz, L, thr[0]);
         } else {
           rx = mi_cpu(rx, L, thr);
           ry = mi_cpu(ry, L, thr);
@@ -539,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
       ++last;
       continue;
     }
-    const float e = RULE == LJ_RDF_RBINS ? __builtin_sqrtf(r2) * est_scale : r2 * est_scale;
+    const float e = RULE == LJ_RDF_RBINS ? __builtin_amdgcn_sqrtf(r2) * est_scale : r2 * est_scale;
     atomicAdd(&hw[rdf_bin(r2, T, nbins, (int)e)], 2u);
   }
   // the wave's last-bin count: one LDS atomic per wave
