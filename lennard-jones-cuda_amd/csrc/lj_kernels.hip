@@ -506,18 +506,8 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
   // pair beyond the range is clamped into it, which would otherwise put most
   // lanes of a wave on one LDS address
   unsigned int last = 0u;
-  for (int jj = 0; jj < nj; ++jj) {
-    if (!valid || (diag && jj <= (int)threadIdx.x)) continue;
-    if constexpr (FAR) {
-      const uint4 qj = qj_s[jj];
-      const float dx = (float)(int)(qi.x - qj.x), dy = (float)(int)(qi.y - qj.y),
-                  dz = (float)(int)(qi.z - qj.z);
-      const float r2q = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) * qs2;
-      if (r2q > far) {
-        if constexpr (RULE == LJ_RDF_REF_GPU) ++last;
-        continue;
-      }
-    }
+  // the reference arithmetic for pair (i, j0 + jj): image, r^2, bin
+  auto exact = [&](int jj) {
     const float4 pj = PJ[jj];
     float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     if constexpr (PERIODIC) {
@@ -549,17 +539,51 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
     float r2;
     if constexpr (RULE == LJ_RDF_REF_GPU) {
       r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));  // MDSystem.cu:35
-      if (!(r2 > 1e-6f)) continue;                                     // MDSystem.cu:37
+      if (!(r2 > 1e-6f)) return;                                       // MDSystem.cu:37
     } else {
       r2 = rx * rx + ry * ry + rz * rz;  // float, no contraction (MDSystem.cpp:279)
-      if (!(r2 < Tmax)) continue;        // beyond the last bin: dropped
+      if (!(r2 < Tmax)) return;          // beyond the last bin: dropped
     }
     if (r2 >= Tlast) {
       ++last;
-      continue;
+      return;
     }
     const float e = RULE == LJ_RDF_RBINS ? __builtin_amdgcn_sqrtf(r2) * est_scale : r2 * est_scale;
     atomicAdd(&hw[rdf_bin(r2, T, nbins, (int)e)], 2u);
+  };
+  if constexpr (FAR) {
+    // 32 j at a time: the fixed-point test marks each lane's near pairs in a
+    // bit mask, then the lanes walk their own set bits, so the exact path runs
+    // as often as the busiest lane needs it, not once per j
+    for (int c0 = 0; c0 < nj; c0 += 32) {
+      const int cn = min(32, nj - c0);
+      unsigned int mask = 0u;
+      for (int b = 0; b < cn; ++b) {
+        const int jj = c0 + b;
+        const uint4 qj = qj_s[jj];
+        const float dx = (float)(int)(qi.x - qj.x), dy = (float)(int)(qi.y - qj.y),
+                    dz = (float)(int)(qi.z - qj.z);
+        const float r2q = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) * qs2;
+        const bool ok = valid && !(diag && jj <= (int)threadIdx.x);
+        if (ok) {
+          if (r2q > far) {
+            if constexpr (RULE == LJ_RDF_REF_GPU) ++last;
+          } else {
+            mask |= 1u << b;
+          }
+        }
+      }
+      while (mask) {
+        const int b = __builtin_ctz(mask);
+        mask &= mask - 1u;
+        exact(c0 + b);
+      }
+    }
+  } else {
+    for (int jj = 0; jj < nj; ++jj) {
+      if (!valid || (diag && jj <= (int)threadIdx.x)) continue;
+      exact(jj);
+    }
   }
   // the wave's last-bin count: one LDS atomic per wave
 #pragma unroll
