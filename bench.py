@@ -289,9 +289,14 @@ def main():
     ap.add_argument("--profile-force", nargs="?", const="pair-once", default=None,
                     choices=["pair-once", "sliced"],
                     help="only a calibration copy and the force kernel (for rocprofv3)")
+    ap.add_argument("--profile-rdf", action="store_true",
+                    help="only the RDF passes of the rdf leg (for rocprofv3)")
     a = ap.parse_args()
     if a.profile_force:
         profile_force(a.profile_force)
+        return
+    if a.profile_rdf:
+        print(json.dumps({"profile": "rdf", "N": N, **rdf_leg()}))
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
