@@ -1,0 +1,118 @@
+"""Small and ragged systems through every device path: the LJ dimer's closed
+forms, N = 1 and 2, sizes just off a wave (64) or a pair-once group (256),
+and the pair-once kernel forced on at small N (one group: a diagonal item
+only; a partial last group of one particle), against the FP64 oracle, the
+reference's drift arithmetic and the exact RDF counts of the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import ljamd
+from helpers import force_errors, perturbed_state, rel, xyz
+from oracle_lib import ORC, RDF_RBINS, RDF_REF_CPU, RDF_REF_GPU, forces_fp64, rdf
+
+pytestmark = pytest.mark.gpu
+
+RULES = {RDF_REF_CPU: ljamd.RDF_REF_CPU, RDF_REF_GPU: ljamd.RDF_REF_GPU,
+         RDF_RBINS: ljamd.RDF_RBINS}
+
+
+@pytest.mark.parametrize("r,V,F", [(2.0 ** (1.0 / 6.0), -1.0, 0.0), (1.0, 0.0, 24.0),
+                                   (1.5, 4 * (1.5 ** -12 - 1.5 ** -6), None)])
+def test_dimer_closed_form(r, V, F):
+    """V = 4 (r^-12 - r^-6), |F| = 24 |2 r^-13 - r^-7| (sigma = eps = 1)."""
+    F = abs(24 * (2 * r ** -13 - r ** -7)) if F is None else F
+    pos = np.array([[1.0, 1.0, 1.0, 0.0], [1.0 + r, 1.0, 1.0, 0.0]], np.float32).reshape(-1)
+    with ljamd.LJSystem(2, 0.001, 1.0, boundary=ljamd.BC_NONE) as s:
+        s.set_state(pos, np.zeros_like(pos))
+        obs = s.CalculateForces()
+        f = s.force.reshape(2, 4)
+    assert abs(obs["V"] - V) <= 1e-6 * max(1.0, abs(V))
+    assert abs(abs(f[0, 0]) - F) <= 1e-5 * max(1.0, F) and abs(f[0, 1]) + abs(f[0, 2]) == 0.0
+    assert np.array_equal(f[0, :3], -f[1, :3])  # Newton's third law, exactly for two bodies
+
+
+def test_single_particle():
+    """No pairs: zero force, energy and virial; a free particle drifts at
+    constant velocity (EVN); every RDF bin is zero."""
+    pos = np.array([2.0, 3.0, 4.0, 0.0], np.float32)
+    vel = np.array([0.5, -0.25, 0.125, 0.0], np.float32)
+    with ljamd.LJSystem(1, 0.01, 1.0, boundary=ljamd.BC_NONE) as s:
+        s.set_state(pos, vel)
+        obs = s.CalculateForces()
+        assert np.all(s.force == 0.0) and obs["V"] == 0.0 and obs["W"] == 0.0
+        s.Integrate(0.01, 10)
+        p = s.pos
+        for rule in (ljamd.RDF_REF_CPU, ljamd.RDF_REF_GPU, ljamd.RDF_RBINS):
+            assert not np.any(s.rdf(rule, 50, rmax=2.0))
+    # the reference's drift x = (float)(x + (dt v + dt^2 f / 2)) in double
+    # (MDSystem.cpp:447-449) with f = 0, ten times: the same bits
+    x = pos[:3].copy()
+    for _ in range(10):
+        x = (x.astype(np.float64) + 0.01 * vel[:3].astype(np.float64)).astype(np.float32)
+    assert np.array_equal(p[:3], x)
+
+
+@pytest.mark.parametrize("N,bc", [(2, ljamd.BC_PERIODIC), (3, ljamd.BC_PERIODIC), (63, ljamd.BC_WALL),
+                                  (65, ljamd.BC_PERIODIC), (257, ljamd.BC_PERIODIC),
+                                  (257, ljamd.BC_NONE)])
+def test_ragged_forces_and_rdf(N, bc):
+    rho = 0.5
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=41)
+    print(f"\n[case] ragged N={N} bc={bc}", flush=True)
+    with ljamd.LJSystem(N, rho, 1.0, boundary=bc) as s:
+        s.set_state(pos, np.zeros_like(pos))
+        obs = s.CalculateForces()
+        f4 = s.force
+        got = {rule: s.rdf(RULES[rule], nb, dr2=ORC.orc_rdf_dr2(N), rmax=L / 2)
+               for rule, nb in ((RDF_REF_CPU, 256), (RDF_REF_GPU, 256), (RDF_RBINS, 40))}
+    F64, _, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
+    err = force_errors(xyz(f4), F64)
+    assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5, err
+    # 3e-6: in these few-particle boxes (L = 1.6 to 5 sigma) e = (b - a) / 2
+    # cancels more of the per-particle float sums (measured 1.23e-6 at N = 2, 65)
+    assert rel(obs["V"], E) <= 3e-6 and rel(obs["W"], W) <= 1e-5
+    for rule, nb in ((RDF_REF_CPU, 256), (RDF_REF_GPU, 256), (RDF_RBINS, 40)):
+        exp = rdf(pos, L, bc == ljamd.BC_PERIODIC, rule, nb, dr2=ORC.orc_rdf_dr2(N), rmax=L / 2)
+        assert np.array_equal(got[rule].astype(np.int64), exp), rule
+
+
+@pytest.fixture
+def pair_once_env():
+    old = os.environ.get("LJ_PAIR_ONCE")
+    yield
+    if old is None:
+        os.environ.pop("LJ_PAIR_ONCE", None)
+    else:
+        os.environ["LJ_PAIR_ONCE"] = old
+
+
+@pytest.mark.parametrize("N,bc", [(256, ljamd.BC_PERIODIC), (257, ljamd.BC_WALL),
+                                  (700, ljamd.BC_PERIODIC)])
+def test_pair_once_forced_small(N, bc, pair_once_env):
+    """LJ_PAIR_ONCE=1 at small N: one group (the diagonal item alone), a last
+    group of a single particle, three groups; forces against FP64, and a
+    20-step TVN run (the step variant, fused 8-step graphs) against the sliced
+    kernel's (LJ_PAIR_ONCE=0) from the same state."""
+    rho = 0.6
+    L, pos, vel = perturbed_state(N, rho, 1.0, seed=43)
+    print(f"\n[case] pair-once forced N={N} bc={bc}", flush=True)
+    runs = {}
+    for flag in ("1", "0"):
+        os.environ["LJ_PAIR_ONCE"] = flag
+        with ljamd.LJSystem(N, rho, 1.0, canonical=ljamd.TVN, boundary=bc) as s:
+            assert s.force_plan()[0] == (flag == "1")
+            s.set_state(pos, vel)
+            obs = s.CalculateForces()
+            f4 = s.force
+            h = s.Integrate(0.005, 20)
+            runs[flag] = (f4, obs, h, s.pos)
+    F64, _, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
+    for flag in ("1", "0"):
+        err = force_errors(xyz(runs[flag][0]), F64)
+        assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5, (flag, err)
+        assert rel(runs[flag][1]["V"], E) <= 1e-6
+    dU = np.max(np.abs(runs["1"][2][:, 0] - runs["0"][2][:, 0])) / N
+    dx = np.max(np.abs(xyz(runs["1"][3]) - xyz(runs["0"][3])))
+    assert dU <= 1e-5 and dx <= 1e-4, (dU, dx)
