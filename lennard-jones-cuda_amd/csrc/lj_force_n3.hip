@@ -83,12 +83,15 @@ __device__ __forceinline__ float rot1(float v)
 // at step t and passes them to lane l - 1 afterwards), so after 64 steps lane l
 // holds those of slot l.  J receives (-fx, -fy, -fz, sum (s/r)^6 - w) per
 // sub-tile.
+// nsub: sub-tiles holding a valid j (1 when the second one is past N)
 template <bool PERIODIC, bool TIES, bool PARTIAL, bool ENERGY>
 __device__ __forceinline__ void pair_tile(const uint4* P0, int lane, const PosT<PERIODIC>& pi1,
                                           const PosT<PERIODIC>& pi2, bool v1, bool v2, float C1,
-                                          Acc& a1, Acc& a2, float (&J)[2][4])
+                                          int nsub, Acc& a1, Acc& a2, float (&J)[2][4])
 {
+#pragma unroll
   for (int st = 0; st < 2; ++st) {
+    if (st >= nsub) break;  // wave-uniform
     const uint4* P = P0 + st * 128;
     float jx = 0.f, jy = 0.f, jz = 0.f, jm = 0.f;
 #pragma unroll 8
@@ -135,9 +138,11 @@ __device__ __forceinline__ void pair_tile(const uint4* P0, int lane, const PosT<
 template <bool PERIODIC, bool TIES>
 __device__ __forceinline__ void self_tile(const uint4* P0, int lane, int i1, int i2, int jbase,
                                           const PosT<PERIODIC>& pi1, const PosT<PERIODIC>& pi2,
-                                          bool v1, bool v2, float C1, Acc& a1, Acc& a2)
+                                          bool v1, bool v2, float C1, int nsub, Acc& a1, Acc& a2)
 {
+#pragma unroll
   for (int st = 0; st < 2; ++st) {
+    if (st >= nsub) break;  // wave-uniform
     const uint4* P = P0 + st * 128;
 #pragma unroll 8
     for (int t = 0; t < 64; ++t) {
@@ -222,7 +227,13 @@ __device__ __forceinline__ void n3_item(const uint4* __restrict__ q, const float
   }
   __syncthreads();
 
-  const int cls = diag ? (wi < wj ? 1 : (wi == wj ? 2 : 0)) : 1;  // pair tile / self tile / idle
+  // tiles entirely past N (the last group of a box, partial) are idle, and a
+  // second sub-tile past N is skipped: wave-uniform tests
+  const int jt0 = kG * h + kTile * wj, it0 = kG * g + kTile * wi;
+  const bool live = jt0 < N && it0 < N;
+  const int nsub = jt0 + 64 < N ? 2 : 1;
+  // pair tile / self tile / idle
+  const int cls = !live ? 0 : (diag ? (wi < wj ? 1 : (wi == wj ? 2 : 0)) : 1);
   const int i1 = kG * g + kTile * wi + lane, i2 = i1 + 64;
   const bool v1 = i1 < N, v2 = i2 < N;
   const PosT<PERIODIC> pi1 = load_pos<PERIODIC>(q, pf, v1 ? i1 : N - 1);
@@ -234,11 +245,12 @@ __device__ __forceinline__ void n3_item(const uint4* __restrict__ q, const float
   float J[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
   if (cls == 1) {
     if (partial)
-      pair_tile<PERIODIC, TIES, true, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, a1, a2, J);
+      pair_tile<PERIODIC, TIES, true, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, nsub, a1, a2, J);
     else
-      pair_tile<PERIODIC, TIES, false, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, a1, a2, J);
+      pair_tile<PERIODIC, TIES, false, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, 2, a1, a2, J);
   } else if (cls == 2) {
-    self_tile<PERIODIC, TIES>(P0, lane, i1, i2, kG * h + kTile * wj, pi1, pi2, v1, v2, C1, a1, a2);
+    self_tile<PERIODIC, TIES>(P0, lane, i1, i2, kG * h + kTile * wj, pi1, pi2, v1, v2, C1, nsub, a1,
+                              a2);
   }
   {
     float* A = s_acc + wave * kAccPerWave;
