@@ -119,23 +119,32 @@ __global__ __launch_bounds__(kBlock) void k_batch_drift(BatchArgs a)
   const int i = pp.gi;
   float4 p = a.pos[i];
   float4 v = a.vel[i];
-  const float4 f = a.force[i];
-  const double dt = a.dt;
-  p.x = (float)(p.x + (dt * v.x + dt * dt * f.x / 2.));  // MDSystem.cpp:447-449
-  p.y = (float)(p.y + (dt * v.y + dt * dt * f.y / 2.));
-  p.z = (float)(p.z + (dt * v.z + dt * dt * f.z / 2.));
-  if constexpr (CANON == LJ_EVN) {
-    v.x = (float)(v.x + dt * f.x / 2.);  // MDSystem.cpp:451-453
-    v.y = (float)(v.y + dt * f.y / 2.);
-    v.z = (float)(v.z + dt * f.z / 2.);
-    a.vel[i] = v;
-  } else {
-    a.tF[i] = make_float4(0.5f * f.x, 0.5f * f.y, 0.5f * f.z, 0.f);  // MDSystem.cpp:477-479
-  }
+  float4 tf;
+  drift_one<CANON>(p, v, tf, a.force[i], a.dt);  // MDSystem.cpp:447-453, 477-479
+  if constexpr (CANON == LJ_EVN) a.vel[i] = v;
+  else a.tF[i] = tf;
   a.pos[i] = p;
   if (a.bc == LJ_BC_PERIODIC) {
     const double qs = a.box[pp.b].qscale;
     a.q[i] = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), 0u);
+  }
+}
+
+// the next step's drift of one particle at the end of a kick kernel (the
+// arithmetic of k_batch_drift on the values the kick just computed); block 0
+// also resets the work-queue head for the next force pass
+template <int CANON>
+__device__ __forceinline__ void batch_drift_next(const BatchArgs& a, const PIdx& pp, float4 p,
+                                                 float4 v, const float4& f)
+{
+  float4 tf;
+  drift_one<CANON>(p, v, tf, f, a.dt);
+  if constexpr (CANON == LJ_EVN) a.vel[pp.gi] = v;
+  else a.tF[pp.gi] = tf;
+  a.pos[pp.gi] = p;
+  if (a.bc == LJ_BC_PERIODIC) {
+    const double qs = a.box[pp.b].qscale;
+    a.q[pp.gi] = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), 0u);
   }
 }
 
@@ -154,9 +163,11 @@ __device__ __forceinline__ float inside(const float4& p, const BoxParam& bp)
              : 0.f;
 }
 
+template <bool DRIFT>
 __global__ __launch_bounds__(kBlock) void k_batch_kick_evn(BatchArgs a)
 {
 #pragma clang fp contract(off)
+  if (DRIFT && blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0u;  // next step's force pass
   const PIdx pp = pidx(a);
   const BoxParam& bp = a.box[pp.b];
   double v4[4] = {0., 0., 0., 0.};
@@ -174,13 +185,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_kick_evn(BatchArgs a)
     float4 p = a.pos[i];
     if (a.bc != LJ_BC_NONE) {
       apply_bc_one(p, v, a.bc, bp.L);
-      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
+      if (!DRIFT && a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
     }
-    a.vel[i] = v;
     v4[0] = kin2(v);
     v4[1] = e;
     v4[2] = s.b;
     v4[3] = inside(p, bp);
+    if constexpr (DRIFT) batch_drift_next<LJ_EVN>(a, pp, p, v, make_float4(fx, fy, fz, 0.f));
+    else a.vel[i] = v;
   }
   const int fld[4] = {BLK_K2, BLK_E, BLK_B, BLK_NSUB};
   block_sum_store<4>(v4, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
@@ -218,9 +230,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_tvn_a(BatchArgs a)
   block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
 }
 
+template <bool DRIFT>
 __global__ __launch_bounds__(kBlock) void k_batch_tvn_b(BatchArgs a)
 {
 #pragma clang fp contract(off)
+  if (DRIFT && blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0u;  // next step's force pass
   const PIdx pp = pidx(a);
   const BoxParam& bp = a.box[pp.b];
   const double tv2 = blk_total(a.blk + (size_t)bp.blk0 * kBlkFields, bp.nblk, BLK_TV2);
@@ -238,11 +252,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_tvn_b(BatchArgs a)
     float4 p = a.pos[i];
     if (a.bc != LJ_BC_NONE) {
       apply_bc_one(p, v, a.bc, bp.L);
-      if (a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
+      if (!DRIFT && a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
     }
     a.vel[i] = v;
     v2[0] = kin2(v);
     v2[1] = inside(p, bp);
+    if constexpr (DRIFT) batch_drift_next<LJ_TVN>(a, pp, p, v, a.force[i]);
   }
   __syncthreads();
   const int fld[2] = {BLK_K2, BLK_NSUB};
@@ -395,9 +410,10 @@ hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic,
   return hipGetLastError();
 }
 
-hipError_t launch_batch_kick_evn(hipStream_t st, const BatchArgs& a)
+hipError_t launch_batch_kick_evn(hipStream_t st, const BatchArgs& a, bool drift_next)
 {
-  hipLaunchKernelGGL(k_batch_kick_evn, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  if (drift_next) hipLaunchKernelGGL(k_batch_kick_evn<true>, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  else hipLaunchKernelGGL(k_batch_kick_evn<false>, dim3(a.nblocks), dim3(kBlock), 0, st, a);
   return hipGetLastError();
 }
 
@@ -407,9 +423,10 @@ hipError_t launch_batch_tvn_a(hipStream_t st, const BatchArgs& a)
   return hipGetLastError();
 }
 
-hipError_t launch_batch_tvn_b(hipStream_t st, const BatchArgs& a)
+hipError_t launch_batch_tvn_b(hipStream_t st, const BatchArgs& a, bool drift_next)
 {
-  hipLaunchKernelGGL(k_batch_tvn_b, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  if (drift_next) hipLaunchKernelGGL(k_batch_tvn_b<true>, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  else hipLaunchKernelGGL(k_batch_tvn_b<false>, dim3(a.nblocks), dim3(kBlock), 0, st, a);
   return hipGetLastError();
 }
 

@@ -47,8 +47,10 @@ struct lj_batch {
   unsigned* counter = nullptr;
   int nblocks = 0, nitems = 0;
   std::vector<BoxParam> hbox;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;  // one step
   hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph_run = nullptr;  // kGraphSteps steps back to back
+  hipGraphExec_t gexec_run = nullptr;
   double g_dt = 0.;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -59,8 +61,12 @@ void bdrop_graph(lj_batch* b)
 {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->gexec_run) (void)hipGraphExecDestroy(b->gexec_run);
+  if (b->graph_run) (void)hipGraphDestroy(b->graph_run);
   b->gexec = nullptr;
   b->graph = nullptr;
+  b->gexec_run = nullptr;
+  b->graph_run = nullptr;
 }
 
 int bsetdev(lj_batch* b)
@@ -207,18 +213,39 @@ int build_tables(lj_batch* b)
   return LJ_OK;
 }
 
-int enqueue_batch_step(lj_batch* b, hipStream_t st, double dt)
+// one step of a run captured together: the first starts with the drift, each
+// later one finds it done by the previous step's kick kernel (drift_next), with
+// the same arithmetic, so a run gives the bits of single steps
+int enqueue_batch_step(lj_batch* b, hipStream_t st, double dt, bool first = true, bool last = true)
 {
   const BatchArgs a = bargs(b, dt, true);
-  LJ_HIP(launch_batch_drift(st, a));
+  if (first) LJ_HIP(launch_batch_drift(st, a));
   LJ_HIP(launch_batch_force(st, a, b->boundary == LJ_BC_PERIODIC, false));
   if (b->canonical == LJ_EVN) {
-    LJ_HIP(launch_batch_kick_evn(st, a));
+    LJ_HIP(launch_batch_kick_evn(st, a, !last));
   } else {
     LJ_HIP(launch_batch_tvn_a(st, a));
-    LJ_HIP(launch_batch_tvn_b(st, a));
+    LJ_HIP(launch_batch_tvn_b(st, a, !last));
   }
   LJ_HIP(launch_batch_finalize(st, a, 1));
+  return LJ_OK;
+}
+
+int capture_batch_steps(lj_batch* b, double dt, int steps, hipGraph_t* graph, hipGraphExec_t* exec)
+{
+  hipStream_t st = b->stream;
+  LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = LJ_OK;
+  for (int k = 0; k < steps && rc == LJ_OK; ++k) rc = enqueue_batch_step(b, st, dt, k == 0, k == steps - 1);
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(st, &g);
+  if (rc != LJ_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  LJ_HIP(ee);
+  *graph = g;
+  LJ_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
   return LJ_OK;
 }
 
@@ -536,20 +563,17 @@ int lj_batch_integrate(lj_batch* b, double dt, int nsteps)
   if (!b->gexec || b->g_dt != dt) {
     bdrop_graph(b);
     LJ_HIP(hipStreamSynchronize(st));
-    LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_batch_step(b, st, dt);
-    hipGraph_t g = nullptr;
-    hipError_t ee = hipStreamEndCapture(st, &g);
+    LJ_TRY(capture_batch_steps(b, dt, 1, &b->graph, &b->gexec));
+    const int rc = capture_batch_steps(b, dt, kGraphSteps, &b->graph_run, &b->gexec_run);
     if (rc != LJ_OK) {
-      if (g) (void)hipGraphDestroy(g);
+      bdrop_graph(b);
       return rc;
     }
-    LJ_HIP(ee);
-    b->graph = g;
-    LJ_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
     b->g_dt = dt;
   }
-  for (int k = 0; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(b->gexec, st));
+  int k = 0;
+  for (; k + kGraphSteps <= nsteps; k += kGraphSteps) LJ_HIP(hipGraphLaunch(b->gexec_run, st));
+  for (; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(b->gexec, st));
   LJ_HIP(hipStreamSynchronize(st));
   return LJ_OK;
 }

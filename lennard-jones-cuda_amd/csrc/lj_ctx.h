@@ -57,10 +57,6 @@ struct lj_ctx {
 };
 
 namespace lj {
-// steps per replay of the multi-step graph: each graph launch costs about 8 us
-// of idle device time between replays, against none between the kernels
-// inside one graph
-constexpr int kGraphSteps = 8;
 int ctx_setdev(lj_ctx* c);
 // step graphs (one step, kGraphSteps steps) captured once per (dt, ensemble,
 // boundary) and replayed

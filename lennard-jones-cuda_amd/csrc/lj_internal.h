@@ -107,6 +107,10 @@ struct StepArgs {
   // particles in the FP32 step kernels, one per kBlock in the FP64 ones
   int nrow;
 };
+// steps per replay of the multi-step graphs (single box and batch): each graph
+// launch costs about 8 us of idle device time between replays, against none
+// between the kernels inside one graph
+constexpr int kGraphSteps = 8;
 // particles per block row of the FP32 time-step kernels
 constexpr int kRowParticles = 64;
 inline int step_rows(int N) { return (N + kRowParticles - 1) / kRowParticles; }
@@ -216,9 +220,10 @@ struct BatchArgs {
 hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic, bool ties);
 hipError_t launch_batch_force_n3(hipStream_t st, const BatchArgs& a, bool periodic, bool ties);
-hipError_t launch_batch_kick_evn(hipStream_t st, const BatchArgs& a);
+// drift_next: also the next step's drift of each particle (as launch_batch_drift)
+hipError_t launch_batch_kick_evn(hipStream_t st, const BatchArgs& a, bool drift_next = false);
 hipError_t launch_batch_tvn_a(hipStream_t st, const BatchArgs& a);
-hipError_t launch_batch_tvn_b(hipStream_t st, const BatchArgs& a);
+hipError_t launch_batch_tvn_b(hipStream_t st, const BatchArgs& a, bool drift_next = false);
 hipError_t launch_batch_finalize(hipStream_t st, const BatchArgs& a, int advance);
 hipError_t launch_batch_prep_q(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_reduce_forces(hipStream_t st, const BatchArgs& a);

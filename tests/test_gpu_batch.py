@@ -121,6 +121,27 @@ def test_batch_pair_once_step_totals(canon, bc):
         assert np.max(np.abs(xyz(a[1][k]) - xyz(c[1][k]))) <= 1e-4, k
 
 
+@pytest.mark.parametrize("canon,tuning", [(ljamd.EVN, (1, 512, 8, 0)), (ljamd.EVN, (1, 512, 0, 1)),
+                                          (ljamd.TVN, (1, 512, 0, 1)), (ljamd.TVN, (2, 1024, 4, 0))])
+def test_batch_graph_replay_deterministic(canon, tuning):
+    """20 steps as 8 + 8 + 4 x 1 (8-step graphs whose kick kernels do the next
+    drift and reset the work queue) and as 7 x 1 + (8 + 5 x 1): the same bits
+    for positions, velocities, observables and statistics."""
+    N, rho, T0 = _small_batch(nb=7, seed=23)
+    print(f"\n[case] batch graph replay canon={canon} tuning={tuning}", flush=True)
+    outs = []
+    for chunks in ([20], [7, 13]):
+        with ljamd.LJBatch(N, rho, T0, canonical=canon, seed=8) as b:
+            b.set_tuning(*tuning)
+            for c in chunks:
+                b.integrate(0.005, c)
+            obs = b.observables()
+            outs.append((b.get_array(ljamd.ARRAY_POS), b.get_array(ljamd.ARRAY_VEL), b.stats(),
+                         [[o[k] for k in ("U", "T", "V", "P", "t", "step")] for o in obs]))
+    for x, y in zip(outs[0], outs[1]):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
 def test_batch_fluctuation_stats():
     N, rho, T0 = _small_batch(nb=8, seed=13)
     frac = 0.3
