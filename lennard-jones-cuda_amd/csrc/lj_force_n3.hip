@@ -68,14 +68,17 @@ __device__ __forceinline__ void terms(const PosT<PERIODIC>& pi, const uint4& pj,
 }
 
 // rotate a value one lane down the wave: lane l receives lane (l + 1) mod 64.
-// Every lane has a source lane, so the "old" operand is never used; passing
-// the value itself lets the compiler rotate in place (with old = 0 it spent a
-// v_mov_b32 per rotation initialising the destination: 24 of 433 VALU per 8
-// steps of the pair loop).
+// ds_bpermute does the gather in the LDS unit, off the VALU that bounds the
+// pair loop: a DPP move (wave_rol:1, and row_ror alike) cost about 2.5 VALU
+// issue slots here, and the kernel ran 92.6 us with it against 77.8 us with
+// the permute (N = 16384, time-step variant, same box).  The LDS unit is nearly
+// idle in the loop (one read per step), and the permute's latency overlaps the
+// next step's i-side work.  Every lane of the wave must be active: the pair
+// loops run with a full execution mask.
 __device__ __forceinline__ float rot1(float v)
 {
-  const int x = __float_as_int(v);
-  return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x134 /* wave_rol:1 */, 0xf, 0xf, false));
+  const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + 1) & 63) << 2, __float_as_int(v)));
 }
 
 // one tile pair evaluated once: i sums in a1/a2; j sums in registers that
