@@ -234,8 +234,8 @@ def cpu_parallel_leg(threads=16, budget_s=5.0):
 
 
 def profiled_sclk_ghz(kernel):
-    """Shader clock measured in the committed profile of the force kernel
-    (GRBM_GUI_ACTIVE cycles per XCD over the kernel's duration)."""
+    """Shader clock under load recorded in the committed profile of the force
+    kernel (amd-smi clock samples taken during a long bench run)."""
     import re
 
     path = os.path.join(ROOT, "profiles", f"{kernel}_n16384_periodic.md")
@@ -357,8 +357,9 @@ def main():
                      "newton3": g["pair_once"],
                      "pair_evaluations_per_launch": (N * (N - 1) // 2 if g["pair_once"]
                                                      else N * N),
-                     # SQ_INSTS_VALU per ordered pair (13.19 measured) + half-rate v_rcp_f32
-                     "issue_slots_per_ordered_pair": 13.7 if g["pair_once"] else 22,
+                     # SQ_INSTS_VALU x 64 / N^2 (12.44 pair-once, 21.46 sliced, measured)
+                     # + the half-rate v_rcp_f32 (one per unordered / ordered pair)
+                     "issue_slots_per_ordered_pair": 12.9 if g["pair_once"] else 22.5,
                      "sliced_kernel_us": sliced_us,
                      "sliced_kernel_tflops": FLOP_PER_PAIR * pairs / (sliced_us * 1e-6) / 1e12},
         "cpu_baseline": cpu,

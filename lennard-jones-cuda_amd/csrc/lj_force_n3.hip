@@ -421,9 +421,10 @@ bool n3_eligible(int N)
   // LJ_PAIR_ONCE=0 / 1 turns the kernel off / on at any size (tuning)
   const char* e = std::getenv("LJ_PAIR_ONCE");
   if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1' && N >= 256;
-  // measured crossover against the sliced kernel: N = 10240 (55 vs 60 us),
-  // N = 8192 (42.5 vs 39.6 us)
-  return N >= 9472;
+  // measured against the sliced kernel (time step, TVN, rho* = 0.85): N = 6144
+  // 33.6 vs 37.3 us per step (force pass 22.4 vs 22.9 us), N = 4096 28.7 vs
+  // 26.1 us (few group pairs leave most CUs idle: 18.6 vs 12.0 us)
+  return N >= 6144;
 }
 
 void n3_items(int ng, int2* out)

@@ -41,7 +41,7 @@ struct ForcePlan {
 };
 ForcePlan make_force_plan(int N, int jslices_override, int ipl);
 ForcePlan make_n3_plan(int N);
-// sizes at which the pair-once kernel beats the sliced one (N >= 9472)
+// sizes at which the pair-once kernel beats the sliced one (N >= 6144)
 bool n3_eligible(int N);
 // the ng (ng + 1) / 2 group pairs of the pair-once kernel, in launch order
 void n3_items(int ng, int2* out);
