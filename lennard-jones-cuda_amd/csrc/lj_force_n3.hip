@@ -13,9 +13,9 @@
 //     immediate offset; after 64 steps every lane has met every j.
 //   * i sums (force, (s/r)^6, (s/r)^6 (2 (s/r)^6 - 1)) stay in registers.
 //   * j sums (minus the force, (s/r)^6 - w) travel with their j particle:
-//     after each step every lane passes its four j sums to the next lower
-//     lane (DPP wave_rol:1), so no atomics are involved and every sum is
-//     taken in a fixed order (bit-reproducible results).
+//     after each step every lane passes its j sums to the next lower lane
+//     (ds_bpermute), so no atomics are involved and every sum is taken in a
+//     fixed order (bit-reproducible results).
 //   * Diagonal items (g == h): waves with wi < wj evaluate their tile pair
 //     once as above, waves with wi == wj evaluate both orders of their own
 //     tile (i sums only, self pair excluded by index), waves with wi > wj
@@ -29,9 +29,9 @@
 //   i side of a once-evaluated pair: bf = 2 b, af = b + a;
 //   j side:                          bf = 0,   af = sum ((s/r)^6 - w);
 //   both orders (self tile):         bf = b,   af = a.
-// Per ordered pair this takes about 15.9 VALU issue slots (15.4 instructions,
-// v_rcp_f32 at half rate) against 22 for the sliced kernel, which evaluates
-// each pair twice.
+// The time-step variant's loop is 24 VALU instructions per unordered pair
+// (12.4 per ordered pair over the whole kernel, measured) against 21.5 per
+// ordered pair for the sliced kernel, which evaluates each pair twice.
 #include <cstdlib>
 
 #include "lj_device.h"
