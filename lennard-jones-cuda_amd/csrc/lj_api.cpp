@@ -54,7 +54,11 @@ void drop_graph(lj_ctx* c)
   c->graph_run = nullptr;
 }
 
-// b: which of the two per-workgroup total buffers the step's force pass fills
+// rows of one of the two block-row buffers
+size_t blk_rows(const lj_ctx* c) { return (size_t)std::max(c->nblk, step_rows(c->cfg.N)); }
+
+// b: which of the two buffers of per-workgroup totals (the step's force pass)
+// and of block rows (its integrator kernels) the step fills
 StepArgs step_args(lj_ctx* c, double dt, int b = 0)
 {
   StepArgs a{};
@@ -65,7 +69,7 @@ StepArgs step_args(lj_ctx* c, double dt, int b = 0)
   a.q = c->q;
   a.fpart = c->fpart;
   a.apart = c->apart;
-  a.blk = c->blk;
+  a.blk = c->blk + (is_fp64(c) ? 0 : (size_t)b * blk_rows(c) * kBlkFields);
   a.obs = c->obs;
   a.hist = c->hist;
   a.hist_base = c->hist_base;
@@ -82,8 +86,8 @@ StepArgs step_args(lj_ctx* c, double dt, int b = 0)
   a.canonical = c->cfg.canonical;
   a.S = fplan(c).S;
   a.Npad = fplan(c).Npad;
-  a.ew = c->step_ew ? c->step_ew + (size_t)b * 2 * c->plan3.nitems : nullptr;
-  a.nitems = c->step_ew ? c->plan3.nitems : 0;
+  a.ew = c->step_ew ? c->step_ew + (size_t)b * 2 * c->plan3.nitems_step : nullptr;
+  a.nitems = c->step_ew ? c->plan3.nitems_step : 0;
   a.nrow = is_fp64(c) ? c->nblk : step_rows(c->cfg.N);
   return a;
 }
@@ -157,12 +161,12 @@ int enqueue_force(lj_ctx* c, hipStream_t st, const float4* d_pos)
 // One step of a run of steps captured together.  The first step starts with
 // the drift; each later one finds it done by the previous step's last kernel
 // (drift_next).  With the pair-once pass the observables of a step are
-// computed by one extra workgroup of the next step's force pass, and only the
-// last step launches the stand-alone finaliser.  Steps alternate between the
-// two buffers of per-workgroup totals (b = step parity), so that the
-// finaliser of step k reads its totals while step k + 1 writes its own.  Every
-// kernel does the arithmetic of the one-step sequence, so a run gives the
-// bits of single steps.
+// computed by one extra block of the next step's first integrator kernel, and
+// only the last step launches the stand-alone finaliser.  Steps alternate
+// between the two buffers of per-workgroup totals and of block rows (b = step
+// parity), so that the finaliser of step k reads its sums while step k + 1
+// writes its own.  Every kernel does the arithmetic of the one-step sequence,
+// so a run gives the bits of single steps.
 int enqueue_step_part(lj_ctx* c, hipStream_t st, double dt, bool first, bool last, int b)
 {
   if (is_fp64(c)) return enqueue_step64(c, st, dt);
@@ -172,17 +176,13 @@ int enqueue_step_part(lj_ctx* c, hipStream_t st, double dt, bool first, bool las
   if (first) LJ_HIP(launch_drift(st, a));
   // the drift kernel already wrote the fixed-point copy for periodic boxes
   double* ew = const_cast<double*>(a.ew);
-  if (fused_fin && !first) {
-    const StepArgs prev = step_args(c, dt, b ^ 1);
-    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false, ew,
-                        &prev));
-  } else {
-    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false, ew));
-  }
+  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false, ew));
+  const StepArgs prev = step_args(c, dt, b ^ 1);
+  const StepArgs* fin = fused_fin && !first ? &prev : nullptr;
   if (c->cfg.canonical == LJ_EVN) {
-    LJ_HIP(launch_kick_evn(st, a, !last));
+    LJ_HIP(launch_kick_evn(st, a, !last, fin));
   } else {
-    LJ_HIP(launch_tvn_a(st, a));
+    LJ_HIP(launch_tvn_a(st, a, fin));
     LJ_HIP(launch_tvn_b(st, a, !last));
   }
   if (last || !fused_fin) LJ_HIP(launch_finalize_step(st, a));
@@ -391,17 +391,20 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
   A(dev_alloc(&c->fpart, npart));
   A(dev_alloc(&c->apart, npart));
   if (c->plan3.n3) {
-    std::vector<int2> it((size_t)c->plan3.nitems);
+    // the full variant's items, then the time step's
+    std::vector<int2> it((size_t)c->plan3.nitems + c->plan3.nitems_step);
     n3_items(c->plan3.ng, it.data());
+    n3_step_items(c->plan3, it.data() + c->plan3.nitems);
     A(dev_alloc(&c->n3items, it.size()));
     if (rc == LJ_OK &&
         hipMemcpy(c->n3items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice) != hipSuccess)
       A(fail(LJ_ERR_DEVICE, "item table upload failed"));
     c->plan3.items = c->n3items;
-    A(dev_alloc(&c->step_ew, (size_t)4 * c->plan3.nitems));  // two buffers
+    c->plan3.items_step = c->n3items + c->plan3.nitems;
+    A(dev_alloc(&c->step_ew, (size_t)4 * c->plan3.nitems_step));  // two buffers
   }
   // rows: nblk for the per-block kernels, step_rows(N) >= nblk for the step
-  A(dev_alloc(&c->blk, (size_t)std::max(c->nblk, step_rows(N)) * kBlkFields));
+  A(dev_alloc(&c->blk, 2 * blk_rows(c) * kBlkFields));
   A(dev_alloc(&c->obs, 1));
   A(dev_alloc(&c->hist_base, 1));
   A(dev_alloc(&c->ev, 2));

@@ -232,13 +232,28 @@ __device__ __forceinline__ Partial sum_range(const float4* __restrict__ fpart,
       p.a += a[k];
     }
   }
-  for (; s < s1; ++s) {
-    const float4 f = fpart[(size_t)s * Npad + i];
-    p.fx += f.x;
-    p.fy += f.y;
-    p.fz += f.z;
-    p.b += f.w;
-    if (WITH_A) p.a += apart[(size_t)s * Npad + i];
+  // the last, shorter batch is loaded the same way, all loads in flight at
+  // once, from slots clamped into the range (re-reading the last one) and
+  // added only up to s1: one slot at a time, each load's latency would add up
+  // (ranges of fewer than 16 slots: 32 slices or 40 groups over 4 lanes)
+  if (s < s1) {
+    float4 f[B];
+    float a[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int sk = min(s + k, s1 - 1);
+      f[k] = fpart[(size_t)sk * Npad + i];
+      a[k] = WITH_A ? apart[(size_t)sk * Npad + i] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const bool in = s + k < s1;
+      p.fx += in ? (double)f[k].x : 0.;
+      p.fy += in ? (double)f[k].y : 0.;
+      p.fz += in ? (double)f[k].z : 0.;
+      p.b += in ? (double)f[k].w : 0.;
+      p.a += in ? (double)a[k] : 0.;
+    }
   }
   return p;
 }
@@ -348,12 +363,17 @@ __device__ __forceinline__ void step_observables(const StepArgs& a, int nrow, bo
 {
   constexpr int NT = 64 * NW;
   double v[3] = {0., 0., 0.};
+  // unrolled so that each thread's loads are in flight together (this block
+  // runs beside the particle blocks of an integrator kernel and should not
+  // outlast them)
+#pragma unroll 4
   for (int r = threadIdx.x; r < nrow; r += NT) {
     v[0] += a.blk[(size_t)r * kBlkFields + BLK_K2];
     v[1] += a.blk[(size_t)r * kBlkFields + BLK_E];
     v[2] += a.blk[(size_t)r * kBlkFields + BLK_B];
   }
   if (force_terms)  // per-workgroup totals of the time step's force pass
+#pragma unroll 8
     for (int r = threadIdx.x; r < a.nitems; r += NT) {
       v[1] += a.ew[2 * (size_t)r];
       v[2] += a.ew[2 * (size_t)r + 1];

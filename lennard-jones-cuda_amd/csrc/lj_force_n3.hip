@@ -45,6 +45,18 @@ using namespace dev;
 constexpr int kTile = 128;  // particles per wave tile
 constexpr int kAccPerWave = 2 * 4 * 64;  // sub-tiles x components x tile
 
+// LDS of one workgroup, declared once by each kernel: the diagonal-pair
+// workgroups of the time step use the same bytes with their own layout
+template <int W>
+struct N3Lds {
+  static constexpr int kG = kTile * W;
+  static constexpr int kAcc = W * W * kAccPerWave > W * 5 * kG ? W * W * kAccPerWave : W * 5 * kG;
+  uint4 pos[W * 2 * 128];  // [wj][sub-tile][doubled 64]
+  // j sums [wave][sub-tile][component][64], then i sums [wj][component][G]
+  float acc[kAcc];
+  double ew[W * W][2];
+};
+
 template <bool PERIODIC, bool TIES, bool GUARD>
 __device__ __forceinline__ void terms(const PosT<PERIODIC>& pi, const uint4& pj, float C1, bool ok,
                                       float& dx, float& dy, float& dz, float& s, float& e6,
@@ -186,6 +198,250 @@ __device__ __forceinline__ void encode(const Acc& a, int cls, float (&v)[5])
   }
 }
 
+// rotate by k lanes: lane l receives lane (l + k) mod 64 (full execution mask)
+__device__ __forceinline__ float rotk(float v, int k)
+{
+  const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + k) & 63) << 2, __float_as_int(v)));
+}
+
+// One step of two once-evaluated pair streams of a wave: the lane's i particle
+// pi1 meets j slot (lane + t) mod 64 of sub-tile P1, pi2 that of P2 (LDS,
+// doubled).  R1 / R2 hold the sums of those j slots (minus the force) and pass
+// one lane down afterwards, as in pair_tile.  GUARD: pairs count only with a
+// valid i (vi1, vi2) and j, and with HALF only on lanes 0-31 (the step t = 32
+// of a triangle, where lanes l and l + 32 meet the same pair).
+template <bool PERIODIC, bool GUARD, bool HALF>
+__device__ __forceinline__ void dual_step(const uint4* P1, const uint4* P2, int lane, int t,
+                                          const PosT<PERIODIC>& pi1, const PosT<PERIODIC>& pi2,
+                                          bool vi1, bool vi2, float C1, float (&a1)[3],
+                                          float (&a2)[3], float& ea, float& eb, float (&R1)[3],
+                                          float (&R2)[3])
+{
+  const uint4 pj1 = P1[lane + t], pj2 = P2[lane + t];
+  bool ok1 = true, ok2 = true;
+  if constexpr (GUARD) {
+    const bool h = !HALF || lane < 32;
+    ok1 = vi1 && pj1.w != 0u && h;
+    ok2 = vi2 && pj2.w != 0u && h;
+  }
+  float dx1, dy1, dz1, s1, e1, w1, dx2, dy2, dz2, s2, e2, w2;
+  terms<PERIODIC, false, GUARD>(pi1, pj1, C1, ok1, dx1, dy1, dz1, s1, e1, w1);
+  terms<PERIODIC, false, GUARD>(pi2, pj2, C1, ok2, dx2, dy2, dz2, s2, e2, w2);
+  a1[0] = __builtin_fmaf(dx1, s1, a1[0]);
+  a1[1] = __builtin_fmaf(dy1, s1, a1[1]);
+  a1[2] = __builtin_fmaf(dz1, s1, a1[2]);
+  a2[0] = __builtin_fmaf(dx2, s2, a2[0]);
+  a2[1] = __builtin_fmaf(dy2, s2, a2[1]);
+  a2[2] = __builtin_fmaf(dz2, s2, a2[2]);
+  ea += e1 + e2;
+  eb += w1 + w2;
+  R1[0] = rot1(__builtin_fmaf(-dx1, s1, R1[0]));
+  R1[1] = rot1(__builtin_fmaf(-dy1, s1, R1[1]));
+  R1[2] = rot1(__builtin_fmaf(-dz1, s1, R1[2]));
+  R2[0] = rot1(__builtin_fmaf(-dx2, s2, R2[0]));
+  R2[1] = rot1(__builtin_fmaf(-dy2, s2, R2[1]));
+  R2[2] = rot1(__builtin_fmaf(-dz2, s2, R2[2]));
+}
+
+// The pairs inside each 64-particle sub-tile of two sub-tiles at once
+// (triangles): at step t = 1..32 lane l meets slot (l + t) mod 64, so every
+// unordered pair of the sub-tile is met once (t = 32 on lanes 0-31 only).
+// Afterwards lane l holds the j sums of slot (l + 33) mod 64.
+// The loops of these two-stream steps advance two steps per iteration: with
+// more the compiler reads further ahead from LDS and, at the 64 registers that
+// eight waves per SIMD allow, spilled to scratch (the whole kernel then ran
+// with a private segment).
+template <bool PERIODIC, bool GUARD>
+__device__ __forceinline__ void tri_pair(const uint4* P1, const uint4* P2, int lane,
+                                         const PosT<PERIODIC>& pi1, const PosT<PERIODIC>& pi2,
+                                         bool vi1, bool vi2, float C1, float (&a1)[3],
+                                         float (&a2)[3], float& ea, float& eb, float (&R1)[3],
+                                         float (&R2)[3])
+{
+#pragma nounroll
+  for (int c = 0; c < 16; ++c) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int t = 2 * c + u + 1;
+      if (u == 1 && c == 15)
+        dual_step<PERIODIC, true, true>(P1, P2, lane, t, pi1, pi2, vi1, vi2, C1, a1, a2, ea, eb, R1,
+                                                  R2);
+      else
+        dual_step<PERIODIC, GUARD, false>(P1, P2, lane, t, pi1, pi2, vi1, vi2, C1, a1, a2, ea, eb, R1,
+                                                  R2);
+    }
+  }
+}
+
+// Every unordered pair of one group of 256 that the pair tile of its diagonal
+// item leaves out (those inside sub-tiles 0-3 and between sub-tiles 0-1 and
+// 2-3), once, by one wave; f[k] receives the sums of the lane's particle of
+// sub-tile k (particles base + 64 k + [0, 64), at P + 128 k).  ea / eb: the
+// wave's sums of (s/r)^6 and w.
+//   1. triangles of sub-tiles 1 and 3; their i and (aligned) j sums become
+//      the starting j sums of
+//   2. sub-tile 0 x sub-tile 1 and sub-tile 2 x sub-tile 3 (64 steps), after
+//      which lane l holds the totals of slot l of sub-tiles 1 and 3;
+//   3. triangles of sub-tiles 0 and 2, adding to the i sums of step 2.
+template <bool PERIODIC, bool GUARD>
+__device__ __forceinline__ void tri_tiles(const uint4* P, const uint4* __restrict__ q,
+                                          const float4* __restrict__ pf, int base, int N, float C1,
+                                          int lane, float (&f)[4][3], float& ea, float& eb)
+{
+  const int i0 = base + lane;
+  const bool v0 = i0 < N, v1 = i0 + 64 < N, v2 = i0 + 128 < N, v3 = i0 + 192 < N;
+  {
+    const PosT<PERIODIC> pi1 = load_pos<PERIODIC>(q, pf, v1 ? i0 + 64 : N - 1);
+    const PosT<PERIODIC> pi3 = load_pos<PERIODIC>(q, pf, v3 ? i0 + 192 : N - 1);
+    float h1[3] = {0.f, 0.f, 0.f}, h3[3] = {0.f, 0.f, 0.f};
+    tri_pair<PERIODIC, GUARD>(P + 128, P + 384, lane, pi1, pi3, v1, v3, C1, f[1], f[3], ea, eb, h1,
+                              h3);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      f[1][c] += rotk(h1[c], 31);
+      f[3][c] += rotk(h3[c], 31);
+    }
+  }
+  const PosT<PERIODIC> pi0 = load_pos<PERIODIC>(q, pf, v0 ? i0 : N - 1);
+  const PosT<PERIODIC> pi2 = load_pos<PERIODIC>(q, pf, v2 ? i0 + 128 : N - 1);
+#pragma unroll 2
+  for (int t = 0; t < 64; ++t)
+    dual_step<PERIODIC, GUARD, false>(P + 128, P + 384, lane, t, pi0, pi2, v0, v2, C1, f[0], f[2],
+                                      ea, eb, f[1], f[3]);
+  float h0[3] = {0.f, 0.f, 0.f}, h2[3] = {0.f, 0.f, 0.f};
+  tri_pair<PERIODIC, GUARD>(P, P + 256, lane, pi0, pi2, v0, v2, C1, f[0], f[2], ea, eb, h0, h2);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    f[0][c] += rotk(h0[c], 31);
+    f[2][c] += rotk(h2[c], 31);
+  }
+}
+
+// Two diagonal items (A, A) and (B, B) of the time step's variant (W = 2) in
+// one workgroup, every unordered pair of a group once: waves 0 / 1 pair the
+// first 128 particles of A / B with the last 128 (pair_tile), waves 2 / 3 take
+// the remaining pairs of A / B (tri_tiles) and pass their sums through LDS to
+// waves 0 / 1, which add them to their own and write slot A / B of the
+// group's particles.  All four waves carry the same work, so with the off-diagonal items a group count of 64 gives
+// 2016 + 32 = 2048 equal workgroups, as many as are resident at once (eight
+// 4-wave workgroups per CU): the 2080 items of the other decomposition leave
+// 32 for a second round.  B < 0: A alone (waves 1 and 3 idle).
+template <bool PERIODIC>
+__device__ __forceinline__ void n3_diag_pair(N3Lds<2>& lds, const uint4* __restrict__ q,
+                                             const float4* __restrict__ pf,
+                                             float4* __restrict__ fpart, double* __restrict__ ew,
+                                             int N, int Npad, float C1, int A, int B)
+{
+  static_assert(sizeof(N3Lds<2>::acc) >= 512 * sizeof(uint4), "group B's positions fit in acc");
+  const int T = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(T >> 6);
+  const int lane = T & 63;
+  uint4* const posA = lds.pos;                            // [sub-tile][doubled 64]
+  uint4* const posB = reinterpret_cast<uint4*>(lds.acc);  // same layout
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int grp = r ? B : A;
+    if (grp < 0) continue;
+    const int j = 256 * grp + T;
+    const int jl = j < N ? j : 0;
+    uint4 p;
+    if constexpr (PERIODIC) {
+      p = q[jl];
+    } else {
+      const float4 f = pf[jl];
+      p = make_uint4(__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z), 0u);
+    }
+    p.w = j < N ? 1u : 0u;
+    uint4* d = (r ? posB : posA) + (T >> 6) * 128;
+    d[T & 63] = p;
+    d[(T & 63) + 64] = p;
+  }
+  __syncthreads();
+
+  const int grp = (wave & 1) ? B : A;
+  const uint4* P = (wave & 1) ? posB : posA;
+  const int base = 256 * grp;
+  const bool partial = base + 256 > N;
+  double se = 0., sw = 0.;
+  // per lane: the sums of its particle of sub-tile k = 0..3 of the group
+  float f[4][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  if (grp >= 0) {
+    if (wave < 2) {
+      const int i1 = base + lane, i2 = i1 + 64, jt0 = base + 128;
+      if (jt0 < N) {  // the last 128 hold a particle
+        const bool v1 = i1 < N, v2 = i2 < N;
+        const PosT<PERIODIC> pi1 = load_pos<PERIODIC>(q, pf, v1 ? i1 : N - 1);
+        const PosT<PERIODIC> pi2 = load_pos<PERIODIC>(q, pf, v2 ? i2 : N - 1);
+        Acc a1, a2;
+        acc_zero(a1);
+        acc_zero(a2);
+        float J[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+        if (partial)
+          pair_tile<PERIODIC, false, true, false>(P + 256, lane, pi1, pi2, v1, v2, C1,
+                                                  jt0 + 64 < N ? 2 : 1, a1, a2, J);
+        else
+          pair_tile<PERIODIC, false, false, false>(P + 256, lane, pi1, pi2, v1, v2, C1, 2, a1, a2,
+                                                   J);
+        f[0][0] = a1.fx, f[0][1] = a1.fy, f[0][2] = a1.fz;
+        f[1][0] = a2.fx, f[1][1] = a2.fy, f[1][2] = a2.fz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          f[2][c] = J[0][c];
+          f[3][c] = J[1][c];
+        }
+        se = (double)a1.a + (double)a2.a;
+        sw = (double)a1.b + (double)a2.b;
+      }
+    } else {
+      float ea = 0.f, eb = 0.f;
+      if (partial)
+        tri_tiles<PERIODIC, true>(P, q, pf, base, N, C1, lane, f, ea, eb);
+      else
+        tri_tiles<PERIODIC, false>(P, q, pf, base, N, C1, lane, f, ea, eb);
+      se = (double)ea;
+      sw = (double)eb;
+    }
+  }
+  // the triangle waves' sums to the pair-tile waves through LDS (the position
+  // tiles are free once every wave is done): [group][sub-tile][component][64]
+  float* const tri = reinterpret_cast<float*>(lds.pos);
+  static_assert(sizeof(lds.pos) + sizeof(lds.acc) >= 2 * 4 * 3 * 64 * sizeof(float), "room");
+  __syncthreads();
+  if (wave >= 2 && grp >= 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tri[(((wave & 1) * 4 + k) * 3 + c) * 64 + lane] = f[k][c];
+  }
+  __syncthreads();
+  if (wave < 2 && grp >= 0) {
+    float4* o = fpart + (size_t)grp * Npad + base;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float* t = tri + ((wave * 4 + k) * 3) * 64 + lane;
+      if (base + 64 * k + lane < N)
+        store_part(o + 64 * k + lane, f[k][0] + t[0], f[k][1] + t[64], f[k][2] + t[128], 0.f);
+    }
+  }
+  // totals in the ordered-pair convention: every pair here was met once
+  double te = sw - se, tw = 2. * sw;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    te += __shfl_xor(te, o, 64);
+    tw += __shfl_xor(tw, o, 64);
+  }
+  if (lane == 0) {
+    lds.ew[wave][0] = te;
+    lds.ew[wave][1] = tw;
+  }
+  __syncthreads();
+  if (T == 0) {
+    ew[0] = lds.ew[0][0] + lds.ew[1][0] + lds.ew[2][0] + lds.ew[3][0];
+    ew[1] = lds.ew[0][1] + lds.ew[1][1] + lds.ew[2][1] + lds.ew[3][1];
+  }
+}
+
 // W x W waves per workgroup, groups of G = 128 W particles.  Eight waves per
 // SIMD (at most 64 VGPRs; the loop body is unchanged by the limit, no spills).
 // ENERGY = false (the time step): no per-particle energies; the partial
@@ -196,16 +452,15 @@ __device__ __forceinline__ void encode(const Acc& a, int cls, float (&v)[5])
 // One group pair (g, h) of one box: q/pf/fpart/apart point at the box's
 // arrays, ew (ENERGY = false) at the item's two totals.
 template <bool PERIODIC, bool TIES, int W, bool ENERGY>
-__device__ __forceinline__ void n3_item(const uint4* __restrict__ q, const float4* __restrict__ pf,
+__device__ __forceinline__ void n3_item(N3Lds<W>& lds, const uint4* __restrict__ q,
+                                        const float4* __restrict__ pf,
                                         float4* __restrict__ fpart, float* __restrict__ apart,
                                         double* __restrict__ ew, int N, int Npad, float C1, int g,
                                         int h)
 {
   constexpr int kG = kTile * W;
-  constexpr int kAcc = W * W * kAccPerWave > W * 5 * kG ? W * W * kAccPerWave : W * 5 * kG;
-  __shared__ uint4 s_pos[W * 2 * 128];  // [wj][sub-tile][doubled 64]
-  // j sums [wave][sub-tile][component][64], then i sums [wj][component][G]
-  __shared__ float s_acc[kAcc];
+  uint4* const s_pos = lds.pos;
+  float* const s_acc = lds.acc;
   const int T = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(T >> 6);
   const int lane = T & 63;
@@ -279,7 +534,7 @@ __device__ __forceinline__ void n3_item(const uint4* __restrict__ q, const float
   if constexpr (!ENERGY) {
     // E and W totals of the item in the ordered-pair convention: pairs of a
     // pair tile count twice, those of a self tile (both orders) once
-    __shared__ double s_ew[W * W][2];
+    double(*s_ew)[2] = lds.ew;
     const double m = cls == 1 ? 2. : (cls == 2 ? 1. : 0.);
     const double se = (double)a1.a + (double)a2.a, sw = (double)a1.b + (double)a2.b;
     double te = m * 0.5 * (sw - se), tw = m * sw;
@@ -348,28 +603,23 @@ __device__ __forceinline__ void n3_item(const uint4* __restrict__ q, const float
   }
 }
 
-// FIN: one workgroup besides the items computes the observables of the
-// previous step from its block rows and force totals (`fin`: that step's
-// arguments, its own ew buffer), in place of a separate finaliser launch
-// between the steps.  It is block 0, so that it is dispatched with the first
-// wave of workgroups: as the last block it would wait for a free slot and
-// finish after the items.
-template <bool PERIODIC, bool TIES, int W, bool ENERGY, bool FIN>
+template <bool PERIODIC, bool TIES, int W, bool ENERGY>
 __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8))) void k_force_n3(
     const uint4* __restrict__ q, const float4* __restrict__ pf, const int2* __restrict__ items,
     float4* __restrict__ fpart, float* __restrict__ apart, double* __restrict__ ew, int N, int Npad,
-    float C1, StepArgs fin)
+    float C1)
 {
-  if constexpr (FIN) {
-    if (blockIdx.x == 0) {
-      step_observables<W * W>(fin, fin.nrow, true, true);
+  __shared__ N3Lds<W> lds;
+  const int k = blockIdx.x;  // item
+  const int2 it = items[k];
+  if constexpr (!ENERGY && W == 2) {
+    if (it.y < 0) {  // diagonal pair (it.x, -2 - it.y), the second one absent when -1
+      n3_diag_pair<PERIODIC>(lds, q, pf, fpart, ew + 2 * (size_t)k, N, Npad, C1, it.x, -2 - it.y);
       return;
     }
   }
-  const int k = FIN ? (int)blockIdx.x - 1 : (int)blockIdx.x;  // item
-  const int2 it = items[k];
-  n3_item<PERIODIC, TIES, W, ENERGY>(q, pf, fpart, apart, ENERGY ? nullptr : ew + 2 * (size_t)k, N,
-                                     Npad, C1, it.x, it.y);
+  n3_item<PERIODIC, TIES, W, ENERGY>(lds, q, pf, fpart, apart, ENERGY ? nullptr : ew + 2 * (size_t)k,
+                                     N, Npad, C1, it.x, it.y);
 }
 
 // batched ensemble: item = (box, g, h) over all boxes; each box's partials at
@@ -382,9 +632,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     float* __restrict__ apart, const BoxParam* __restrict__ box, const BatchItem* __restrict__ items,
     double* __restrict__ ew)
 {
+  __shared__ N3Lds<2> lds;
   const BatchItem w = items[blockIdx.x];
   const BoxParam bp = box[w.box];
-  n3_item<PERIODIC, TIES, 2, ENERGY>(q + bp.off, pf + bp.off, fpart + bp.pbase, apart + bp.pbase,
+  n3_item<PERIODIC, TIES, 2, ENERGY>(lds, q + bp.off, pf + bp.off, fpart + bp.pbase, apart + bp.pbase,
                                      ENERGY ? nullptr : ew + 2 * (size_t)w.e, bp.N, bp.Npad, bp.C1,
                                      w.i0, w.s);
 }
@@ -413,6 +664,8 @@ ForcePlan make_n3_plan(int N)
   pl.jchunk = G;
   pl.Npad = (N + 63) & ~63;
   pl.nitems = pl.ng * (pl.ng + 1) / 2;
+  // the time step pairs its diagonal items (n3_step_items)
+  pl.nitems_step = W == 2 ? pl.ng * (pl.ng - 1) / 2 + (pl.ng + 1) / 2 : pl.nitems;
   return pl;
 }
 
@@ -436,40 +689,44 @@ void n3_items(int ng, int2* out)
   for (int g = 0; g < ng; ++g) out[k++] = make_int2(g, g);
 }
 
-namespace {
-
-template <int W, bool ENERGY, bool FIN>
-void launch_n3_f(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
-                 const float4* posf, float4* fpart, float* apart, double* ew, bool ties,
-                 const StepArgs& fin)
+void n3_step_items(const ForcePlan& pl, int2* out)
 {
-  dim3 grid(pl.nitems + (FIN ? 1 : 0)), block(64 * W * W);
-  if (ps.periodic) {
-    if (ties)
-      hipLaunchKernelGGL((k_force_n3<true, true, W, ENERGY, FIN>), grid, block, 0, st, q, posf,
-                         pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1, fin);
-    else
-      hipLaunchKernelGGL((k_force_n3<true, false, W, ENERGY, FIN>), grid, block, 0, st, q, posf,
-                         pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1, fin);
-  } else {
-    hipLaunchKernelGGL((k_force_n3<false, false, W, ENERGY, FIN>), grid, block, 0, st, q, posf,
-                       pl.items, fpart, apart, ew, pl.N, pl.Npad, ps.C1, fin);
+  if (pl.nitems_step == pl.nitems) {
+    n3_items(pl.ng, out);
+    return;
   }
+  // the diagonal items in pairs (g, g + half) encoded as (g, -2 - (g + half)),
+  // a last odd one alone as (g, -1), first: when the workgroups outnumber the
+  // resident slots the longer ones should start in the first round; then the
+  // off-diagonal items
+  const int ng = pl.ng, half = (ng + 1) / 2;
+  int k = 0;
+  for (int g = 0; g < half; ++g) out[k++] = make_int2(g, g + half < ng ? -2 - (g + half) : -1);
+  for (int g = 0; g < ng; ++g)
+    for (int h = g + 1; h < ng; ++h) out[k++] = make_int2(g, h);
 }
+
+namespace {
 
 template <int W, bool ENERGY>
 hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
-                       const float4* posf, float4* fpart, float* apart, double* ew, bool ties,
-                       const StepArgs* fin)
+                       const float4* posf, float4* fpart, float* apart, double* ew, bool ties)
 {
-  if constexpr (!ENERGY) {  // the finaliser workgroup rides on the time step's variant only
-    if (fin) {
-      launch_n3_f<W, ENERGY, true>(st, pl, ps, q, posf, fpart, apart, ew, ties, *fin);
-      return hipGetLastError();
-    }
+  // the time step's variant runs its own item list (diagonal items paired)
+  const int nitems = ENERGY ? pl.nitems : pl.nitems_step;
+  const int2* items = ENERGY ? pl.items : pl.items_step;
+  dim3 grid(nitems), block(64 * W * W);
+  if (ps.periodic) {
+    if (ties)
+      hipLaunchKernelGGL((k_force_n3<true, true, W, ENERGY>), grid, block, 0, st, q, posf, items,
+                         fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+    else
+      hipLaunchKernelGGL((k_force_n3<true, false, W, ENERGY>), grid, block, 0, st, q, posf, items,
+                         fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+  } else {
+    hipLaunchKernelGGL((k_force_n3<false, false, W, ENERGY>), grid, block, 0, st, q, posf, items,
+                       fpart, apart, ew, pl.N, pl.Npad, ps.C1);
   }
-  if (fin) return hipErrorInvalidValue;
-  launch_n3_f<W, ENERGY, false>(st, pl, ps, q, posf, fpart, apart, ew, ties, StepArgs{});
   return hipGetLastError();
 }
 
@@ -504,16 +761,15 @@ hipError_t launch_batch_force_n3(hipStream_t st, const BatchArgs& a, bool period
 
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,
-                           bool ties, double* ew, const StepArgs* fin)
+                           bool ties, double* ew)
 {
-  // ew != null: the time step's variant (force partials + per-workgroup totals),
-  // with the previous step's observables when fin != null
+  // ew != null: the time step's variant (force partials + per-workgroup totals)
   if (ew) {
-    if (pl.n3 == 2) return launch_n3_w<2, false>(st, pl, ps, q, posf, fpart, apart, ew, false, fin);
-    return launch_n3_w<4, false>(st, pl, ps, q, posf, fpart, apart, ew, false, fin);
+    if (pl.n3 == 2) return launch_n3_w<2, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
+    return launch_n3_w<4, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
   }
-  if (pl.n3 == 2) return launch_n3_w<2, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties, nullptr);
-  return launch_n3_w<4, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties, nullptr);
+  if (pl.n3 == 2) return launch_n3_w<2, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
+  return launch_n3_w<4, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
 }
 
 }  // namespace lj

@@ -38,6 +38,10 @@ struct ForcePlan {
   int ng = 0;
   int nitems = 0;
   const int2* items = nullptr;  // device table of (g, h) group pairs
+  // the time step's variant: nitems_step workgroups (the diagonal items in
+  // pairs with W = 2)
+  int nitems_step = 0;
+  const int2* items_step = nullptr;
 };
 ForcePlan make_force_plan(int N, int jslices_override, int ipl);
 ForcePlan make_n3_plan(int N);
@@ -45,6 +49,8 @@ ForcePlan make_n3_plan(int N);
 bool n3_eligible(int N);
 // the ng (ng + 1) / 2 group pairs of the pair-once kernel, in launch order
 void n3_items(int ng, int2* out);
+// the pair-once workgroups of the time step's variant (nitems_step entries)
+void n3_step_items(const ForcePlan& pl, int2* out);
 
 // Scalars of the pair kernel for one box.
 struct PairScalars {
@@ -69,11 +75,10 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 struct StepArgs;
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
-                        bool compensated, bool ties, double* step_ew = nullptr,
-                        const StepArgs* fin = nullptr);
+                        bool compensated, bool ties, double* step_ew = nullptr);
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,
-                           bool ties, double* ew, const StepArgs* fin);
+                           bool ties, double* ew);
 // sum slices -> force (w = e_i); block partials of e, b
 hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                                 const float4* fpart, const float* apart, float4* force,
@@ -116,8 +121,11 @@ constexpr int kRowParticles = 64;
 inline int step_rows(int N) { return (N + kRowParticles - 1) / kRowParticles; }
 hipError_t launch_drift(hipStream_t st, const StepArgs& a);
 // drift_next: also the next step's drift (as launch_drift) for each particle
-hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a, bool drift_next = false);
-hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a);
+// fin != null (pair-once time step only): one extra block computes the
+// observables of the previous step of the run
+hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a, bool drift_next = false,
+                           const StepArgs* fin = nullptr);
+hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a, const StepArgs* fin = nullptr);
 hipError_t launch_tvn_b(hipStream_t st, const StepArgs& a, bool drift_next = false);
 // observables of the step; advance t/step, write history
 hipError_t launch_finalize_step(hipStream_t st, const StepArgs& a);

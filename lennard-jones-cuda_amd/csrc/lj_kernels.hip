@@ -163,16 +163,27 @@ __global__ __launch_bounds__(kBlock) void k_drift(StepArgs a)
 // DRIFT: the kernel that ends a step also does the next step's drift for its
 // particle (the arithmetic of k_drift on the values it just wrote), so that a
 // run of steps needs no separate drift launch; MAKEQ: periodic fixed-point copy.
+// FIN: block 0 computes the observables of the previous step of the run
+// (`fin`: that step's arguments, whose block rows and force totals sit in the
+// other of the two buffers) in place of a finaliser launch between the steps;
+// the particle rows start at block 1.
 constexpr int kSumLanes = 4;
 constexpr int kSumBlock = kRowParticles * kSumLanes;
 static_assert(kSumBlock == kBlock, "the row sums assume kBlock threads");
 
-template <bool WITH_A, bool DRIFT, bool MAKEQ>
-__global__ __launch_bounds__(kSumBlock) void k_kick_evn(StepArgs a)
+template <bool WITH_A, bool DRIFT, bool MAKEQ, bool FIN>
+__global__ __launch_bounds__(kSumBlock) void k_kick_evn(StepArgs a, StepArgs fin)
 {
 #pragma clang fp contract(off)
+  if constexpr (FIN) {
+    if (blockIdx.x == 0) {
+      step_observables<kSumBlock / 64>(fin, fin.nrow, true, true);
+      return;
+    }
+  }
+  const int row = (int)blockIdx.x - (FIN ? 1 : 0);
   const int r = threadIdx.x % kSumLanes;
-  const int i = blockIdx.x * kRowParticles + threadIdx.x / kSumLanes;
+  const int i = row * kRowParticles + threadIdx.x / kSumLanes;
   const bool lead = r == 0 && i < a.N;
   // the particle's own loads go out ahead of the slot sums (their latencies overlap)
   float4 v = {}, p = {};
@@ -208,15 +219,22 @@ __global__ __launch_bounds__(kSumBlock) void k_kick_evn(StepArgs a)
     a.vel[i] = v;
   }
   const int fld[3] = {BLK_K2, BLK_E, BLK_B};
-  block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+  block_sum_store<3>(v3, a.blk + (size_t)row * kBlkFields, fld);
 }
 
-template <bool WITH_A>
-__global__ __launch_bounds__(kSumBlock) void k_tvn_a(StepArgs a)
+template <bool WITH_A, bool FIN>
+__global__ __launch_bounds__(kSumBlock) void k_tvn_a(StepArgs a, StepArgs fin)
 {
 #pragma clang fp contract(off)
+  if constexpr (FIN) {
+    if (blockIdx.x == 0) {
+      step_observables<kSumBlock / 64>(fin, fin.nrow, true, true);
+      return;
+    }
+  }
+  const int row = (int)blockIdx.x - (FIN ? 1 : 0);
   const int r = threadIdx.x % kSumLanes;
-  const int i = blockIdx.x * kRowParticles + threadIdx.x / kSumLanes;
+  const int i = row * kRowParticles + threadIdx.x / kSumLanes;
   const bool lead = r == 0 && i < a.N;
   // the particle's own loads go out ahead of the slot sums (their latencies overlap)
   float4 tf = {}, v = {};
@@ -246,7 +264,7 @@ __global__ __launch_bounds__(kSumBlock) void k_tvn_a(StepArgs a)
     v3[2] = pp.b;
   }
   const int fld[3] = {BLK_TV2, BLK_E, BLK_B};
-  block_sum_store<3>(v3, a.blk + (size_t)blockIdx.x * kBlkFields, fld);
+  block_sum_store<3>(v3, a.blk + (size_t)row * kBlkFields, fld);
 }
 
 // one particle per thread; each wave (kRowParticles particles) writes the K2
@@ -712,12 +730,11 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
-                        bool compensated, bool ties, double* step_ew, const StepArgs* fin)
+                        bool compensated, bool ties, double* step_ew)
 {
   const bool ga = ps.thr > 0.f;
   if (pl.n3 && !compensated && !ga)
-    return launch_force_n3(st, pl, ps, q, posf, fpart, apart, ties, step_ew, step_ew ? fin : nullptr);
-  if (fin) return hipErrorInvalidValue;  // the finaliser workgroup exists in the pair-once pass only
+    return launch_force_n3(st, pl, ps, q, posf, fpart, apart, ties, step_ew);
   if (ps.periodic) {
     if (ties)
       return ga ? launch_force_c<true, true, true>(st, pl, ps, q, posf, fpart, apart, compensated)
@@ -760,30 +777,43 @@ hipError_t launch_drift(hipStream_t st, const StepArgs& a)
   return hipGetLastError();
 }
 
-template <bool WITH_A>
-void kick_evn_w(hipStream_t st, const StepArgs& a, bool drift_next)
+template <bool WITH_A, bool FIN>
+void kick_evn_w(hipStream_t st, const StepArgs& a, bool drift_next, const StepArgs& fin)
 {
-  const dim3 g(a.nrow), b(kSumBlock);
-  if (!drift_next) hipLaunchKernelGGL((k_kick_evn<WITH_A, false, false>), g, b, 0, st, a);
-  else if (a.bc == LJ_BC_PERIODIC) hipLaunchKernelGGL((k_kick_evn<WITH_A, true, true>), g, b, 0, st, a);
-  else hipLaunchKernelGGL((k_kick_evn<WITH_A, true, false>), g, b, 0, st, a);
+  const dim3 g(a.nrow + (FIN ? 1 : 0)), b(kSumBlock);
+  if (!drift_next)
+    hipLaunchKernelGGL((k_kick_evn<WITH_A, false, false, FIN>), g, b, 0, st, a, fin);
+  else if (a.bc == LJ_BC_PERIODIC)
+    hipLaunchKernelGGL((k_kick_evn<WITH_A, true, true, FIN>), g, b, 0, st, a, fin);
+  else
+    hipLaunchKernelGGL((k_kick_evn<WITH_A, true, false, FIN>), g, b, 0, st, a, fin);
 }
 
-hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a, bool drift_next)
+hipError_t launch_kick_evn(hipStream_t st, const StepArgs& a, bool drift_next, const StepArgs* fin)
 {
-  if (a.nitems)  // force-only partials of the pair-once time-step pass
-    kick_evn_w<false>(st, a, drift_next);
-  else
-    kick_evn_w<true>(st, a, drift_next);
+  if (a.nitems) {  // force-only partials of the pair-once time-step pass
+    if (fin)
+      kick_evn_w<false, true>(st, a, drift_next, *fin);
+    else
+      kick_evn_w<false, false>(st, a, drift_next, a);
+  } else {
+    if (fin) return hipErrorInvalidValue;  // the observables block rides on the pair-once step only
+    kick_evn_w<true, false>(st, a, drift_next, a);
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a)
+hipError_t launch_tvn_a(hipStream_t st, const StepArgs& a, const StepArgs* fin)
 {
-  if (a.nitems)
-    hipLaunchKernelGGL(k_tvn_a<false>, dim3(a.nrow), dim3(kSumBlock), 0, st, a);
-  else
-    hipLaunchKernelGGL(k_tvn_a<true>, dim3(a.nrow), dim3(kSumBlock), 0, st, a);
+  if (a.nitems) {
+    if (fin)
+      hipLaunchKernelGGL((k_tvn_a<false, true>), dim3(a.nrow + 1), dim3(kSumBlock), 0, st, a, *fin);
+    else
+      hipLaunchKernelGGL((k_tvn_a<false, false>), dim3(a.nrow), dim3(kSumBlock), 0, st, a, a);
+  } else {
+    if (fin) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_tvn_a<true, false>), dim3(a.nrow), dim3(kSumBlock), 0, st, a, a);
+  }
   return hipGetLastError();
 }
 
