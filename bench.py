@@ -357,9 +357,9 @@ def main():
                      "newton3": g["pair_once"],
                      "pair_evaluations_per_launch": (N * (N - 1) // 2 if g["pair_once"]
                                                      else N * N),
-                     # SQ_INSTS_VALU x 64 / N^2 (12.44 pair-once, 21.46 sliced, measured)
+                     # SQ_INSTS_VALU x 64 / N^2 (12.35 pair-once, 21.46 sliced, measured)
                      # + the half-rate v_rcp_f32 (one per unordered / ordered pair)
-                     "issue_slots_per_ordered_pair": 12.9 if g["pair_once"] else 22.5,
+                     "issue_slots_per_ordered_pair": 12.85 if g["pair_once"] else 22.5,
                      "sliced_kernel_us": sliced_us,
                      "sliced_kernel_tflops": FLOP_PER_PAIR * pairs / (sliced_us * 1e-6) / 1e12},
         "cpu_baseline": cpu,

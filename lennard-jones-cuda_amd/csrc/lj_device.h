@@ -352,7 +352,7 @@ __device__ __forceinline__ float kin2(const float4& v)
 
 // Observables of a step from its block rows: CalculateParameters
 // (MDSystem.cpp:325-359).  Run by one block of NW waves (the stand-alone
-// finaliser, or one extra workgroup of the next step's force pass); the
+// finaliser, or block 0 of the next step's first integrator kernel); the
 // reduction tree depends on NW only, so both give the same bits for NW = 4.
 // force_terms: V and W from the block rows' E/B fields plus the per-workgroup
 // totals of the force pass (a.ew, a.nitems); advance: t and the step counter
