@@ -206,7 +206,16 @@ struct Partial {
 // are issued 16 at a time ahead of the (sequential, fixed-order) adds: few
 // threads per particle leave few waves in flight, so the sum is bound by load
 // latency unless each thread keeps many loads outstanding.
-// WITH_A = false: force-only partials (apart is not read; a = 0).
+// WITH_A = false: force-only partials of three floats (store_part3; apart is
+// not read; a = b = 0).
+template <bool WITH_A>
+__device__ __forceinline__ float4 load_entry(const float4* __restrict__ fpart, size_t e)
+{
+  if constexpr (WITH_A) return fpart[e];
+  const float* p = reinterpret_cast<const float*>(fpart) + 3 * e;
+  return make_float4(p[0], p[1], p[2], 0.f);
+}
+
 template <bool WITH_A = true>
 __device__ __forceinline__ Partial sum_range(const float4* __restrict__ fpart,
                                              const float* __restrict__ apart, int i, int s0,
@@ -220,7 +229,7 @@ __device__ __forceinline__ Partial sum_range(const float4* __restrict__ fpart,
     float a[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-      f[k] = fpart[(size_t)(s + k) * Npad + i];
+      f[k] = load_entry<WITH_A>(fpart, (size_t)(s + k) * Npad + i);
       a[k] = WITH_A ? apart[(size_t)(s + k) * Npad + i] : 0.f;
     }
 #pragma unroll
@@ -242,7 +251,7 @@ __device__ __forceinline__ Partial sum_range(const float4* __restrict__ fpart,
 #pragma unroll
     for (int k = 0; k < B; ++k) {
       const int sk = min(s + k, s1 - 1);
-      f[k] = fpart[(size_t)sk * Npad + i];
+      f[k] = load_entry<WITH_A>(fpart, (size_t)sk * Npad + i);
       a[k] = WITH_A ? apart[(size_t)sk * Npad + i] : 0.f;
     }
 #pragma unroll
@@ -306,6 +315,17 @@ __device__ __forceinline__ void store_part(float4* p, float x, float y, float z,
 }
 
 __device__ __forceinline__ void store_part(float* p, float x) { __builtin_nontemporal_store(x, p); }
+
+// Force-only partial entries (the pair-once time step): three floats per
+// entry, entry e = s Npad + i at float 3 e of the slot buffer, a quarter fewer
+// bytes written by the force pass and read by the integrator than a float4
+__device__ __forceinline__ void store_part3(float4* base, size_t e, float x, float y, float z)
+{
+  float* p = reinterpret_cast<float*>(base) + 3 * e;
+  __builtin_nontemporal_store(x, p);
+  __builtin_nontemporal_store(y, p + 1);
+  __builtin_nontemporal_store(z, p + 2);
+}
 
 __device__ __forceinline__ void store_part(double4* p, double x, double y, double z, double w)
 {

@@ -416,12 +416,12 @@ __device__ __forceinline__ void n3_diag_pair(N3Lds<2>& lds, const uint4* __restr
   }
   __syncthreads();
   if (wave < 2 && grp >= 0) {
-    float4* o = fpart + (size_t)grp * Npad + base;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const float* t = tri + ((wave * 4 + k) * 3) * 64 + lane;
       if (base + 64 * k + lane < N)
-        store_part(o + 64 * k + lane, f[k][0] + t[0], f[k][1] + t[64], f[k][2] + t[128], 0.f);
+        store_part3(fpart, (size_t)grp * Npad + base + 64 * k + lane, f[k][0] + t[0],
+                    f[k][1] + t[64], f[k][2] + t[128]);
     }
   }
   // totals in the ordered-pair convention: every pair here was met once
@@ -445,7 +445,8 @@ __device__ __forceinline__ void n3_diag_pair(N3Lds<2>& lds, const uint4* __restr
 // W x W waves per workgroup, groups of G = 128 W particles.  Eight waves per
 // SIMD (at most 64 VGPRs; the loop body is unchanged by the limit, no spills).
 // ENERGY = false (the time step): no per-particle energies; the partial
-// entries carry the force only (w = 0, no apart), and each workgroup writes its
+// entries carry the force only (three floats, store_part3; no apart), and each
+// workgroup writes its
 // totals ew[2 item] = sum over ordered pairs of ((s/r)^12 - (s/r)^6) and
 // ew[2 item + 1] = sum of w = (s/r)^6 (2 (s/r)^6 - 1), the finaliser's E and W
 // terms.  The j side then carries three sums instead of four.
@@ -590,14 +591,22 @@ __device__ __forceinline__ void n3_item(N3Lds<W>& lds, const uint4* __restrict__
     }
     const int i = kG * g + T;
     if (i < N) {
-      store_part(fpart + (size_t)h * Npad + i, v[0], v[1], v[2], ENERGY ? v[3] : 0.f);
-      if constexpr (ENERGY) store_part(apart + (size_t)h * Npad + i, v[4]);
+      if constexpr (ENERGY) {
+        store_part(fpart + (size_t)h * Npad + i, v[0], v[1], v[2], v[3]);
+        store_part(apart + (size_t)h * Npad + i, v[4]);
+      } else {
+        store_part3(fpart, (size_t)h * Npad + i, v[0], v[1], v[2]);
+      }
     }
     if (!diag) {
       const int j = kG * h + T;
       if (j < N) {
-        store_part(fpart + (size_t)g * Npad + j, jf[0], jf[1], jf[2], 0.f);
-        if constexpr (ENERGY) store_part(apart + (size_t)g * Npad + j, jf[3]);
+        if constexpr (ENERGY) {
+          store_part(fpart + (size_t)g * Npad + j, jf[0], jf[1], jf[2], 0.f);
+          store_part(apart + (size_t)g * Npad + j, jf[3]);
+        } else {
+          store_part3(fpart, (size_t)g * Npad + j, jf[0], jf[1], jf[2]);
+        }
       }
     }
   }
