@@ -116,3 +116,43 @@ def test_pair_once_forced_small(N, bc, pair_once_env):
     dU = np.max(np.abs(runs["1"][2][:, 0] - runs["0"][2][:, 0])) / N
     dx = np.max(np.abs(xyz(runs["1"][3]) - xyz(runs["0"][3])))
     assert dU <= 1e-5 and dx <= 1e-4, (dU, dx)
+
+
+@pytest.fixture
+def waves_env():
+    old = {k: os.environ.get(k) for k in ("LJ_PAIR_ONCE", "LJ_PAIR_ONCE_WAVES")}
+    yield
+    for k, v in old.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+
+
+@pytest.mark.parametrize("N,bc", [(1300, ljamd.BC_PERIODIC), (1100, ljamd.BC_WALL)])
+def test_pair_once_sixteen_wave_groups(N, bc, waves_env):
+    """LJ_PAIR_ONCE_WAVES=4 (groups of 512, 16-wave workgroups, single diagonal
+    items also in the time step): forces against FP64 and a 20-step TVN run
+    against the default groups of 256 from the same state."""
+    rho = 0.6
+    L, pos, vel = perturbed_state(N, rho, 1.0, seed=47)
+    print(f"\n[case] pair-once 16-wave groups N={N} bc={bc}", flush=True)
+    os.environ["LJ_PAIR_ONCE"] = "1"
+    runs = {}
+    for waves in ("4", "2"):
+        os.environ["LJ_PAIR_ONCE_WAVES"] = waves
+        with ljamd.LJSystem(N, rho, 1.0, canonical=ljamd.TVN, boundary=bc) as s:
+            assert s.force_plan() == (True, (N + 128 * int(waves) - 1) // (128 * int(waves)))
+            s.set_state(pos, vel)
+            obs = s.CalculateForces()
+            f4 = s.force
+            h = s.Integrate(0.005, 20)
+            runs[waves] = (f4, obs, h, s.pos)
+    F64, _, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
+    for waves in ("4", "2"):
+        err = force_errors(xyz(runs[waves][0]), F64)
+        assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5, (waves, err)
+        assert rel(runs[waves][1]["V"], E) <= 1e-6 and rel(runs[waves][1]["W"], W) <= 1e-5
+    dU = np.max(np.abs(runs["4"][2][:, 0] - runs["2"][2][:, 0])) / N
+    dx = np.max(np.abs(xyz(runs["4"][3]) - xyz(runs["2"][3])))
+    assert dU <= 1e-5 and dx <= 1e-4, (dU, dx)
