@@ -576,6 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
     for (int c0 = 0; c0 < nj; c0 += 32) {
       const int cn = min(32, nj - c0);
       unsigned int mask = 0u;
+#pragma unroll 8
       for (int b = 0; b < cn; ++b) {
         const int jj = c0 + b;
         const uint4 qj = qj_s[jj];
@@ -598,6 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
       }
     }
   } else {
+#pragma unroll 4
     for (int jj = 0; jj < nj; ++jj) {
       if (!valid || (diag && jj <= (int)threadIdx.x)) continue;
       exact(jj);
