@@ -48,10 +48,14 @@ void drop_graph(lj_ctx* c)
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->gexec_run) (void)hipGraphExecDestroy(c->gexec_run);
   if (c->graph_run) (void)hipGraphDestroy(c->graph_run);
+  if (c->gexec_long) (void)hipGraphExecDestroy(c->gexec_long);
+  if (c->graph_long) (void)hipGraphDestroy(c->graph_long);
   c->gexec = nullptr;
   c->graph = nullptr;
   c->gexec_run = nullptr;
   c->graph_run = nullptr;
+  c->gexec_long = nullptr;
+  c->graph_long = nullptr;
 }
 
 // rows of one of the two block-row buffers
@@ -238,7 +242,8 @@ int ensure_graph(lj_ctx* c, double dt)
   drop_graph(c);
   LJ_HIP(hipStreamSynchronize(c->stream));
   LJ_TRY(capture_steps(c, dt, 1, &c->graph, &c->gexec));
-  const int rc = capture_steps(c, dt, kGraphSteps, &c->graph_run, &c->gexec_run);
+  int rc = capture_steps(c, dt, kGraphSteps, &c->graph_run, &c->gexec_run);
+  if (rc == LJ_OK) rc = capture_steps(c, dt, kLongGraphSteps, &c->graph_long, &c->gexec_long);
   if (rc != LJ_OK) {
     drop_graph(c);
     return rc;
@@ -252,6 +257,8 @@ int ensure_graph(lj_ctx* c, double dt)
 int launch_steps(lj_ctx* c, int nsteps)
 {
   int k = 0;
+  for (; k + kLongGraphSteps <= nsteps; k += kLongGraphSteps)
+    LJ_HIP(hipGraphLaunch(c->gexec_long, c->stream));
   for (; k + kGraphSteps <= nsteps; k += kGraphSteps) LJ_HIP(hipGraphLaunch(c->gexec_run, c->stream));
   for (; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(c->gexec, c->stream));
   return LJ_OK;

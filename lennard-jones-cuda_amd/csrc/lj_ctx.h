@@ -43,6 +43,8 @@ struct lj_ctx {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph_run = nullptr;    // kGraphSteps steps back to back
   hipGraphExec_t gexec_run = nullptr;
+  hipGraph_t graph_long = nullptr;   // kLongGraphSteps steps back to back
+  hipGraphExec_t gexec_long = nullptr;
   double g_dt = 0.;
   int g_canon = -1, g_bc = -1;
   hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -107,16 +107,17 @@ def test_time_reversibility():
     (10000, ljamd.TVN, ljamd.BC_PERIODIC), (10000, ljamd.EVN, ljamd.BC_PERIODIC),
     (10000, ljamd.TVN, ljamd.BC_WALL), (10000, ljamd.EVN, ljamd.BC_NONE)])
 def test_graph_replay_deterministic(N, canon, bc):
-    # 20 steps run as 8 + 8 + 4 x 1 (8-step graphs, where each step's kick
-    # kernel does the next drift and, with the pair-once pass, the next force
-    # pass finalises the step; then single steps) and as 7 x 1 + (8 + 5 x 1):
-    # the same arithmetic, the same bits, history rows included; N = 10000
-    # takes the pair-once force pass
+    # 41 steps run as 32 + 8 + 1 (graphs of 32 and 8 steps, where each step's
+    # kick kernel does the next drift and, with the pair-once pass, the next
+    # step's first integrator kernel finalises the step; then single steps), as
+    # (8 + 1) + 32 and as 7 x 1 + (8 + 5 x 1) + (8 + 8 + 5 x 1): the same
+    # arithmetic, the same bits, history rows included; N = 10000 takes the
+    # pair-once force pass
     rho, T0 = 0.85, 1.0
     L, pos, vel = lattice_state(N, rho, T0, seed=9)
     print(f"\n[case] graph replay N={N} canon={canon} bc={bc}", flush=True)
     outs = []
-    for chunks in ([20], [7, 13]):
+    for chunks in ([41], [9, 32], [7, 13, 21]):
         with ljamd.LJSystem(N, rho, T0, canonical=canon, boundary=bc) as s:
             assert s.force_plan()[0] == (N >= 6144)
             s.set_state(pos, vel)
@@ -125,9 +126,10 @@ def test_graph_replay_deterministic(N, canon, bc):
             obs = s.observables()
             outs.append((s.pos, s.vel, np.concatenate(hs), s.force,
                          [obs[k] for k in ("U", "T", "K", "V", "P", "t", "step", "av_iters")]))
-    for a, b in zip(outs[0], outs[1]):
-        assert np.array_equal(a, b)
-    assert outs[0][4][6] == 20
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
+    assert outs[0][4][6] == 41
 
 
 def test_renormalize_and_modes():
