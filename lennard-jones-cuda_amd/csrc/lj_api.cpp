@@ -234,6 +234,15 @@ int capture_steps(lj_ctx* c, double dt, int steps, hipGraph_t* graph, hipGraphEx
 }
 }  // namespace
 
+// LJ_LONG_GRAPH=0 keeps runs to the 8-step and single-step graphs: under
+// rocprofv3's kernel trace the 32-step replays left the device idle about 80 us
+// per step (N = 16384), so step traces are taken without them
+static bool long_graph_enabled()
+{
+  const char* e = std::getenv("LJ_LONG_GRAPH");
+  return !(e && e[0] == '0');
+}
+
 // the step captured once per (dt, ensemble, boundary) and replayed
 int ensure_graph(lj_ctx* c, double dt)
 {
@@ -243,7 +252,8 @@ int ensure_graph(lj_ctx* c, double dt)
   LJ_HIP(hipStreamSynchronize(c->stream));
   LJ_TRY(capture_steps(c, dt, 1, &c->graph, &c->gexec));
   int rc = capture_steps(c, dt, kGraphSteps, &c->graph_run, &c->gexec_run);
-  if (rc == LJ_OK) rc = capture_steps(c, dt, kLongGraphSteps, &c->graph_long, &c->gexec_long);
+  if (rc == LJ_OK && long_graph_enabled())
+    rc = capture_steps(c, dt, kLongGraphSteps, &c->graph_long, &c->gexec_long);
   if (rc != LJ_OK) {
     drop_graph(c);
     return rc;
@@ -257,8 +267,9 @@ int ensure_graph(lj_ctx* c, double dt)
 int launch_steps(lj_ctx* c, int nsteps)
 {
   int k = 0;
-  for (; k + kLongGraphSteps <= nsteps; k += kLongGraphSteps)
-    LJ_HIP(hipGraphLaunch(c->gexec_long, c->stream));
+  if (c->gexec_long)
+    for (; k + kLongGraphSteps <= nsteps; k += kLongGraphSteps)
+      LJ_HIP(hipGraphLaunch(c->gexec_long, c->stream));
   for (; k + kGraphSteps <= nsteps; k += kGraphSteps) LJ_HIP(hipGraphLaunch(c->gexec_run, c->stream));
   for (; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(c->gexec, c->stream));
   return LJ_OK;
