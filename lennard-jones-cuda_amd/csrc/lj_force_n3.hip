@@ -19,12 +19,15 @@
 //   * Diagonal items (g == h): waves with wi < wj evaluate their tile pair
 //     once as above, waves with wi == wj evaluate both orders of their own
 //     tile (i sums only, self pair excluded by index), waves with wi > wj
-//     are idle.
+//     are idle.  The time step's variant (W = 2) instead takes the diagonal
+//     items two to a workgroup with every pair once (n3_diag_pair), so that
+//     every workgroup carries the same work.
 // Output slots.  Particle p of group a receives exactly one entry per group s:
 // s > a from the i side of item (a, s), s < a from the j side of item (s, a),
 // s == a from the diagonal item.  The partial buffers are therefore the
 // [S = groups][Npad] layout of the sliced kernel and the reduction and
-// integrator kernels are shared.  Entry encoding (fx, fy, fz, bf) + af with
+// integrator kernels are shared (the time step's entries are the three force
+// components only, store_part3).  Entry encoding (fx, fy, fz, bf) + af with
 // e = (bf - af)/2 and W = 12 eps sum bf, as for the sliced kernel:
 //   i side of a once-evaluated pair: bf = 2 b, af = b + a;
 //   j side:                          bf = 0,   af = sum ((s/r)^6 - w);
