@@ -173,16 +173,18 @@ __global__ __launch_bounds__(kBlock) void k_batch_kick_evn(BatchArgs a)
   double v4[4] = {0., 0., 0., 0.};
   if (pp.valid) {
     const int i = pp.gi;
+    // the particle's own loads go out ahead of the slot sums (their latencies
+    // overlap; the force store below would otherwise keep them after it)
+    float4 v = a.vel[i];
+    float4 p = a.pos[i];
     const Partial s = box_slices(a, bp, pp.i);
     const double e = 0.5 * (s.b - s.a);
     const float fx = (float)(bp.FS * s.fx), fy = (float)(bp.FS * s.fy), fz = (float)(bp.FS * s.fz);
     a.force[i] = make_float4(fx, fy, fz, (float)e);
-    float4 v = a.vel[i];
     const double dt = a.dt;
     v.x = (float)(v.x + dt * fx / 2.);  // MDSystem.cpp:460-462
     v.y = (float)(v.y + dt * fy / 2.);
     v.z = (float)(v.z + dt * fz / 2.);
-    float4 p = a.pos[i];
     if (a.bc != LJ_BC_NONE) {
       apply_bc_one(p, v, a.bc, bp.L);
       if (!DRIFT && a.bc == LJ_BC_PERIODIC) a.pos[i] = p;
@@ -206,16 +208,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_tvn_a(BatchArgs a)
   double v3[3] = {0., 0., 0.};
   if (pp.valid) {
     const int i = pp.gi;
+    // the particle's own loads go out ahead of the slot sums
+    float4 tf = a.tF[i];
+    const float4 v = a.vel[i];
     const Partial s = box_slices(a, bp, pp.i);
     const double e = 0.5 * (s.b - s.a);
     const float fx = (float)(bp.FS * s.fx), fy = (float)(bp.FS * s.fy), fz = (float)(bp.FS * s.fz);
     a.force[i] = make_float4(fx, fy, fz, (float)e);
-    float4 tf = a.tF[i];
     tf.x += 0.5f * fx;  // MDSystem.cpp:486-488
     tf.y += 0.5f * fy;
     tf.z += 0.5f * fz;
     a.tF[i] = tf;
-    const float4 v = a.vel[i];
     const double dt = a.dt;
     float4 tv;
     tv.x = (float)(v.x + dt * tf.x / 2.);  // MDSystem.cpp:493-495
