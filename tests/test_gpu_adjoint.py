@@ -20,7 +20,7 @@ from oracle_lib import adjoint_pass_fp64, lattice_state, traj_grad_fp64, traj_lo
 
 pytestmark = pytest.mark.gpu
 
-N4, RHO4, T4, DT4, M4, K4 = 2048, 0.85, 1.0, 0.005, 200, 20
+N_LIQ, RHO_LIQ, T_LIQ, DT4, M4, K4 = 2048, 0.85, 1.0, 0.005, 200, 20
 
 
 @pytest.mark.parametrize("periodic", [True, False])
@@ -42,14 +42,14 @@ def test_adjoint_pass_vs_fp64(periodic):
         assert abs(sc[k] - sc0[k]) <= 1e-5 * max(abs(sc0[k]), 1e-3 * np.abs(sc0[:6]).max()), k
 
 
-def _config4_state(seed=4):
-    return lattice_state(N4, RHO4, T4, seed)
+def _liquid_n2048_state(seed=4):
+    return lattice_state(N_LIQ, RHO_LIQ, T_LIQ, seed)
 
 
-def test_trajectory_gradient_config4():
-    L, pos, vel = _config4_state()
-    print("\n[case] adjoint config N=2048 M=200 k=20", flush=True)
-    with ljamd.LJSystem(N4, RHO4, T4) as s:
+def test_trajectory_gradient_liquid_n2048():
+    L, pos, vel = _liquid_n2048_state()
+    print("\n[case] adjoint liquid N=2048 M=200 k=20", flush=True)
+    with ljamd.LJSystem(N_LIQ, RHO_LIQ, T_LIQ) as s:
         s.set_state(pos, vel)
         loss_g = s.trajectory_loss(DT4, M4)
         out, g2, dx0, times = s.trajectory_grad(DT4, M4, K4)
@@ -105,9 +105,9 @@ def test_trajectory_gradient_config4():
 def test_trajectory_gradient_open_box_fd():
     """Same workload in an open box (smooth loss): the GPU gradient against
     central differences of the FP64 loss for eps, sigma and x0."""
-    L, pos, vel = _config4_state()
+    L, pos, vel = _liquid_n2048_state()
     print("\n[case] adjoint open box N=2048 M=200 k=20", flush=True)
-    with ljamd.LJSystem(N4, RHO4, T4, boundary=ljamd.BC_NONE) as s:
+    with ljamd.LJSystem(N_LIQ, RHO_LIQ, T_LIQ, boundary=ljamd.BC_NONE) as s:
         s.set_state(pos, vel)
         out, g2, dx0, _ = s.trajectory_grad(DT4, M4, K4)
     x, v = xyz(pos), xyz(vel)

@@ -5,8 +5,8 @@ Checks, on identical seeded states:
   * against the FP64 oracle: per-particle forces, E_pot, virial W;
   * against the reference-faithful CPU path (oracle restatement, bit-exact to
     the compiled reference): forces, V and the virial pressure term;
-at N = 400 (config 1), 2048, 16384 (config 2) with periodic and hard-wall
-boxes, plus the exact config lattice states.
+at N = 400 (rho* = 0.05), 2048 and 16384 (rho* = 0.85) with periodic and hard-wall
+boxes, plus the exact lattice states of these workloads.
 
 Tolerances (FP32 pair arithmetic, FP64 cross-slice sums):
   forces vs FP64:  global ||dF||/||F|| <= 2e-6,
@@ -81,7 +81,7 @@ def test_forces_vs_reference_cpu(N, rho, T0, bc):
 
 @pytest.mark.parametrize("N,rho", [(400, 0.05), (16384, 0.85), (2048, 0.85)])
 def test_config_lattice_state(N, rho):
-    """The configs' own initial state (lattice, no displacement)."""
+    """The workloads' own initial state (lattice, no displacement)."""
     L, pos, vel = lattice_state(N, rho, 1.0, seed=1)
     f4, obs = _gpu_forces(N, rho, ljamd.BC_PERIODIC, pos)
     F64, _, E, W = forces_fp64(pos, L, True)

@@ -1,6 +1,6 @@
 """RDF histogram parity: bit-exact bin counts against the oracle on identical
 positions, for the three binning rules (MDSystem.cpp:273-285 CPU rule,
-MDSystem.cu:26-42 GPU rule, uniform-r bins of the configs)."""
+MDSystem.cu:26-42 GPU rule, uniform-r bins of the workloads)."""
 import numpy as np
 import pytest
 
@@ -17,11 +17,11 @@ RULES = {RDF_REF_CPU: ljamd.RDF_REF_CPU, RDF_REF_GPU: ljamd.RDF_REF_GPU,
 @pytest.mark.parametrize("N,rho,bc,rule,nbins", [
     (400, 0.05, ljamd.BC_PERIODIC, RDF_REF_CPU, 256),
     (400, 0.05, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),
-    (400, 0.05, ljamd.BC_PERIODIC, RDF_RBINS, 200),   # config 1: 200 bins to L/2
+    (400, 0.05, ljamd.BC_PERIODIC, RDF_RBINS, 200),   # the N = 400 workload: 200 bins to L/2
     (2048, 0.85, ljamd.BC_WALL, RDF_REF_CPU, 256),
     (2048, 0.85, ljamd.BC_WALL, RDF_RBINS, 300),
     (3001, 0.3, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),
-    (16384, 0.85, ljamd.BC_PERIODIC, RDF_RBINS, 500),  # config 2: 500 bins
+    (16384, 0.85, ljamd.BC_PERIODIC, RDF_RBINS, 500),  # the N = 16384 workload: 500 bins
     (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 256),
 ])
 def test_rdf_bitexact(N, rho, bc, rule, nbins):

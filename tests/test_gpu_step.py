@@ -41,9 +41,9 @@ def test_initial_state_matches_oracle():
 
 
 @pytest.mark.parametrize("N,rho,T0,canon,bc,nsteps,tolU", [
-    (400, 0.05, 1.4, ljamd.EVN, ljamd.BC_PERIODIC, 400, 2e-5),   # config 1 slice
-    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_PERIODIC, 60, 2e-4),   # config 2 (TVN)
-    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_WALL, 60, 2e-4),       # config 2 (hard wall)
+    (400, 0.05, 1.4, ljamd.EVN, ljamd.BC_PERIODIC, 400, 2e-5),   # the N = 400 workload, a slice
+    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_PERIODIC, 60, 2e-4),   # the liquid at N = 2048 (TVN)
+    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_WALL, 60, 2e-4),       # the liquid at N = 2048 (hard wall)
     (1000, 0.3, 1.0, ljamd.EVN, ljamd.BC_WALL, 100, 1e-4),
     (600, 0.2, 1.0, ljamd.EVN, ljamd.BC_NONE, 60, 1e-4),
     # the pair-once pass with a partial last group of 256 (16 and 201 particles)

@@ -196,7 +196,7 @@ def test_periodic_lattice_force_free_and_lattice_sum():
     x = np.stack([(k % n + 0.5) * a, ((k // n) % n + 0.5) * a, (k // (n * n) + 0.5) * a], 1)
     F, _, E, _ = forces_fp64d(x, L, 1)
     assert np.abs(F).max() < 1e-9
-    # the float-rounded lattice of the configs is force-free up to rounding
+    # the float-rounded lattice of the workloads is force-free up to rounding
     Ff, _, Ef, _ = forces_fp64(pos, Lc, 1)
     assert np.abs(Ff).max() < 1e-4 and abs(Ef - E) < 1e-6 * abs(E)
     # lattice sum: each particle sees offsets k in [-(n-1)/2, (n-1)/2]^3 \ 0 (minimum image)
