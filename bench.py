@@ -246,7 +246,7 @@ def profiled_sclk_ghz(kernel):
         return None
 
 
-def profile_force(kernel, reps=20, warm=600, copy_mib=256):
+def profile_force(kernel, reps=20, warm=600, copy_mib=256, boundary="periodic"):
     """Launch pattern for counter collection: a device-to-device copy of a
     known size (calibrates FETCH_SIZE / WRITE_SIZE per byte moved), then the
     force kernel alone `warm` times (the clocks settle over the first ~0.1 s
@@ -267,11 +267,13 @@ def profile_force(kernel, reps=20, warm=600, copy_mib=256):
     hip.hipFree(src)
     hip.hipFree(dst)
     jsl = 32 if kernel == "sliced" else 0
-    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED, jslices=jsl) as s:
+    bc = ljamd.BC_WALL if boundary == "wall" else ljamd.BC_PERIODIC
+    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, boundary=bc, seed=SEED, jslices=jsl) as s:
         s.time_force_kernel(warm)
         us = s.time_force_kernel(reps)
         pair_once, nslots = s.force_plan()
-    print(json.dumps({"profile": "force", "N": N, "warm": warm, "reps": reps, "kernel_us": us,
+    print(json.dumps({"profile": "force", "N": N, "boundary": boundary, "warm": warm, "reps": reps,
+                      "kernel_us": us,
                       "pair_once": pair_once, "nslots": nslots, "copy_bytes": nbytes}))
 
 
@@ -289,11 +291,13 @@ def main():
     ap.add_argument("--profile-force", nargs="?", const="pair-once", default=None,
                     choices=["pair-once", "sliced"],
                     help="only a calibration copy and the force kernel (for rocprofv3)")
+    ap.add_argument("--profile-boundary", default="periodic", choices=["periodic", "wall"],
+                    help="boundary of the --profile-force workload")
     ap.add_argument("--profile-rdf", action="store_true",
                     help="only the RDF passes of the rdf leg (for rocprofv3)")
     a = ap.parse_args()
     if a.profile_force:
-        profile_force(a.profile_force)
+        profile_force(a.profile_force, boundary=a.profile_boundary)
         return
     if a.profile_rdf:
         print(json.dumps({"profile": "rdf", "N": N, **rdf_leg()}))
