@@ -69,9 +69,7 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 // ties: resolve exact half-box separations as the reference does (single
 // evaluations of arbitrary states); the step graph uses ties = false
 // step_ew (pair-once plans only, may be null): the time step's variant, which
-// writes force-only partials and per-workgroup (E, W) totals to step_ew;
-// fin (with step_ew only, may be null): one more workgroup computes the
-// observables of the step described by *fin (the previous one) meanwhile
+// writes force-only partials and per-workgroup (E, W) totals to step_ew
 struct StepArgs;
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
