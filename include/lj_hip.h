@@ -229,7 +229,7 @@ int lj_device_arrays(lj_ctx* ctx, float** pos, float** vel, float** force);
  * context stream, measured with HIP events (for roofline reporting). */
 int lj_time_force_kernel(lj_ctx* ctx, int reps, double* us_per_launch);
 /* which force kernel the context runs: pair_once = 1 for the kernel that
- * evaluates each unordered pair once (FP32 mode, N >= 6144, j slices
+ * evaluates each unordered pair once (FP32 mode, N >= 3072, j slices
  * automatic), 0 for the sliced kernel; nslots = partial sums per particle */
 int lj_force_plan_info(lj_ctx* ctx, int* pair_once, int* nslots);
 

@@ -20,7 +20,7 @@
  *     return their curves as two vectors instead of a SplineFunction;
  *   - there is no CPU branch: setHardwareMode(false) throws.  The reference's
  *     own class linked against libljhip.so keeps its CPU/GPU switch intact;
- *   - at N >= 6144 (FP32) h_Force[4 i + 3] is 0 after Integrate (the step sums
+ *   - at N >= 3072 (FP32) h_Force[4 i + 3] is 0 after Integrate (the step sums
  *     the potential energy per workgroup; V, P are unaffected) and is
  *     refreshed by CalculateForces();
  *   - every failure throws std::runtime_error with lj_last_error().

@@ -90,8 +90,8 @@ StepArgs step_args(lj_ctx* c, double dt, int b = 0)
   a.canonical = c->cfg.canonical;
   a.S = fplan(c).S;
   a.Npad = fplan(c).Npad;
-  a.ew = c->step_ew ? c->step_ew + (size_t)b * 2 * c->plan3.nitems_step : nullptr;
-  a.nitems = c->step_ew ? c->plan3.nitems_step : 0;
+  a.ew = c->step_ew ? c->step_ew + (size_t)b * 2 * c->plan3.step_wgs() : nullptr;
+  a.nitems = c->step_ew ? c->plan3.step_wgs() : 0;
   a.nrow = is_fp64(c) ? c->nblk : step_rows(c->cfg.N);
   return a;
 }
@@ -419,7 +419,7 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
       A(fail(LJ_ERR_DEVICE, "item table upload failed"));
     c->plan3.items = c->n3items;
     c->plan3.items_step = c->n3items + c->plan3.nitems;
-    A(dev_alloc(&c->step_ew, (size_t)4 * c->plan3.nitems_step));  // two buffers
+    A(dev_alloc(&c->step_ew, (size_t)4 * c->plan3.step_wgs()));  // two buffers
   }
   // rows: nblk for the per-block kernels, step_rows(N) >= nblk for the step
   A(dev_alloc(&c->blk, 2 * blk_rows(c) * kBlkFields));

@@ -22,6 +22,10 @@
 //     are idle.  The time step's variant (W = 2) instead takes the diagonal
 //     items two to a workgroup with every pair once (n3_diag_pair), so that
 //     every workgroup carries the same work.
+//   * Split items (groups of 256, sizes whose items fill few of the resident
+//     workgroups, n3_split): two workgroups per item, each taking one of the
+//     two 64-particle j sub-tiles of every wave, with their own slots (S = 2
+//     groups); the diagonal items stay single.
 // Output slots.  Particle p of group a receives exactly one entry per group s:
 // s > a from the i side of item (a, s), s < a from the j side of item (s, a),
 // s == a from the diagonal item.  The partial buffers are therefore the
@@ -101,15 +105,17 @@ __device__ __forceinline__ float rot1(float v)
 // at step t and passes them to lane l - 1 afterwards), so after 64 steps lane l
 // holds those of slot l.  J receives (-fx, -fy, -fz, sum (s/r)^6 - w) per
 // sub-tile.
-// nsub: sub-tiles holding a valid j (1 when the second one is past N)
+// nsub: sub-tiles holding a valid j (1 when the second one is past N); st0:
+// the first sub-tile taken (1 for the second half of a split item)
 template <bool PERIODIC, bool TIES, bool PARTIAL, bool ENERGY>
 __device__ __forceinline__ void pair_tile(const uint4* P0, int lane, const PosT<PERIODIC>& pi1,
                                           const PosT<PERIODIC>& pi2, bool v1, bool v2, float C1,
-                                          int nsub, Acc& a1, Acc& a2, float (&J)[2][4])
+                                          int nsub, Acc& a1, Acc& a2, float (&J)[2][4], int st0 = 0)
 {
 #pragma unroll
   for (int st = 0; st < 2; ++st) {
     if (st >= nsub) break;  // wave-uniform
+    if (st < st0) continue;
     const uint4* P = P0 + st * 128;
     float jx = 0.f, jy = 0.f, jz = 0.f, jm = 0.f;
 #pragma unroll 8
@@ -156,11 +162,13 @@ __device__ __forceinline__ void pair_tile(const uint4* P0, int lane, const PosT<
 template <bool PERIODIC, bool TIES>
 __device__ __forceinline__ void self_tile(const uint4* P0, int lane, int i1, int i2, int jbase,
                                           const PosT<PERIODIC>& pi1, const PosT<PERIODIC>& pi2,
-                                          bool v1, bool v2, float C1, int nsub, Acc& a1, Acc& a2)
+                                          bool v1, bool v2, float C1, int nsub, Acc& a1, Acc& a2,
+                                          int st0 = 0)
 {
 #pragma unroll
   for (int st = 0; st < 2; ++st) {
     if (st >= nsub) break;  // wave-uniform
+    if (st < st0) continue;
     const uint4* P = P0 + st * 128;
 #pragma unroll 8
     for (int t = 0; t < 64; ++t) {
@@ -455,12 +463,18 @@ __device__ __forceinline__ void n3_diag_pair(N3Lds<2>& lds, const uint4* __restr
 // terms.  The j side then carries three sums instead of four.
 // One group pair (g, h) of one box: q/pf/fpart/apart point at the box's
 // arrays, ew (ENERGY = false) at the item's two totals.
-template <bool PERIODIC, bool TIES, int W, bool ENERGY>
+// SPLIT: the item is run by two workgroups, half = 0 / 1 taking the first /
+// second 64-particle sub-tile of every wave's 128 j (half the pairs each), so
+// that sizes whose items fill less than one round of resident workgroups run
+// twice the waves.  Each half writes its own slots, 2 h + half for the i side
+// and 2 g + half for the j side (zeros for the j it did not meet): every
+// particle still receives exactly one entry per slot, S = 2 groups.
+template <bool PERIODIC, bool TIES, int W, bool ENERGY, bool SPLIT = false>
 __device__ __forceinline__ void n3_item(N3Lds<W>& lds, const uint4* __restrict__ q,
                                         const float4* __restrict__ pf,
                                         float4* __restrict__ fpart, float* __restrict__ apart,
                                         double* __restrict__ ew, int N, int Npad, float C1, int g,
-                                        int h)
+                                        int h, int half = 0)
 {
   constexpr int kG = kTile * W;
   uint4* const s_pos = lds.pos;
@@ -493,7 +507,8 @@ __device__ __forceinline__ void n3_item(N3Lds<W>& lds, const uint4* __restrict__
   // second sub-tile past N is skipped: wave-uniform tests
   const int jt0 = kG * h + kTile * wj, it0 = kG * g + kTile * wi;
   const bool live = jt0 < N && it0 < N;
-  const int nsub = jt0 + 64 < N ? 2 : 1;
+  const int nsub = SPLIT ? min(jt0 + 64 < N ? 2 : 1, half + 1) : (jt0 + 64 < N ? 2 : 1);
+  const int st0 = SPLIT ? half : 0;
   // pair tile / self tile / idle
   const int cls = !live ? 0 : (diag ? (wi < wj ? 1 : (wi == wj ? 2 : 0)) : 1);
   const int i1 = kG * g + kTile * wi + lane, i2 = i1 + 64;
@@ -507,12 +522,13 @@ __device__ __forceinline__ void n3_item(N3Lds<W>& lds, const uint4* __restrict__
   float J[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
   if (cls == 1) {
     if (partial)
-      pair_tile<PERIODIC, TIES, true, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, nsub, a1, a2, J);
+      pair_tile<PERIODIC, TIES, true, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, nsub, a1, a2, J, st0);
     else
-      pair_tile<PERIODIC, TIES, false, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, 2, a1, a2, J);
+      pair_tile<PERIODIC, TIES, false, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, SPLIT ? nsub : 2, a1,
+                                               a2, J, st0);
   } else if (cls == 2) {
     self_tile<PERIODIC, TIES>(P0, lane, i1, i2, kG * h + kTile * wj, pi1, pi2, v1, v2, C1, nsub, a1,
-                              a2);
+                              a2, st0);
   }
   {
     float* A = s_acc + wave * kAccPerWave;
@@ -593,45 +609,51 @@ __device__ __forceinline__ void n3_item(N3Lds<W>& lds, const uint4* __restrict__
       v[4] += jf[3];
     }
     const int i = kG * g + T;
+    const size_t si = SPLIT ? 2 * h + half : h, sj = SPLIT ? 2 * g + half : g;
     if (i < N) {
       if constexpr (ENERGY) {
-        store_part(fpart + (size_t)h * Npad + i, v[0], v[1], v[2], v[3]);
-        store_part(apart + (size_t)h * Npad + i, v[4]);
+        store_part(fpart + si * Npad + i, v[0], v[1], v[2], v[3]);
+        store_part(apart + si * Npad + i, v[4]);
       } else {
-        store_part3(fpart, (size_t)h * Npad + i, v[0], v[1], v[2]);
+        store_part3(fpart, si * Npad + i, v[0], v[1], v[2]);
       }
     }
     if (!diag) {
       const int j = kG * h + T;
       if (j < N) {
         if constexpr (ENERGY) {
-          store_part(fpart + (size_t)g * Npad + j, jf[0], jf[1], jf[2], 0.f);
-          store_part(apart + (size_t)g * Npad + j, jf[3]);
+          store_part(fpart + sj * Npad + j, jf[0], jf[1], jf[2], 0.f);
+          store_part(apart + sj * Npad + j, jf[3]);
         } else {
-          store_part3(fpart, (size_t)g * Npad + j, jf[0], jf[1], jf[2]);
+          store_part3(fpart, sj * Npad + j, jf[0], jf[1], jf[2]);
         }
       }
     }
   }
 }
 
-template <bool PERIODIC, bool TIES, int W, bool ENERGY>
+// SPLIT: two workgroups per item (blockIdx.x = 2 item + half), no paired
+// diagonal items; the (E, W) totals of the step variant per workgroup
+template <bool PERIODIC, bool TIES, int W, bool ENERGY, bool SPLIT>
 __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8))) void k_force_n3(
     const uint4* __restrict__ q, const float4* __restrict__ pf, const int2* __restrict__ items,
     float4* __restrict__ fpart, float* __restrict__ apart, double* __restrict__ ew, int N, int Npad,
     float C1)
 {
+  static_assert(!SPLIT || W == 2, "split items: groups of 256");
   __shared__ N3Lds<W> lds;
-  const int k = blockIdx.x;  // item
+  const int wg = blockIdx.x;
+  const int k = SPLIT ? wg >> 1 : wg;  // item
   const int2 it = items[k];
-  if constexpr (!ENERGY && W == 2) {
+  if constexpr (!ENERGY && W == 2 && !SPLIT) {
     if (it.y < 0) {  // diagonal pair (it.x, -2 - it.y), the second one absent when -1
       n3_diag_pair<PERIODIC>(lds, q, pf, fpart, ew + 2 * (size_t)k, N, Npad, C1, it.x, -2 - it.y);
       return;
     }
   }
-  n3_item<PERIODIC, TIES, W, ENERGY>(lds, q, pf, fpart, apart, ENERGY ? nullptr : ew + 2 * (size_t)k,
-                                     N, Npad, C1, it.x, it.y);
+  n3_item<PERIODIC, TIES, W, ENERGY, SPLIT>(lds, q, pf, fpart, apart,
+                                            ENERGY ? nullptr : ew + 2 * (size_t)wg, N, Npad, C1,
+                                            it.x, it.y, SPLIT ? wg & 1 : 0);
 }
 
 // batched ensemble: item = (box, g, h) over all boxes; each box's partials at
@@ -664,6 +686,39 @@ int n3_waves(int /*N*/)
   return 2;
 }
 
+// Relative pair rate of one CU holding w four-wave workgroups of the pair
+// loop (one wave per SIMD each), measured on the time step's pass at
+// rho* = 0.85: 1 : 2.0 : 2.15 : 2.6 for w = 1, 2, 3 and >= 4 (one wave per
+// SIMD is bound by its own latency; from four on the VALU is nearly full).
+double n3_cu_rate(int w) { return w <= 1 ? 1.0 : w == 2 ? 2.0 : w == 3 ? 2.15 : 2.6; }
+
+// modelled duration of a pass of n workgroups, each `work` items of pairs,
+// spread evenly over `cus` CUs: the busiest CU holds ceil(n / cus) of them
+double n3_pass_time(long n, double work, int cus)
+{
+  const int w = (int)((n + cus - 1) / cus);
+  return w * work / n3_cu_rate(w);
+}
+
+int n3_split(int W, int ng)
+{
+  // LJ_PAIR_ONCE_SPLIT=0 / 1 turns the split items off / on (groups of 256)
+  if (W != 2) return 1;
+  const char* e = std::getenv("LJ_PAIR_ONCE_SPLIT");
+  if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1' ? 2 : 1;
+  int cus = 256, dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  // split when the model gives the split pass at least 10% less time (its
+  // twice as many partial slots cost the integrator about that much).
+  // Measured step times (TVN, N: split / unsplit us): 4096 21.3 / 32.6, 6144
+  // 26.4 / 33.7, 8192 34.6 / 34.5, 10000 44.6 / 47.2, 11264 48.9 / 48.4.
+  const long full = (long)ng * (ng + 1) / 2, paired = (long)ng * (ng - 1) / 2 + (ng + 1) / 2;
+  return n3_pass_time(2 * full, 0.5, cus) < 0.9 * n3_pass_time(paired, 1.0, cus) ? 2 : 1;
+}
+
 ForcePlan make_n3_plan(int N)
 {
   ForcePlan pl;
@@ -672,12 +727,14 @@ ForcePlan make_n3_plan(int N)
   pl.N = N;
   pl.n3 = W;
   pl.ng = (N + G - 1) / G;
-  pl.S = pl.ng;
+  pl.split = n3_split(W, pl.ng);
+  pl.S = pl.ng * pl.split;
   pl.jchunk = G;
   pl.Npad = (N + 63) & ~63;
   pl.nitems = pl.ng * (pl.ng + 1) / 2;
-  // the time step pairs its diagonal items (n3_step_items)
-  pl.nitems_step = W == 2 ? pl.ng * (pl.ng - 1) / 2 + (pl.ng + 1) / 2 : pl.nitems;
+  // the time step pairs its diagonal items (n3_step_items), unless split
+  pl.nitems_step =
+      W == 2 && pl.split == 1 ? pl.ng * (pl.ng - 1) / 2 + (pl.ng + 1) / 2 : pl.nitems;
   return pl;
 }
 
@@ -686,10 +743,13 @@ bool n3_eligible(int N)
   // LJ_PAIR_ONCE=0 / 1 turns the kernel off / on at any size (tuning)
   const char* e = std::getenv("LJ_PAIR_ONCE");
   if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1' && N >= 256;
-  // measured against the sliced kernel (time step, TVN, rho* = 0.85): N = 6144
-  // 33.6 vs 37.3 us per step (force pass 22.4 vs 22.9 us), N = 4096 28.7 vs
-  // 26.1 us (few group pairs leave most CUs idle: 18.6 vs 12.0 us)
-  return N >= 6144;
+  // measured against the sliced kernel (TVN step at rho* = 0.85, us per step,
+  // with split items): N = 2560 19.2 vs 21.4, 3072 19.3 vs 22.6, 4096 21.2 vs
+  // 25.1, 6144 26.4 vs 36.8.  Below N = 3584 the pass itself is slower (one
+  // wave per SIMD: 10.9 against 9.4 us at 3072), the step faster for its
+  // fused observables; the threshold keeps single evaluations of small boxes
+  // on the sliced kernel
+  return N >= 3072;
 }
 
 void n3_items(int ng, int2* out)
@@ -720,24 +780,24 @@ void n3_step_items(const ForcePlan& pl, int2* out)
 
 namespace {
 
-template <int W, bool ENERGY>
+template <int W, bool ENERGY, bool SPLIT>
 hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
                        const float4* posf, float4* fpart, float* apart, double* ew, bool ties)
 {
   // the time step's variant runs its own item list (diagonal items paired)
   const int nitems = ENERGY ? pl.nitems : pl.nitems_step;
   const int2* items = ENERGY ? pl.items : pl.items_step;
-  dim3 grid(nitems), block(64 * W * W);
+  dim3 grid(nitems * (SPLIT ? 2 : 1)), block(64 * W * W);
   if (ps.periodic) {
     if (ties)
-      hipLaunchKernelGGL((k_force_n3<true, true, W, ENERGY>), grid, block, 0, st, q, posf, items,
-                         fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+      hipLaunchKernelGGL((k_force_n3<true, true, W, ENERGY, SPLIT>), grid, block, 0, st, q, posf,
+                         items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
     else
-      hipLaunchKernelGGL((k_force_n3<true, false, W, ENERGY>), grid, block, 0, st, q, posf, items,
-                         fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+      hipLaunchKernelGGL((k_force_n3<true, false, W, ENERGY, SPLIT>), grid, block, 0, st, q, posf,
+                         items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
   } else {
-    hipLaunchKernelGGL((k_force_n3<false, false, W, ENERGY>), grid, block, 0, st, q, posf, items,
-                       fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+    hipLaunchKernelGGL((k_force_n3<false, false, W, ENERGY, SPLIT>), grid, block, 0, st, q, posf,
+                       items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
   }
   return hipGetLastError();
 }
@@ -776,12 +836,16 @@ hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalar
                            bool ties, double* ew)
 {
   // ew != null: the time step's variant (force partials + per-workgroup totals)
+  if (pl.split == 2 && pl.n3 != 2) return hipErrorInvalidValue;
   if (ew) {
-    if (pl.n3 == 2) return launch_n3_w<2, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
-    return launch_n3_w<4, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
+    if (pl.split == 2) return launch_n3_w<2, false, true>(st, pl, ps, q, posf, fpart, apart, ew, false);
+    if (pl.n3 == 2) return launch_n3_w<2, false, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
+    return launch_n3_w<4, false, false>(st, pl, ps, q, posf, fpart, apart, ew, false);
   }
-  if (pl.n3 == 2) return launch_n3_w<2, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
-  return launch_n3_w<4, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
+  if (pl.split == 2)
+    return launch_n3_w<2, true, true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
+  if (pl.n3 == 2) return launch_n3_w<2, true, false>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
+  return launch_n3_w<4, true, false>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
 }
 
 }  // namespace lj

@@ -38,14 +38,20 @@ struct ForcePlan {
   int ng = 0;
   int nitems = 0;
   const int2* items = nullptr;  // device table of (g, h) group pairs
-  // the time step's variant: nitems_step workgroups (the diagonal items in
-  // pairs with W = 2)
+  // the time step's variant: nitems_step items (the diagonal items in pairs
+  // with W = 2 unless split)
   int nitems_step = 0;
   const int2* items_step = nullptr;
+  // workgroups per item: 2 splits every item's j sub-tiles between two
+  // workgroups (sizes whose items fill under half the resident workgroups);
+  // S = split x ng slots
+  int split = 1;
+  // workgroups of the time step's pass, one (E, W) pair of totals each
+  int step_wgs() const { return nitems_step * split; }
 };
 ForcePlan make_force_plan(int N, int jslices_override, int ipl);
 ForcePlan make_n3_plan(int N);
-// sizes at which the pair-once kernel beats the sliced one (N >= 6144)
+// sizes at which the pair-once kernel serves the time step (N >= 3072)
 bool n3_eligible(int N);
 // the ng (ng + 1) / 2 group pairs of the pair-once kernel, in launch order
 void n3_items(int ng, int2* out);

@@ -164,7 +164,7 @@ def test_newton_third_law():
 @pytest.mark.parametrize("N,bc", [(12800, ljamd.BC_PERIODIC), (13001, ljamd.BC_PERIODIC),
                                   (13001, ljamd.BC_WALL), (16384, ljamd.BC_NONE)])
 def test_pair_once_kernel(N, bc):
-    """The pair-once kernel (FP32 mode, N >= 6144, groups of 256; 13001 ends in
+    """The pair-once kernel (FP32 mode, N >= 3072, groups of 256; 13001 ends in
     a partial group) against FP64 and against the sliced kernel (jslices = 32)
     on the same state; two evaluations are bit-identical."""
     rho = 0.85

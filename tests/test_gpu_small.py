@@ -142,7 +142,9 @@ def test_pair_once_sixteen_wave_groups(N, bc, waves_env):
     for waves in ("4", "2"):
         os.environ["LJ_PAIR_ONCE_WAVES"] = waves
         with ljamd.LJSystem(N, rho, 1.0, canonical=ljamd.TVN, boundary=bc) as s:
-            assert s.force_plan() == (True, (N + 128 * int(waves) - 1) // (128 * int(waves)))
+            ng = (N + 128 * int(waves) - 1) // (128 * int(waves))
+            # groups of 256 this few run every item as two workgroups (2 ng slots)
+            assert s.force_plan() == (True, 2 * ng if waves == "2" else ng)
             s.set_state(pos, vel)
             obs = s.CalculateForces()
             f4 = s.force
@@ -155,4 +157,62 @@ def test_pair_once_sixteen_wave_groups(N, bc, waves_env):
         assert rel(runs[waves][1]["V"], E) <= 1e-6 and rel(runs[waves][1]["W"], W) <= 1e-5
     dU = np.max(np.abs(runs["4"][2][:, 0] - runs["2"][2][:, 0])) / N
     dx = np.max(np.abs(xyz(runs["4"][3]) - xyz(runs["2"][3])))
+    assert dU <= 1e-5 and dx <= 1e-4, (dU, dx)
+
+
+@pytest.fixture
+def split_env():
+    old = {k: os.environ.get(k) for k in ("LJ_PAIR_ONCE", "LJ_PAIR_ONCE_SPLIT")}
+    yield
+    for k, v in old.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+
+
+@pytest.mark.parametrize("N,bc", [(8192, ljamd.BC_PERIODIC), (7001, ljamd.BC_WALL),
+                                  (6144, ljamd.BC_NONE), (300, ljamd.BC_PERIODIC)])
+def test_pair_once_split_items(N, bc, split_env):
+    """Split items (each group pair run by two workgroups, one 64-particle j
+    sub-tile of every wave each; the default while twice the items fit in one
+    round of resident workgroups) against the unsplit pass (LJ_PAIR_ONCE_SPLIT
+    = 0, the diagonal items paired in the time step) on the same state: slot
+    counts, forces against FP64 for both, the two passes' forces against each
+    other, V and W of the step variant against a full evaluation, and a 20-step
+    TVN run.  7001 ends in a partial group of 89; N = 300 forces the kernel on
+    at two groups."""
+    rho = 0.85
+    L, pos, vel = perturbed_state(N, rho, 1.0, seed=53)
+    print(f"\n[case] pair-once split N={N} bc={bc}", flush=True)
+    if N < 3072:
+        os.environ["LJ_PAIR_ONCE"] = "1"
+    ng = (N + 255) // 256
+    runs = {}
+    for flag in ("1", "0"):
+        os.environ["LJ_PAIR_ONCE_SPLIT"] = flag
+        with ljamd.LJSystem(N, rho, 1.0, canonical=ljamd.TVN, boundary=bc) as s:
+            assert s.force_plan() == (True, 2 * ng if flag == "1" else ng)
+            s.set_state(pos, vel)
+            obs = s.CalculateForces()
+            f4 = s.force
+            h = s.Integrate(0.005, 20)
+            o_step = s.observables()
+            o_full = s.CalculateForces()  # full evaluation at the positions of step 20
+            runs[flag] = (f4, obs, h, s.pos, o_step, o_full)
+    F64, _, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
+    for flag in ("1", "0"):
+        f4, obs, h, _, o_step, o_full = runs[flag]
+        err = force_errors(xyz(f4), F64)
+        print(f"  split={flag}: {err} V {rel(obs['V'], E):.2e} W {rel(obs['W'], W):.2e}")
+        assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5, (flag, err)
+        # 3e-6 (as for the ragged boxes above): e = (b - a) / 2 cancels part of
+        # the per-particle float sums; measured 1.23e-6 unsplit at N = 7001 with
+        # hard walls, 2.27e-6 split at N = 300 (V = -280 over 300 particles)
+        assert rel(obs["V"], E) <= 3e-6 and rel(obs["W"], W) <= 1e-5
+        assert abs(h[-1, 3] - o_full["V"]) <= 1e-6 * abs(o_full["V"])
+        assert abs(o_step["W"] - o_full["W"]) <= 1e-6 * abs(o_full["W"])
+    assert force_errors(xyz(runs["1"][0]), xyz(runs["0"][0]))["global_rel"] <= 2e-6
+    dU = np.max(np.abs(runs["1"][2][:, 0] - runs["0"][2][:, 0])) / N
+    dx = np.max(np.abs(xyz(runs["1"][3]) - xyz(runs["0"][3])))
     assert dU <= 1e-5 and dx <= 1e-4, (dU, dx)

@@ -54,7 +54,7 @@ def test_trajectory_vs_reference(N, rho, T0, canon, bc, nsteps, tolU):
     L, pos, vel = lattice_state(N, rho, T0, seed=77)
     print(f"\n[case] traj N={N} canon={canon} bc={bc}", flush=True)
     g, o = _pair(N, rho, T0, canon, bc, pos, vel)
-    assert g.force_plan()[0] == (N >= 6144)
+    assert g.force_plan()[0] == (N >= 3072)
     try:
         hg = g.Integrate(0.005, nsteps)
         ho = o.integrate(0.005, nsteps)
@@ -119,7 +119,7 @@ def test_graph_replay_deterministic(N, canon, bc):
     outs = []
     for chunks in ([41], [9, 32], [7, 13, 21]):
         with ljamd.LJSystem(N, rho, T0, canonical=canon, boundary=bc) as s:
-            assert s.force_plan()[0] == (N >= 6144)
+            assert s.force_plan()[0] == (N >= 3072)
             s.set_state(pos, vel)
             s.CalculateForces()
             hs = [s.Integrate(0.005, c) for c in chunks]
@@ -153,7 +153,7 @@ def test_renormalize_and_modes():
 
 @pytest.mark.parametrize("canon", [ljamd.EVN, ljamd.TVN])
 def test_step_totals_of_pair_once_pass(canon):
-    """At N >= 6144 the step's force pass sums E and W per workgroup (no
+    """At N >= 3072 the step's force pass sums E and W per workgroup (no
     per-particle energies).  Its V and W must equal a full force evaluation
     at the same positions; Force.w reads 0 after the step, as documented."""
     N, rho, T0, dt = 16384, 0.85, 1.0, 0.005
