@@ -277,6 +277,56 @@ def profile_force(kernel, reps=20, warm=600, copy_mib=256, boundary="periodic"):
                       "pair_once": pair_once, "nslots": nslots, "copy_bytes": nbytes}))
 
 
+SWEEP_N = (2048, 3072, 4096, 6144, 8192, 10000, 12288, 16384)
+# kernel choices of the sweep: the default plan, the pair-once pass with and
+# without split items, the sliced kernel (environment switches read at
+# context creation)
+SWEEP_PLANS = (("default", {}), ("pair_once_split", {"LJ_PAIR_ONCE": "1", "LJ_PAIR_ONCE_SPLIT": "1"}),
+               ("pair_once_unsplit", {"LJ_PAIR_ONCE": "1", "LJ_PAIR_ONCE_SPLIT": "0"}),
+               ("sliced", {"LJ_PAIR_ONCE": "0"}))
+
+
+def sweep_n(steps=800, reps=200):
+    """TVN step (rho* = 0.85, periodic, dt = 0.005) and force pass over N for
+    each kernel choice: the best of three timed runs of `steps` steps after 400
+    warmup steps, and the force pass alone averaged over `reps` launches (HIP
+    events). One JSON line per (N, plan)."""
+    import ljamd
+
+    keys = ("LJ_PAIR_ONCE", "LJ_PAIR_ONCE_SPLIT")
+    saved = {k: os.environ.get(k) for k in keys}
+    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED) as s:
+        s.time_force_kernel(1500)  # clocks settle
+    try:
+        for n in SWEEP_N:
+            for name, env in SWEEP_PLANS:
+                for k in keys:
+                    os.environ.pop(k, None)
+                os.environ.update(env)
+                with ljamd.LJSystem(n, RHO, T0, canonical=ljamd.TVN, seed=SEED) as s:
+                    pair_once, nslots = s.force_plan()
+                    s.time_force_kernel(reps)
+                    us = s.time_force_kernel(reps)
+                    s.Integrate(DT, 400)
+                    s.synchronize()
+                    best = float("inf")
+                    for _ in range(3):
+                        t0 = time.perf_counter()
+                        s.Integrate(DT, steps)
+                        s.synchronize()
+                        best = min(best, time.perf_counter() - t0)
+                print(json.dumps({"sweep": "tvn_step", "N": n, "plan": name, "pair_once": bool(pair_once),
+                                  "slots": nslots, "force_us": round(us, 2),
+                                  "step_us": round(1e6 * best / steps, 2),
+                                  "pairs_per_s": n * n * steps / best}), flush=True)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,7 +345,12 @@ def main():
                     help="boundary of the --profile-force workload")
     ap.add_argument("--profile-rdf", action="store_true",
                     help="only the RDF passes of the rdf leg (for rocprofv3)")
+    ap.add_argument("--sweep-n", action="store_true",
+                    help="only the TVN step and force pass over a range of N (kernel choice)")
     a = ap.parse_args()
+    if a.sweep_n:
+        sweep_n()
+        return
     if a.profile_force:
         profile_force(a.profile_force, boundary=a.profile_boundary)
         return
