@@ -462,11 +462,11 @@ __device__ __forceinline__ void n3_diag_pair(N3Lds<2>& lds, const uint4* __restr
 // ew[2 item + 1] = sum of w = (s/r)^6 (2 (s/r)^6 - 1), the finaliser's E and W
 // terms.  The j side then carries three sums instead of four.
 // One group pair (g, h) of one box: q/pf/fpart/apart point at the box's
-// arrays, ew (ENERGY = false) at the item's two totals.
+// arrays, ew (ENERGY = false) at the workgroup's two totals.
 // SPLIT: the item is run by two workgroups, half = 0 / 1 taking the first /
 // second 64-particle sub-tile of every wave's 128 j (half the pairs each), so
-// that sizes whose items fill less than one round of resident workgroups run
-// twice the waves.  Each half writes its own slots, 2 h + half for the i side
+// that sizes whose items leave most SIMDs one to three waves run twice the
+// waves.  Each half writes its own slots, 2 h + half for the i side
 // and 2 g + half for the j side (zeros for the j it did not meet): every
 // particle still receives exactly one entry per slot, S = 2 groups.
 template <bool PERIODIC, bool TIES, int W, bool ENERGY, bool SPLIT = false>
