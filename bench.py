@@ -246,11 +246,13 @@ def profiled_sclk_ghz(kernel):
         return None
 
 
-def profile_force(kernel, reps=20, warm=600, copy_mib=256, boundary="periodic"):
+def profile_force(kernel, reps=20, warm=600, copy_mib=256, boundary="periodic", n=None):
     """Launch pattern for counter collection: a device-to-device copy of a
     known size (calibrates FETCH_SIZE / WRITE_SIZE per byte moved), then the
     force kernel alone `warm` times (the clocks settle over the first ~0.1 s
-    of load) and `reps` times more, timed, at the headline workload."""
+    of load) and `reps` times more, timed, at the headline workload (or at n
+    particles, same density)."""
+    n = N if n is None else n
     import ctypes as C
 
     import ljamd
@@ -268,11 +270,11 @@ def profile_force(kernel, reps=20, warm=600, copy_mib=256, boundary="periodic"):
     hip.hipFree(dst)
     jsl = 32 if kernel == "sliced" else 0
     bc = ljamd.BC_WALL if boundary == "wall" else ljamd.BC_PERIODIC
-    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, boundary=bc, seed=SEED, jslices=jsl) as s:
+    with ljamd.LJSystem(n, RHO, T0, canonical=ljamd.TVN, boundary=bc, seed=SEED, jslices=jsl) as s:
         s.time_force_kernel(warm)
         us = s.time_force_kernel(reps)
         pair_once, nslots = s.force_plan()
-    print(json.dumps({"profile": "force", "N": N, "boundary": boundary, "warm": warm, "reps": reps,
+    print(json.dumps({"profile": "force", "N": n, "boundary": boundary, "warm": warm, "reps": reps,
                       "kernel_us": us,
                       "pair_once": pair_once, "nslots": nslots, "copy_bytes": nbytes}))
 
@@ -343,6 +345,8 @@ def main():
                     help="only a calibration copy and the force kernel (for rocprofv3)")
     ap.add_argument("--profile-boundary", default="periodic", choices=["periodic", "wall"],
                     help="boundary of the --profile-force workload")
+    ap.add_argument("--profile-n", type=int, default=None,
+                    help="particles of the --profile-force workload (default: the headline N)")
     ap.add_argument("--profile-rdf", action="store_true",
                     help="only the RDF passes of the rdf leg (for rocprofv3)")
     ap.add_argument("--sweep-n", action="store_true",
@@ -352,7 +356,7 @@ def main():
         sweep_n()
         return
     if a.profile_force:
-        profile_force(a.profile_force, boundary=a.profile_boundary)
+        profile_force(a.profile_force, boundary=a.profile_boundary, n=a.profile_n)
         return
     if a.profile_rdf:
         print(json.dumps({"profile": "rdf", "N": N, **rdf_leg()}))
