@@ -216,3 +216,22 @@ def test_pair_once_split_items(N, bc, split_env):
     dU = np.max(np.abs(runs["1"][2][:, 0] - runs["0"][2][:, 0])) / N
     dx = np.max(np.abs(xyz(runs["1"][3]) - xyz(runs["0"][3])))
     assert dU <= 1e-5 and dx <= 1e-4, (dU, dx)
+
+
+# the kernel choice at the sizes the measurements cover (rho* = 0.85, FP32):
+# (N, pair-once, partial slots per particle); split items give 2 per group
+PLAN_CASES = [(2048, False, 32), (3071, False, 48), (3072, True, 24), (4096, True, 32),
+              (6144, True, 48), (8192, True, 32), (9216, True, 72), (10000, True, 80),
+              (11264, True, 44), (12288, True, 48), (13001, True, 51), (16384, True, 64)]
+
+
+@pytest.mark.parametrize("N,pair_once,slots", PLAN_CASES)
+def test_force_plan_choice(N, pair_once, slots, split_env):
+    """Sliced below N = 3072; pair-once above, with split items (2 slots per
+    group of 256) where the occupancy model of a 256-CU device predicts a 10%
+    faster pass: 12-31, 33-38 and 40-41 groups (N = 3072-7936, 8193-9728,
+    9985-10496), not 32 groups (N = 8192) nor from 42 on."""
+    for k in ("LJ_PAIR_ONCE", "LJ_PAIR_ONCE_SPLIT"):
+        os.environ.pop(k, None)
+    with ljamd.LJSystem(N, 0.85, 1.0, canonical=ljamd.TVN) as s:
+        assert s.force_plan() == (pair_once, slots)
