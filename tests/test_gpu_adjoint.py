@@ -147,6 +147,32 @@ def test_trajectory_gradient_open_box_small():
     assert np.linalg.norm(dx0[:, :3] - ref_dx0) <= 1e-4 * np.linalg.norm(ref_dx0)
 
 
+def test_trajectory_gradient_pair_once_forces():
+    """The forward and recompute force passes of the gradient on the pair-once
+    kernel (split items, forced on at N = 300: two groups of 256, 4 slots per
+    particle) against the FP64 gradient, as on the sliced kernel above."""
+    import os
+
+    N, rho = 300, 0.3
+    L, pos, vel = perturbed_state(N, rho, 1.0, seed=21)
+    old = os.environ.get("LJ_PAIR_ONCE")
+    os.environ["LJ_PAIR_ONCE"] = "1"
+    try:
+        with ljamd.LJSystem(N, rho, 1.0, boundary=ljamd.BC_NONE, eps=1.1, sigma=1.05) as s:
+            assert s.force_plan() == (True, 4)
+            s.set_state(pos, vel)
+            out, g2, dx0, _ = s.trajectory_grad(0.004, 50, 10, p_target=0.2, w_p=3.0)
+    finally:
+        if old is None:
+            os.environ.pop("LJ_PAIR_ONCE", None)
+        else:
+            os.environ["LJ_PAIR_ONCE"] = old
+    ref_out, ref_g2, ref_dx0 = traj_grad_fp64(xyz(pos), xyz(vel), L, 0, 1.1, 1.05, 0.004, 50,
+                                              pt=0.2, wp=3.0)
+    assert np.all(np.abs(g2 - ref_g2) <= 1e-4 * np.abs(ref_g2))
+    assert np.linalg.norm(dx0[:, :3] - ref_dx0) <= 1e-4 * np.linalg.norm(ref_dx0)
+
+
 def test_trajectory_errors():
     with ljamd.LJSystem(200, 0.5, 1.0, canonical=ljamd.TVN) as s:
         with pytest.raises(ljamd.LJError):
