@@ -15,9 +15,16 @@ against the FP32 vector peak.  At this size the kernel evaluates each unordered
 pair once (Newton's third law, N (N - 1) / 2 evaluations); it is still
 credited N^2 ordered pairs, and the sliced kernel that evaluates both orders
 is timed beside it.
-cpu_baseline: the reference's own CPU path (MDSystem.cpp compiled from
-source into oracle/_ref) running the same workload for a bounded number of
-steps on one host thread, as the reference does.
+cpu_baseline: the oracle's C restatement of the reference CPU path
+(oracle/lj_oracle.c, bit-identical to the reference's MDSystem.cpp compiled
+from source: tests/test_oracle_pins.py::test_oracle_bitexact_vs_reference and
+the committed reference-run fixtures, tests/test_oracle_golden.py) running the
+same workload for a bounded number of steps on one host thread, as the
+reference does; the same code over 16 host threads beside it.
+
+Clocks: the GPU's shader clock ramps up over the first ~0.1 s of sustained
+load, so every timed leg first runs force passes for at least SETTLE_MS
+(untimed, bounded), reported as settle_ms.
 """
 import argparse
 import json
@@ -33,6 +40,12 @@ PEAK_FP32_TFLOPS = 157.3        # MI355X FP32 vector peak (packed FMA issue)
 PEAK_FP32_SCALAR_TFLOPS = 78.65  # the same without packed FP32 issue
 FLOP_PER_PAIR = 20
 N, RHO, T0, DT, SEED = 16384, 0.85, 1.0, 0.005, 2024
+SETTLE_MS = 250.0       # untimed force passes before each timed leg
+SETTLE_MAX_MS = 3000.0  # bound of the settle phase
+KERNEL_REPS = 200       # force launches averaged for roofline.kernel_us
+# SQ_INSTS_VALU x 64 / N^2 of the pair-once step pass (12.35, rocprofv3) plus
+# the half-rate v_rcp_f32, one per unordered pair
+ISSUE_SLOTS_N3 = 12.85
 
 
 def log(msg):
@@ -51,12 +64,25 @@ def profiled_traffic(kernel):
         return None
 
 
+def settle(s, min_ms=SETTLE_MS, max_ms=SETTLE_MAX_MS):
+    """Untimed force passes on context s until min_ms of wall time have passed
+    (at most max_ms): the shader clock settles under sustained load. Returns
+    the milliseconds spent."""
+    t0 = time.perf_counter()
+    while True:
+        s.time_force_kernel(100)
+        ms = 1000.0 * (time.perf_counter() - t0)
+        if ms >= min_ms or ms >= max_ms:
+            return ms
+
+
 def gpu_leg(steps, warmup, boundary=None):
     import ljamd
 
     bc = ljamd.BC_PERIODIC if boundary is None else boundary
     s = ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, boundary=bc, seed=SEED)
-    log(f"[bench] warmup {warmup} steps")
+    settle_ms = settle(s)
+    log(f"[bench] clocks settled ({settle_ms:.0f} ms of force passes); warmup {warmup} steps")
     s.Integrate(DT, max(warmup, 1))
     s.synchronize()
     log(f"[bench] timing {steps} steps")
@@ -65,23 +91,24 @@ def gpu_leg(steps, warmup, boundary=None):
     s.synchronize()
     t = time.perf_counter() - t0
     obs = s.observables()
-    reps = 50
+    reps = KERNEL_REPS
     us = s.time_force_kernel(reps)
     pair_once, nslots = s.force_plan()
     s.close()
     return {"t": t, "hist": hist, "obs": obs, "force_us": us, "reps": reps,
-            "pair_once": pair_once, "nslots": nslots}
+            "pair_once": pair_once, "nslots": nslots, "settle_ms": settle_ms}
 
 
-def sliced_kernel_us(reps=50):
+def sliced_kernel_us(reps=KERNEL_REPS):
     """The sliced kernel (both orders of every pair) on the same workload."""
     import ljamd
 
     with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED, jslices=32) as s:
+        settle(s)
         return s.time_force_kernel(reps)
 
 
-def batch_leg(steps, plain_steps=10, nboxes=512):
+def batch_leg(steps, plain_steps=40, nboxes=512):
     """512 independent periodic EVN boxes (N ~ U{200..4000}, rho* ~ U[0.05,0.9],
     T* ~ U[0.8,2.0], seeded) stepped by one captured batched graph, against a
     loop that launches each box's own single-box step graph."""
@@ -94,6 +121,9 @@ def batch_leg(steps, plain_steps=10, nboxes=512):
     b = ljamd.LJBatch(N, rho, T0, canonical=ljamd.EVN, seed=SEED)
     log("[bench] batch autotune")
     tuning, tune_us = b.autotune(reps=3)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < SETTLE_MS / 1000.0:  # clocks settle
+        b.time_force(20)
     b.integrate(DT, 3)
     log(f"[bench] batch timing {steps} steps (tuning {tuning})")
     t0 = time.perf_counter()
@@ -137,6 +167,7 @@ def adjoint_leg():
 
     Na, rho, M, dt = 2048, 0.85, 200, 0.005
     with ljamd.LJSystem(Na, rho, 1.0, seed=4) as s:
+        settle(s)
         s.trajectory_grad(dt, M, 20)  # warm: graph capture, scratch
         out, g2, _, t20 = s.trajectory_grad(dt, M, 20)
         _, _, _, t1 = s.trajectory_grad(dt, M, 1)
@@ -148,33 +179,36 @@ def adjoint_leg():
 
 
 def cpu_leg(budget_s=15.0):
-    """Reference CPU path (oracle/_ref/libref_cpu.so) on the same workload."""
+    """The reference CPU path on the same workload, as the oracle restates it
+    (oracle/lj_oracle.c: float state, double pair terms, float force sums over
+    j ascending, MDSystem.cpp:232-311,438-510), one host thread as the
+    reference runs. The restatement is bit-identical to the reference compiled
+    from its sources (tests/test_oracle_pins.py::test_oracle_bitexact_vs_reference)
+    and to the committed reference-run fixtures (tests/test_oracle_golden.py)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import RefSystem, lattice_state, ref_lib
+    from oracle_lib import OracleSystem, lattice_state
 
-    lib = ref_lib("cpu")
-    kind = "reference"
     L, pos, vel = lattice_state(N, RHO, T0, SEED)
-    if lib is None:
-        return None
-    r = RefSystem(lib, N, T0, RHO, 1, 0, use_gpu=False)
-    r.set_state(pos, vel)
-    r.forces()
-    r.params()
+    o = OracleSystem(N, T0, RHO, 1, 0, nthreads=1)
+    o.set_state(pos, vel)
+    o.forces()
+    o.params()
     t0 = time.perf_counter()
-    r.integrate(DT, 1)
+    o.integrate(DT, 1)
     t1 = time.perf_counter() - t0
     k = max(1, min(10, int(budget_s / max(t1, 1e-3)) - 1))
     t0 = time.perf_counter()
-    r.integrate(DT, k)
+    o.integrate(DT, k)
     tk = time.perf_counter() - t0
-    r.close()
     steps = k + 1
     tt = t1 + tk
     return {"value": N * N * steps / tt, "unit": "pair interactions/s", "cores": 1,
-            "kind": kind,
-            "sample": f"{steps} TVN steps of MDSystem::Integrate (useCUDA=false) at N={N}, "
-                      f"rho*={RHO}, 1 host thread, {tt:.1f} s",
+            "kind": "port",
+            "sample": f"{steps} TVN steps of the oracle's restatement of MDSystem::Integrate "
+                      f"(the reference CPU path, bit-identical to it) at N={N}, rho*={RHO}, "
+                      f"1 host thread, {tt:.1f} s",
+            "pin": "tests/test_oracle_pins.py::test_oracle_bitexact_vs_reference, "
+                   "tests/test_oracle_golden.py",
             "steps_per_s": steps / tt}
 
 
@@ -185,6 +219,7 @@ def rdf_leg(reps=20):
     import ljamd
 
     with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED) as s:
+        settle(s)
         s.Integrate(DT, 20)
         s.synchronize()
         us_r = s.time_rdf(ljamd.RDF_RBINS, 500, reps=reps)
@@ -411,20 +446,30 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "frac_scalar_fp32": achieved / PEAK_FP32_SCALAR_TFLOPS,
                      "sclk_ghz_measured": sclk,
+                     "sclk_ghz_source": f"profiles/{kname}_n16384_periodic.md (amd-smi samples "
+                                        "under load); not measured in this run",
                      "peak_scalar_at_measured_sclk": peak_sclk,
                      "frac_scalar_at_measured_sclk": achieved / peak_sclk if peak_sclk else None,
                      "kernel": ("k_force_n3 (each unordered pair once, periodic fixed point)"
                                 if g["pair_once"] else "k_force (sliced, periodic fixed point)"),
                      "kernel_us": g["force_us"],
+                     "kernel_us_source": f"HIP events over {g['reps']} back-to-back launches "
+                                         "on the kernel's stream, this run, after "
+                                         f"{g['settle_ms']:.0f} ms of settling force passes",
                      "traffic": profiled_traffic(kname),
+                     "traffic_source": f"profiles/{kname}_n16384_periodic.md (rocprofv3 --pmc "
+                                       "FETCH_SIZE and WRITE_SIZE, scaled by a calibration copy); "
+                                       "not measured in this run",
                      "newton3": g["pair_once"],
                      "pair_evaluations_per_launch": (N * (N - 1) // 2 if g["pair_once"]
                                                      else N * N),
-                     # SQ_INSTS_VALU x 64 / N^2 (12.35 pair-once, 21.46 sliced, measured)
-                     # + the half-rate v_rcp_f32 (one per unordered / ordered pair)
-                     "issue_slots_per_ordered_pair": 12.85 if g["pair_once"] else 22.5,
+                     "issue_slots_per_ordered_pair": ISSUE_SLOTS_N3 if g["pair_once"] else 22.5,
+                     "issue_slots_source": f"profiles/{kname}_n16384_periodic.md (rocprofv3 "
+                                           "--pmc SQ_INSTS_VALU); a constant, not measured in "
+                                           "this run",
                      "sliced_kernel_us": sliced_us,
                      "sliced_kernel_tflops": FLOP_PER_PAIR * pairs / (sliced_us * 1e-6) / 1e12},
+        "settle_ms": g["settle_ms"],
         "cpu_baseline": cpu,
         "hard_wall": wall,
         "batched": batched,

@@ -26,6 +26,7 @@
 
 #include "lj_adjoint.h"
 #include "lj_ctx.h"
+#include "lj_fp64.h"
 
 using namespace lj;
 
@@ -46,6 +47,7 @@ struct AdjState {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   double g_dt = 0., g_eps = 0., g_sigma = 0.;
+  int g_bc = -1;  // boundary the captured step was built for (periodic flag, image scale)
 };
 
 namespace {
@@ -133,7 +135,9 @@ int enqueue_dstep(lj_ctx* c, hipStream_t st, double dt)
 int ensure_dgraph(lj_ctx* c, double dt)
 {
   AdjState* a = c->adj;
-  if (a->gexec && a->g_dt == dt && a->g_eps == c->cfg.eps && a->g_sigma == c->cfg.sigma) return LJ_OK;
+  if (a->gexec && a->g_dt == dt && a->g_eps == c->cfg.eps && a->g_sigma == c->cfg.sigma &&
+      a->g_bc == c->cfg.boundary)
+    return LJ_OK;
   adj_drop_graph(a);
   hipStream_t st = c->stream;
   LJ_HIP(hipStreamSynchronize(st));
@@ -151,6 +155,7 @@ int ensure_dgraph(lj_ctx* c, double dt)
   a->g_dt = dt;
   a->g_eps = c->cfg.eps;
   a->g_sigma = c->cfg.sigma;
+  a->g_bc = c->cfg.boundary;
   return LJ_OK;
 }
 
@@ -249,6 +254,8 @@ int lj_set_lj_params(lj_ctx* c, double eps, double sigma)
 {
   LJ_TRY(ctx_setdev(c));
   if (!(eps > 0.) || !(sigma > 0.)) return fail(LJ_ERR_ARG, "lj_set_lj_params: eps and sigma must be positive");
+  // replays enqueued by lj_integrate_async may still run the graphs dropped here
+  LJ_HIP(hipStreamSynchronize(c->stream));
   c->cfg.eps = eps;
   c->cfg.sigma = sigma;
   refresh_scalars(c);
@@ -263,6 +270,8 @@ int lj_set_array_device(lj_ctx* c, int type, const float* d_src4, void* stream)
   if (!d || !d_src4) return fail(LJ_ERR_ARG, "lj_set_array_device: bad type or pointer");
   hipStream_t st = stream ? (hipStream_t)stream : c->stream;
   LJ_HIP(hipMemcpyAsync(d, d_src4, sizeof(float4) * c->cfg.N, hipMemcpyDeviceToDevice, st));
+  // an FP64 context steps its double state: widen the new values into it, as lj_set_array does
+  if (is_fp64(c)) LJ_HIP(launch_widen(st, d, state64(c, type), c->cfg.N));
   LJ_HIP(hipStreamSynchronize(st));
   return LJ_OK;
 }

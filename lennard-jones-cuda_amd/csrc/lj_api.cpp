@@ -792,6 +792,8 @@ int lj_set_mode(lj_ctx* c, int canonical, int boundary)
   LJ_TRY(ctx_setdev(c));
   if (canonical != 0 && canonical != 1) return fail(LJ_ERR_ARG, "lj_set_mode: canonical must be 0 or 1");
   if (boundary < 0 || boundary > 2) return fail(LJ_ERR_ARG, "lj_set_mode: boundary must be 0, 1 or 2");
+  // replays enqueued by lj_integrate_async may still run the graphs dropped here
+  LJ_HIP(hipStreamSynchronize(c->stream));
   c->cfg.canonical = canonical;
   c->cfg.boundary = boundary;
   refresh_scalars(c);

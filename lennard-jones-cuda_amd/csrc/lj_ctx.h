@@ -72,6 +72,8 @@ void drop_graph(lj_ctx* c);
 // releases the adjoint scratch of a context
 void adj_free(lj_ctx* c);
 inline bool is_fp64(const lj_ctx* c) { return c->cfg.precision == LJ_PREC_FP64; }
+// the double state array of LJ_ARRAY_POS / VEL / FORCE (LJ_PREC_FP64 contexts)
+double4* state64(lj_ctx* c, int type);
 // plan of the force pass that fills fpart/apart
 inline const ForcePlan& fplan(const lj_ctx* c) { return c->plan3.n3 ? c->plan3 : c->plan; }
 }  // namespace lj

@@ -82,6 +82,7 @@ def test_batch_trajectory_matches_single_box(canon):
             p = s.pos
         du = abs(obs_b[k]["U"] - o["U"]) / N[k]
         dx = np.max(np.abs(xyz(pos_b[k]) - xyz(p)))
+        print(f"  box {k} N={N[k]}: |dU|/N {du:.2e} max|dx| {dx:.2e}")
         assert obs_b[k]["step"] == nsteps
         assert du <= 1e-4 and dx <= 1e-3, (k, du, dx)
 
@@ -168,14 +169,67 @@ def test_batch_fluctuation_stats():
         assert np.all(var >= -1e-9 * np.abs(st[:, 2] / 6))
 
 
-def test_batch_config_workload_builds():
-    """The 512-box workload (N ~ U{200..4000}) builds, steps and stays finite."""
+def _ensemble_picks(N, rho, k_each=4):
+    """Boxes of the 512-box workload that stress the batched pass: the largest
+    N, the densest, the most dilute, and the smallest partial last groups of
+    256 (N mod 256 in 1..16)."""
+    picks = []
+    for order in (np.argsort(-N, kind="stable"), np.argsort(-rho, kind="stable"),
+                  np.argsort(rho, kind="stable")):
+        picks += [int(k) for k in order if int(k) not in picks][:k_each]
+    part = [int(k) for k in np.argsort(N % 256, kind="stable") if N[k] % 256 > 0]
+    picks += [k for k in part if k not in picks][:k_each]
+    return picks
+
+
+def test_batch_ensemble_512_full_size():
+    """The 512-box workload itself (N ~ U{200..4000}, rho* ~ U[0.05, 0.9],
+    T* ~ U[0.8, 2.0], seed 7) with the autotuner's tuning: forces, V and W of
+    16 boxes (largest N, densest, most dilute, smallest partial groups) against
+    the FP64 oracle on perturbed lattice states, then 20 EVN steps from the
+    seeded initial state against single-box contexts of the same boxes."""
     N, rho, T0 = ljamd.ensemble_boxes(512, seed=7)
-    with ljamd.LJBatch(N, rho, T0, seed=1) as b:
-        b.integrate(0.005, 3)
+    picks = _ensemble_picks(N, rho)
+    assert len(picks) == 16
+    print(f"\n[case] 512-box ensemble, boxes {picks}", flush=True)
+    states = [_perturbed(N[k], rho[k], T0[k], 300 + k) for k in range(len(N))]
+    with ljamd.LJBatch(N, rho, T0) as b:
+        tuning, _ = b.autotune(reps=2)
+        print(f"  autotuned {tuning}", flush=True)
+        b.set_array(ljamd.ARRAY_POS, np.concatenate([st[1] for st in states]))
+        b.set_array(ljamd.ARRAY_VEL, np.concatenate([st[2] for st in states]))
+        b.compute_forces()
         obs = b.observables()
-    U = np.array([o["U"] for o in obs])
-    assert np.all(np.isfinite(U)) and all(o["step"] == 3 for o in obs)
+        worst = {"F": 0.0, "V": 0.0, "W": 0.0}
+        for k in picks:
+            L, pos, _ = states[k]
+            F64, _, E, W = forces_fp64(pos, L, True)
+            err = force_errors(xyz(b.get_array(ljamd.ARRAY_FORCE, box=k)), F64)
+            assert err["global_rel"] <= 2e-6 and err["max_rel_floor"] <= 1e-5, (k, int(N[k]), err)
+            assert rel(obs[k]["V"], E) <= 1e-6 and rel(obs[k]["W"], W) <= 1e-5, k
+            worst["F"] = max(worst["F"], err["global_rel"])
+            worst["V"] = max(worst["V"], rel(obs[k]["V"], E))
+            worst["W"] = max(worst["W"], rel(obs[k]["W"], W))
+        print(f"  vs FP64 over the 16 boxes: F {worst['F']:.2e} V {worst['V']:.2e} "
+              f"W {worst['W']:.2e}", flush=True)
+        nsteps, seed = 20, 31
+        b.sample_initial_state(seed)
+        b.integrate(0.005, nsteps)
+        obs_b = b.observables()
+        pos_b = {k: b.get_array(ljamd.ARRAY_POS, box=k) for k in picks}
+    du_max = dx_max = 0.0
+    for k in picks:
+        with ljamd.LJSystem(int(N[k]), float(rho[k]), float(T0[k]), canonical=ljamd.EVN) as s:
+            s.sample_initial_state(seed + k)
+            s.Integrate(0.005, nsteps)
+            o = s.observables()
+            p = s.pos
+        assert obs_b[k]["step"] == nsteps
+        du = abs(obs_b[k]["U"] - o["U"]) / N[k]
+        dx = float(np.max(np.abs(xyz(pos_b[k]) - xyz(p))))
+        du_max, dx_max = max(du_max, du), max(dx_max, dx)
+        assert du <= 1e-4 and dx <= 1e-3, (k, int(N[k]), du, dx)
+    print(f"  20 steps vs single-box contexts: max |dU|/N {du_max:.2e}, max |dx| {dx_max:.2e}")
 
 
 def test_batch_bad_args():
