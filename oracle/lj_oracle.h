@@ -178,6 +178,38 @@ void orc_traj_grad_fp64(int N, const double* x0, const double* v0, double L, int
                         double eps, double sigma, double dt, int M, double p_target, double w_p,
                         double* out3, double* grad2, double* dx0, int nthreads);
 
+/* ---------------- fluctuation observables (lj_oracle_flucs.c) ---------------- */
+
+/* alpha ladder of the batch counters: alpha = step, += step while alpha < limit
+ * (run-fluctuations-aux.h:198 with limit 1 - 1e-9, :251 with 1 + 1e-9);
+ * returns its size, writes up to cap values (alphas may be NULL) */
+int orc_fl_ladder(double step, double limit, double* alphas, int cap);
+/* GetNSubsystemBatch (run-fluctuations-aux.h:191-241): cumulative counts */
+int orc_fl_nsub_batch(int N, const float* pos4, double L, double alpha_step, int type, int* cnts,
+                      int cap);
+/* GetNsubVzBatch (run-fluctuations-aux.h:246-278) */
+int orc_fl_vz_batch(int N, const float* vel4, double vcut_max, double alpha_step, int type,
+                    int* cnts, int cap);
+/* GetNSubsystem (:128-160), GetNsubVz (:165-186) */
+int orc_fl_nsub(int N, const float* pos4, double L, double fraction, int type);
+int orc_fl_vz(int N, const float* vel4, double vcut, int type);
+/* count of MDSystem::Fluctuations (MDSystem.cpp:708-719) */
+long long orc_fl_closed_cube(int N, const float* pos4, double L, double fraction);
+
+/* TimeAverage (time-average-aux.h:27-67) over NumberStatistics moment sums */
+typedef struct orc_tavg {
+  double sums[5]; /* sum of N^k, k = 0..4 */
+  int64_t nobs;
+  double last, corr;
+  int iters;
+} orc_tavg;
+#define ORC_TAVG_NOUT 7
+void orc_tavg_init(orc_tavg* a);
+void orc_tavg_add(orc_tavg* a, double val);
+/* {mean, variance, s, mean error * sqrt(s), scaled variance, scaled variance
+ * error, plain mean error} */
+void orc_tavg_eval(const orc_tavg* a, double* out);
+
 #ifdef __cplusplus
 }
 #endif

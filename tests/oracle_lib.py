@@ -82,10 +82,67 @@ def _load_oracle():
     lib.orc_traj_grad_fp64.argtypes = [C.c_int, _f64p, _f64p, C.c_double, C.c_int, C.c_double,
                                        C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                        _f64p, _f64p, C.c_void_p, C.c_int]
+    _i32p_ = _i32p
+    lib.orc_fl_ladder.restype = C.c_int
+    lib.orc_fl_ladder.argtypes = [C.c_double, C.c_double, C.c_void_p, C.c_int]
+    lib.orc_fl_nsub_batch.restype = C.c_int
+    lib.orc_fl_nsub_batch.argtypes = [C.c_int, _f32p, C.c_double, C.c_double, C.c_int, _i32p_,
+                                      C.c_int]
+    lib.orc_fl_vz_batch.restype = C.c_int
+    lib.orc_fl_vz_batch.argtypes = [C.c_int, _f32p, C.c_double, C.c_double, C.c_int, _i32p_,
+                                    C.c_int]
+    lib.orc_fl_nsub.restype = C.c_int
+    lib.orc_fl_nsub.argtypes = [C.c_int, _f32p, C.c_double, C.c_double, C.c_int]
+    lib.orc_fl_vz.restype = C.c_int
+    lib.orc_fl_vz.argtypes = [C.c_int, _f32p, C.c_double, C.c_int]
+    lib.orc_fl_closed_cube.restype = C.c_longlong
+    lib.orc_fl_closed_cube.argtypes = [C.c_int, _f32p, C.c_double, C.c_double]
+    lib.orc_tavg_init.argtypes = [C.c_void_p]
+    lib.orc_tavg_add.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_tavg_eval.argtypes = [C.c_void_p, _f64p]
     return lib
 
 
 ORC = _load_oracle()
+
+
+class OrcTavg(C.Structure):
+    _fields_ = [("sums", C.c_double * 5), ("nobs", C.c_int64), ("last", C.c_double),
+                ("corr", C.c_double), ("iters", C.c_int)]
+
+
+def fl_ladder(step, limit):
+    """Alpha ladder of the reference's batch counters (accumulated steps)."""
+    n = ORC.orc_fl_ladder(step, limit, None, 0)
+    a = np.zeros(n, np.float64)
+    ORC.orc_fl_ladder(step, limit, a.ctypes.data, n)
+    return a
+
+
+def fl_nsub_batch(pos, L, step, typ):
+    cnt = np.zeros(4096, np.int32)
+    m = ORC.orc_fl_nsub_batch(pos.size // 4, np.ascontiguousarray(pos, np.float32), L, step, typ,
+                              cnt, cnt.size)
+    return cnt[:m].copy()
+
+
+def fl_vz_batch(vel, vmax, step, typ):
+    cnt = np.zeros(4096, np.int32)
+    m = ORC.orc_fl_vz_batch(vel.size // 4, np.ascontiguousarray(vel, np.float32), vmax, step, typ,
+                            cnt, cnt.size)
+    return cnt[:m].copy()
+
+
+def tavg_eval(values):
+    """TimeAverage statistics of a sequence (mean, variance, s, mean error,
+    scaled variance, scaled variance error, plain mean error)."""
+    a = OrcTavg()
+    ORC.orc_tavg_init(C.byref(a))
+    for v in values:
+        ORC.orc_tavg_add(C.byref(a), float(v))
+    out = np.zeros(7, np.float64)
+    ORC.orc_tavg_eval(C.byref(a), out)
+    return out
 
 
 def lattice_state(N, rho, T0, seed):
@@ -224,6 +281,18 @@ def ref_lib(kind="cpu"):
     lib.refh_L.argtypes = [C.c_void_p]
     lib.refh_dr2.restype = C.c_float
     lib.refh_dr2.argtypes = [C.c_void_p]
+    if hasattr(lib, "refh_fl_nsub_batch"):
+        for n in ("refh_fl_nsub_batch", "refh_fl_vz_batch"):
+            getattr(lib, n).restype = C.c_int
+        lib.refh_fl_nsub_batch.argtypes = [C.c_void_p, C.c_double, C.c_int, _i32p, C.c_int]
+        lib.refh_fl_vz_batch.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _i32p,
+                                         C.c_int]
+        lib.refh_fl_nsub.restype = C.c_int
+        lib.refh_fl_nsub.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        lib.refh_fl_vz.restype = C.c_int
+        lib.refh_fl_vz.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        lib.refh_fl_fluctuations.argtypes = [C.c_void_p, C.c_double, _f64p]
+        lib.refh_tavg.argtypes = [_f64p, C.c_int, _f64p]
     lib.refh_fast_round.restype = C.c_int
     lib.refh_fast_round.argtypes = [C.c_void_p, C.c_float]
     lib.refh_rdf_normalised.argtypes = [C.c_void_p, _f64p, _f64p]

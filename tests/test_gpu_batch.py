@@ -84,7 +84,8 @@ def test_batch_trajectory_matches_single_box(canon):
         dx = np.max(np.abs(xyz(pos_b[k]) - xyz(p)))
         print(f"  box {k} N={N[k]}: |dU|/N {du:.2e} max|dx| {dx:.2e}")
         assert obs_b[k]["step"] == nsteps
-        assert du <= 1e-4 and dx <= 1e-3, (k, du, dx)
+        # about 10x the measured divergence (max 1.6e-6 and 3.8e-6 on MI355X)
+        assert du <= 2e-5 and dx <= 4e-5, (k, du, dx)
 
 
 @pytest.mark.parametrize("canon,bc", [(ljamd.EVN, ljamd.BC_PERIODIC), (ljamd.TVN, ljamd.BC_WALL)])
@@ -228,7 +229,8 @@ def test_batch_ensemble_512_full_size():
         du = abs(obs_b[k]["U"] - o["U"]) / N[k]
         dx = float(np.max(np.abs(xyz(pos_b[k]) - xyz(p))))
         du_max, dx_max = max(du_max, du), max(dx_max, dx)
-        assert du <= 1e-4 and dx <= 1e-3, (k, int(N[k]), du, dx)
+        # about 10x the measured divergence (max 6.9e-7 and 1.9e-6 on MI355X)
+        assert du <= 7e-6 and dx <= 2e-5, (k, int(N[k]), du, dx)
     print(f"  20 steps vs single-box contexts: max |dU|/N {du_max:.2e}, max |dx| {dx_max:.2e}")
 
 

@@ -2,7 +2,7 @@
 C++ program the way the reference's task programs drive MDSystem
 (configure, Integrate in a loop, read the public fields), against the
 reference CPU path restated in the oracle from the same seeded state:
-per-step U within |dU|/N <= 2e-5 over 200 EVN steps (N = 400, rho* = 0.05,
+per-step U within |dU|/N <= 5e-6 over 200 EVN steps (N = 400, rho* = 0.05,
 T* = 1.4, periodic, dt = 0.005) and NdNdr2 bit-exact with the reference GPU
 binning on the final positions; the speed distribution (initvelo /
 updatevelo) bit-exact against the oracle's counts."""
@@ -53,7 +53,7 @@ def test_mdsystem_class_vs_reference_cpu():
     assert np.allclose(st[:, 1], dt * (np.arange(steps) + 1), rtol=0, atol=1e-12)
     dU = np.abs(st[:, 2] - ho[:, 0]) / N
     print(f"  max |dU|/N {dU.max():.2e}  final U {st[-1, 2]:.9g} vs {ho[-1, 0]:.9g}")
-    assert dU.max() <= 2e-5
+    assert dU.max() <= 5e-6  # about 10x the measured 5.4e-7
     assert np.max(np.abs(st[:, 3] - ho[:, 1])) <= 2e-4
     # NdNdr2 of the final positions: the reference GPU rule (MDSystem.cu:39), bit-exact
     p = np.array(out["pos"], np.float32)

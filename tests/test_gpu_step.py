@@ -41,11 +41,13 @@ def test_initial_state_matches_oracle():
 
 
 @pytest.mark.parametrize("N,rho,T0,canon,bc,nsteps,tolU", [
-    (400, 0.05, 1.4, ljamd.EVN, ljamd.BC_PERIODIC, 400, 2e-5),   # the N = 400 workload, a slice
-    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_PERIODIC, 60, 2e-4),   # the liquid at N = 2048 (TVN)
-    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_WALL, 60, 2e-4),       # the liquid at N = 2048 (hard wall)
-    (1000, 0.3, 1.0, ljamd.EVN, ljamd.BC_WALL, 100, 1e-4),
-    (600, 0.2, 1.0, ljamd.EVN, ljamd.BC_NONE, 60, 1e-4),
+    # tolerances about 10x the measured divergence (MI355X): 7e-7, 7e-7, 1.4e-6,
+    # 4.4e-7, 1.4e-7, 3.5e-7, 8.5e-7 at the sampled steps
+    (400, 0.05, 1.4, ljamd.EVN, ljamd.BC_PERIODIC, 400, 1e-5),   # the N = 400 workload, a slice
+    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_PERIODIC, 60, 1e-5),   # the liquid at N = 2048 (TVN)
+    (2048, 0.85, 1.0, ljamd.TVN, ljamd.BC_WALL, 60, 2e-5),       # the liquid at N = 2048 (hard wall)
+    (1000, 0.3, 1.0, ljamd.EVN, ljamd.BC_WALL, 100, 5e-6),
+    (600, 0.2, 1.0, ljamd.EVN, ljamd.BC_NONE, 60, 2e-6),
     # the pair-once pass with a partial last group of 256 (16 and 201 particles)
     (10000, 0.85, 1.0, ljamd.TVN, ljamd.BC_PERIODIC, 10, 2e-6),
     (13001, 0.85, 1.0, ljamd.EVN, ljamd.BC_WALL, 10, 2e-6),
@@ -62,7 +64,8 @@ def test_trajectory_vs_reference(N, rho, T0, canon, bc, nsteps, tolU):
         dT = np.abs(hg[:, 1] - ho[:, 1])
         dx = np.max(np.abs(xyz(g.pos) - xyz(o.pos)))
         k = [0, nsteps // 4, nsteps // 2, nsteps - 1]
-        print(f"  |dU|/N at steps {k}: {dU[k]}  |dT|: {dT[k]}  final max|dx| {dx:.2e}")
+        print(f"  |dU|/N at steps {k}: {dU[k]}  |dT|: {dT[k]}  final max|dx| {dx:.2e}; "
+              f"max |dU|/N {dU.max():.2e}, max |dT| {dT.max():.2e}")
         assert dU.max() <= tolU
         assert dT.max() <= 10 * tolU
         obs = g.observables()
