@@ -93,6 +93,22 @@ void threadSync(void);
 void calculateNForces(float* Pos, float* Force, float* host_pressure, int numBodies,
                       float host_L, int Lperiodic, int* host_RDF, float host_dr2,
                       int blockSize, int q);
+/*
+ * calculateNForces without device allocation.  No counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and synchronous behaviour as calculateNForces, with
+ * the same kernels (bit-identical results), but every device scratch buffer
+ * lives in a file-scope device array of 12.3 MiB that each GPU the library
+ * runs on holds until the process ends; no hipMalloc, hipFree or stream
+ * creation.  Runs on the null stream.  numBodies <= 65536.  Returns LJ_OK or
+ * a negative status (message in lj_last_error()).  The array is shared by
+ * every call on a GPU and keeps the last call's contents: calls overlapping
+ * on one GPU would race, so the calls of one process are serialised.
+ */
+int calculateNForcesNoAlloc(float* Pos, float* Force, float* host_pressure, int numBodies,
+                            float host_L, int Lperiodic, int* host_RDF, float host_dr2,
+                            int blockSize, int q);
 /* Releases the compatibility path's pooled scratch.  Unlike cudaThreadExit
  * (MDSystem.cu:294-297) it does not reset the device, so other live arrays and
  * contexts stay valid. */

@@ -35,7 +35,8 @@ HIPFLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-SOURCES = ["lj_kernels.hip", "lj_force_n3.hip", "lj_batch.hip", "lj_adjoint.hip", "lj_fp64.hip", "lj_host.cpp",
+SOURCES = ["lj_kernels.hip", "lj_force_n3.hip", "lj_batch.hip", "lj_adjoint.hip", "lj_fp64.hip",
+           "lj_noalloc.hip", "lj_host.cpp",
            "lj_api.cpp", "lj_batch_api.cpp", "lj_adjoint_api.cpp"]
 
 

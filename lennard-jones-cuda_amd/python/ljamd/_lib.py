@@ -72,6 +72,7 @@ SIGNATURES = [
     ("unregisterGLBufferObject", None, [C.c_uint]),
     ("threadSync", None, []),
     ("calculateNForces", None, [_vp, _vp, _vp, _i, _f, _i, _vp, _f, _i, _i]),
+    ("calculateNForcesNoAlloc", _i, [_vp, _vp, _vp, _i, _f, _i, _vp, _f, _i, _i]),
     ("threadExit", None, []),
     ("lj_config_init", None, [C.POINTER(LJConfig)]),
     ("lj_create", _i, [C.POINTER(LJConfig), C.POINTER(_vp)]),
