@@ -249,6 +249,77 @@ int lj_time_force_kernel(lj_ctx* ctx, int reps, double* us_per_launch);
  * automatic), 0 for the sliced kernel; nslots = partial sums per particle */
 int lj_force_plan_info(lj_ctx* ctx, int* pair_once, int* nslots);
 
+/* ---- fluctuation observables: particle counts of sub-volumes and velocity
+ * windows, counted and accumulated on the device ---- */
+enum {
+  LJ_FL_SLAB_X = 0, /* x / L below alpha      run-fluctuations-aux.h:223-234 (types 0-2) */
+  LJ_FL_SLAB_Y = 1,
+  LJ_FL_SLAB_Z = 2,
+  LJ_FL_CUBE = 3,   /* centred open cube of volume fraction alpha              :209-222 */
+  LJ_FL_VEL_X = 4,  /* |v_x| / vmax below alpha                                 :246-278 */
+  LJ_FL_VEL_Y = 5,
+  LJ_FL_VEL_Z = 6,
+  LJ_FL_CUBE_CLOSED = 7 /* closed centred cube, MDSystem::Fluctuations MDSystem.cpp:708-719 */
+};
+/* One counter.  single = 0: the reference's ladder of windows alpha = step,
+ * 2 step, ... accumulated in double below 1 - 1e-9 (slabs, cubes;
+ * GetNSubsystemBatch, run-fluctuations-aux.h:191-241) or 1 + 1e-9 (velocity
+ * windows with vcut_max = vmax; GetNsubVzBatch, :246-278), one entry per
+ * window with the cumulative count.  single = 1: one window, alpha the volume
+ * fraction (GetNSubsystem :128-160; MDSystem::Fluctuations) or, for velocity
+ * windows, |v| < vmax (GetNsubVz :165-186).  Counts use the reference's double
+ * arithmetic on the float state and match it exactly. */
+typedef struct lj_fl_ladder {
+  int type;     /* LJ_FL_* */
+  int single;   /* 0: ladder of step alpha; 1: one window */
+  double alpha; /* ladder step, or the volume fraction of one window */
+  double vmax;  /* velocity types: vcut_max (ladder) or vcut (one window) */
+} lj_fl_ladder;
+/* statistics of an entry, LJ_FL_NSTAT doubles: sum n^0 .. sum n^4 (as the
+ * moment sums of NumberStatistics), sum over samples t >= 2 of n_t n_(t-1),
+ * the last n, the number of samples (TimeAverage, time-average-aux.h:27-48) */
+#define LJ_FL_NSTAT 8
+/* Sets the counters of the context (n = 0 removes them) and clears their
+ * statistics.  At most 64 ladders and 4096 entries. */
+int lj_fl_configure(lj_ctx* ctx, const lj_fl_ladder* ladders, int n);
+/* entries of the configured counters (the ladders' entries in order) */
+int lj_fl_num_entries(lj_ctx* ctx, int* entries);
+/* per entry: its alpha (ladders: the accumulated step; one window: the
+ * fraction, or vcut for a velocity window) */
+int lj_fl_values(lj_ctx* ctx, double* values);
+/* counts the current state and adds the counts to the statistics */
+int lj_fl_sample(lj_ctx* ctx);
+/* the last sample's cumulative counts, per entry */
+int lj_fl_counts(lj_ctx* ctx, int64_t* counts);
+/* accumulated statistics, LJ_FL_NSTAT per entry */
+int lj_fl_stats(lj_ctx* ctx, double* stats);
+int lj_fl_reset(lj_ctx* ctx);
+
+/* Sampling inside the time step (captured into the step graphs).
+ *   rdf = LJ_SAMPLE_RDF_LAST: every step bins the positions its force pass
+ *         sees, replacing the previous step's histogram, as the reference
+ *         refills NdNdr2 in every CalculateForces (MDSystem.cpp:254,281-285);
+ *   rdf = LJ_SAMPLE_RDF_SUM: the steps' histograms are added up;
+ *   rdf_rule, rdf_nbins, rdf_dr2, rdf_rmax as lj_rdf (FP32 modes only);
+ *   fluctuations = 1: lj_fl_sample after every step, on the state the step
+ *         leaves (what a caller reads after Integrate).
+ * Steps that sample run without fusing the next step's drift into the end of
+ * the step, so the sampled state is the step's final one. */
+enum { LJ_SAMPLE_RDF_OFF = 0, LJ_SAMPLE_RDF_LAST = 1, LJ_SAMPLE_RDF_SUM = 2 };
+typedef struct lj_step_sampling {
+  int rdf;
+  int rdf_rule;
+  int rdf_nbins;
+  float rdf_dr2;
+  double rdf_rmax;
+  int fluctuations;
+} lj_step_sampling;
+int lj_set_step_sampling(lj_ctx* ctx, const lj_step_sampling* s);
+/* the step-sampled RDF histogram (uint64[rdf_nbins]) and the number of steps
+ * it holds (1 for LJ_SAMPLE_RDF_LAST once a step has run); clear != 0 zeroes
+ * it afterwards */
+int lj_step_rdf_counts(lj_ctx* ctx, uint64_t* counts, int64_t* nsamples, int clear);
+
 /* ------------------------------------------------------------------ */
 /* (3) device-pointer API                                              */
 /* ------------------------------------------------------------------ */
@@ -358,6 +429,18 @@ int lj_batch_time_force(lj_batch* b, int reps, double* us);
 /* Times every candidate tiling on this batch (reps force passes each), keeps
  * the fastest; cached in the file named by LJ_AUTOTUNE_CACHE when set. */
 int lj_batch_autotune(lj_batch* b, int reps, lj_batch_tuning* best, double* best_us);
+
+/* Fluctuation counters of every box (the single-box lj_fl_* for each box with
+ * its own side L): E entries per box, outputs box-major (nboxes x E). */
+int lj_batch_fl_configure(lj_batch* b, const lj_fl_ladder* ladders, int n);
+int lj_batch_fl_num_entries(lj_batch* b, int* entries);
+int lj_batch_fl_sample(lj_batch* b);
+int lj_batch_fl_counts(lj_batch* b, int64_t* counts /* nboxes * E */);
+int lj_batch_fl_stats(lj_batch* b, double* stats /* nboxes * E * LJ_FL_NSTAT */);
+int lj_batch_fl_reset(lj_batch* b);
+/* fluctuations = 1: the counters sample every box after every step of
+ * lj_batch_integrate (inside the captured step) */
+int lj_batch_set_step_sampling(lj_batch* b, int fluctuations);
 
 #ifdef __cplusplus
 }

@@ -15,21 +15,26 @@
  *     from the clock-seeded MTRand (MersenneTwister.h:262);
  *   - setArray writes the device state, which here is authoritative (on the
  *     reference's GPU branch the next CalculateForces overwrites it from h_Pos);
- *   - NdNdr2 is refreshed by CalculateForces() and UpdateRDF(), not by every
- *     step of Integrate() (an extra O(N^2) pass per step); RDF() and getvelo()
- *     return their curves as two vectors instead of a SplineFunction;
+ *   - NdNdr2 is refreshed by CalculateForces() and, as in the reference,
+ *     by every step of Integrate(): each step bins the positions its force
+ *     pass sees (an O(N^2) pass per step, on the device).  setRDFEveryStep(false)
+ *     drops that pass; NdNdr2 then changes only on CalculateForces() and
+ *     UpdateRDF().  RDF() and getvelo() return their curves as two vectors
+ *     instead of a SplineFunction;
  *   - there is no CPU branch: setHardwareMode(false) throws.  The reference's
  *     own class linked against libljhip.so keeps its CPU/GPU switch intact;
  *   - at N >= 3072 (FP32) h_Force[4 i + 3] is 0 after Integrate (the step sums
  *     the potential energy per workgroup; V, P are unaffected) and is
  *     refreshed by CalculateForces();
  *   - every failure throws std::runtime_error with lj_last_error().
- *   Fluctuations (MDSystem.cpp:708-730, unused there) is not provided.
+ *   Fluctuations (MDSystem.cpp:708-730) counts the sub-volume on the device
+ *   (lj_fl_*) and keeps the reference's moment sums and printed line.
  */
 #pragma once
 
 #include <cmath>
 #include <cstdint>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -81,6 +86,8 @@ class MDSystem {
   int veloIters = 0;
   double av_U_tot = 0, av_T_tot = 0, av_p_tot = 0;
   int av_iters = 0;
+  double momN = 0, momN2 = 0, momN3 = 0, momN4 = 0;  // MDSystem.h:94-95 (Fluctuations)
+  int momiters = 0;
 
   explicit MDSystem(const MDSystemConfiguration& config = MDSystemConfiguration())
   {
@@ -116,6 +123,12 @@ class MDSystem {
     check(lj_device_arrays(ctx_, &d_Pos, &d_Vel, &d_Force));
     rdf_dr2 = lj_rdf_dr2_default(config.N);
     NdNdr2.assign(256, 0);
+    momN = momN2 = momN3 = momN4 = 0.;  // MDSystem.cpp:109-110
+    momiters = 0;
+    fl_fraction_ = -1.;
+    // the step's RDF pass runs in the FP32 modes
+    step_rdf_on_ = false;
+    if (rdf_every_step_ && config.precision != LJ_PREC_FP64) enable_step_rdf(true);
     seed_ = config.seed;
     SampleInitialConditions();
     CalculateForces();
@@ -211,7 +224,46 @@ class MDSystem {
   {
     check(lj_integrate(ctx_, dt, nsteps, nullptr));
     pull_fields();
+    if (step_rdf_on_ && nsteps > 0) pull_step_rdf();
     if (sync_) pull_arrays();
+  }
+
+  // the RDF pass inside every step (default on, as the reference fills
+  // NdNdr2 in every CalculateForces, MDSystem.cpp:254,281-285)
+  void setRDFEveryStep(bool on)
+  {
+    rdf_every_step_ = on;
+    enable_step_rdf(on && m_config.precision != LJ_PREC_FP64);
+  }
+
+  // MDSystem.cpp:708-730: particles in the closed centred cube of volume
+  // fraction `fraction` of the current positions, added to the moment sums,
+  // and the reference's progress line on stdout
+  void Fluctuations(double fraction)
+  {
+    if (fraction != fl_fraction_) {
+      lj_fl_ladder l;
+      l.type = LJ_FL_CUBE_CLOSED;
+      l.single = 1;
+      l.alpha = fraction;
+      l.vmax = 0.;
+      check(lj_fl_configure(ctx_, &l, 1));
+      fl_fraction_ = fraction;
+    }
+    check(lj_fl_sample(ctx_));
+    int64_t c = 0;
+    check(lj_fl_counts(ctx_, &c));
+    const long long tN = (long long)c;
+    const int N = m_config.N;
+    momN += tN;
+    momN2 += tN * tN;
+    momN3 += tN * tN * tN;
+    momN4 += tN * tN * tN * tN;
+    momiters++;
+    std::cout << "Iteration: " << momiters << "\t" << "<N> = " << momN / momiters << "\t"
+              << "w[N] = " << (momN2 / momiters - (momN / momiters) * (momN / momiters)) / (momN / momiters)
+              << "\t" << "EV: " << (1. - 2. * pi() / 3. * N / L / L / L) * (1. - 2. * pi() / 3. * N / L / L / L)
+              << "\t" << "Binom: " << 1. - fraction << "\n";
   }
 
   // MDSystem.cpp:586-630 (type 0 positions, 1 velocities; 2 forces added)
@@ -333,7 +385,30 @@ class MDSystem {
   }
   lj_ctx* ctx_ = nullptr;
   bool sync_ = true;
+  bool rdf_every_step_ = true;
+  bool step_rdf_on_ = false;
+  double fl_fraction_ = -1.;  // fraction the device counter is set up for
   uint64_t seed_ = 1;
+
+  void enable_step_rdf(bool on)
+  {
+    lj_step_sampling smp;
+    smp.rdf = on ? LJ_SAMPLE_RDF_LAST : LJ_SAMPLE_RDF_OFF;
+    smp.rdf_rule = LJ_RDF_REF_GPU;
+    smp.rdf_nbins = (int)NdNdr2.size();
+    smp.rdf_dr2 = rdf_dr2;
+    smp.rdf_rmax = 0.;
+    smp.fluctuations = 0;
+    check(lj_set_step_sampling(ctx_, &smp));
+    step_rdf_on_ = on;
+  }
+
+  void pull_step_rdf()
+  {
+    std::vector<uint64_t> c(NdNdr2.size());
+    check(lj_step_rdf_counts(ctx_, c.data(), nullptr, 0));
+    for (size_t k = 0; k < c.size(); ++k) NdNdr2[k] = (int)c[k];
+  }
 
   static double pi() { return 3.141592653589793238462643; }  // MDSystem.cpp:7
 

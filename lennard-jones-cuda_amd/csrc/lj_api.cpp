@@ -150,6 +150,8 @@ int enqueue_step64(lj_ctx* c, hipStream_t st, double dt)
     LJ_HIP(launch_tvn64_b(st, a));
   }
   LJ_HIP(launch_finalize_step(st, step_args(c, dt)));
+  // the float mirrors the step kernels wrote are the sampled state
+  if (c->smp_fl) LJ_HIP(launch_fl_sample(st, c->fl, c->pos, c->vel, c->cfg.N, nullptr, nullptr, 0));
   return LJ_OK;
 }
 
@@ -181,6 +183,12 @@ int enqueue_step_part(lj_ctx* c, hipStream_t st, double dt, bool first, bool las
   // the drift kernel already wrote the fixed-point copy for periodic boxes
   double* ew = const_cast<double*>(a.ew);
   LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false, ew));
+  if (c->smp_rdf != LJ_SAMPLE_RDF_OFF) {
+    // the positions the force pass saw (drifted, before the boundary
+    // conditions), as CalculateForces bins them inside Integrate
+    if (c->smp_rdf == LJ_SAMPLE_RDF_LAST) LJ_HIP(launch_fill_u64(st, c->rdf_step, c->smp_args.nbins));
+    LJ_HIP(launch_rdf(st, c->smp_args));
+  }
   const StepArgs prev = step_args(c, dt, b ^ 1);
   const StepArgs* fin = fused_fin && !first ? &prev : nullptr;
   if (c->cfg.canonical == LJ_EVN) {
@@ -190,6 +198,7 @@ int enqueue_step_part(lj_ctx* c, hipStream_t st, double dt, bool first, bool las
     LJ_HIP(launch_tvn_b(st, a, !last));
   }
   if (last || !fused_fin) LJ_HIP(launch_finalize_step(st, a));
+  if (c->smp_fl) LJ_HIP(launch_fl_sample(st, c->fl, c->pos, c->vel, c->cfg.N, nullptr, nullptr, 0));
   return LJ_OK;
 }
 
@@ -219,8 +228,11 @@ int capture_steps(lj_ctx* c, double dt, int steps, hipGraph_t* graph, hipGraphEx
   hipStream_t st = c->stream;
   LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int rc = LJ_OK;
+  // steps that sample end with their own final state (no drift of the next
+  // step fused into them)
+  const bool alone = c->sampling();
   for (int k = 0; k < steps && rc == LJ_OK; ++k)
-    rc = enqueue_step_part(c, st, dt, k == 0, k == steps - 1, k & 1);
+    rc = enqueue_step_part(c, st, dt, alone || k == 0, alone || k == steps - 1, k & 1);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(st, &g);
   if (rc != LJ_OK) {
@@ -356,6 +368,9 @@ int lj_destroy(lj_ctx* c)
   dev_free(c->tfd);
   dev_free(c->part64);
   dev_free(c->apart64);
+  fl_free(c->fl);
+  dev_free(c->rdf_step);
+  dev_free(c->step_tab);
   if (c->e0) (void)hipEventDestroy(c->e0);
   if (c->e1) (void)hipEventDestroy(c->e1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -596,6 +611,7 @@ int lj_integrate(lj_ctx* c, double dt, int nsteps, double* hist)
   hipStream_t st = c->stream;
   LJ_HIP(launch_set_hist_base(st, c->obs, c->hist_base));
   LJ_TRY(launch_steps(c, nsteps));
+  c->rdf_step_n = c->smp_rdf == LJ_SAMPLE_RDF_LAST ? 1 : c->rdf_step_n + nsteps;
   if (hist)
     LJ_HIP(hipMemcpyAsync(hist, c->hist, sizeof(double) * 5 * nsteps, hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
@@ -611,6 +627,7 @@ int lj_integrate_async(lj_ctx* c, double dt, int nsteps)
   LJ_TRY(ensure_graph(c, dt));
   LJ_HIP(launch_set_hist_base(c->stream, c->obs, c->hist_base));
   LJ_TRY(launch_steps(c, nsteps));
+  c->rdf_step_n = c->smp_rdf == LJ_SAMPLE_RDF_LAST ? 1 : c->rdf_step_n + nsteps;
   return LJ_OK;
 }
 

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "lj_fluct.h"
 #include "lj_internal.h"
 #include "lj_util.h"
 
@@ -53,6 +54,10 @@ struct lj_batch {
   hipGraphExec_t gexec_run = nullptr;
   double g_dt = 0.;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // fluctuation counters of every box (lj_batch_fl_*), sampled after every
+  // step when smp_fl is set
+  FlSampler fl;
+  int smp_fl = 0;
 };
 
 namespace {
@@ -228,6 +233,8 @@ int enqueue_batch_step(lj_batch* b, hipStream_t st, double dt, bool first = true
     LJ_HIP(launch_batch_tvn_b(st, a, !last));
   }
   LJ_HIP(launch_batch_finalize(st, a, 1));
+  if (b->smp_fl)
+    LJ_HIP(launch_fl_sample(st, b->fl, b->pos, b->vel, 0, b->box, b->pblock, b->nblocks));
   return LJ_OK;
 }
 
@@ -236,7 +243,11 @@ int capture_batch_steps(lj_batch* b, double dt, int steps, hipGraph_t* graph, hi
   hipStream_t st = b->stream;
   LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int rc = LJ_OK;
-  for (int k = 0; k < steps && rc == LJ_OK; ++k) rc = enqueue_batch_step(b, st, dt, k == 0, k == steps - 1);
+  // steps that sample end with their own final state (no drift of the next
+  // step fused into their kick kernels)
+  const bool alone = b->smp_fl != 0;
+  for (int k = 0; k < steps && rc == LJ_OK; ++k)
+    rc = enqueue_batch_step(b, st, dt, alone || k == 0, alone || k == steps - 1);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(st, &g);
   if (rc != LJ_OK) {
@@ -375,6 +386,7 @@ int lj_batch_destroy(lj_batch* b)
   dev_free(b->items);
   dev_free(b->ew);
   dev_free(b->counter);
+  fl_free(b->fl);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -575,6 +587,74 @@ int lj_batch_integrate(lj_batch* b, double dt, int nsteps)
   for (; k + kGraphSteps <= nsteps; k += kGraphSteps) LJ_HIP(hipGraphLaunch(b->gexec_run, st));
   for (; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(b->gexec, st));
   LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_batch_fl_configure(lj_batch* b, const lj_fl_ladder* ladders, int n)
+{
+  LJ_TRY(bsetdev(b));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  if (b->smp_fl) bdrop_graph(b);
+  LJ_TRY(fl_configure(b->fl, b->stream, b->nboxes, b->L.data(), ladders, n));
+  if (!b->fl.active()) b->smp_fl = 0;
+  return LJ_OK;
+}
+
+int lj_batch_fl_num_entries(lj_batch* b, int* entries)
+{
+  if (!b || !entries) return fail(LJ_ERR_ARG, "lj_batch_fl_num_entries: null argument");
+  *entries = b->fl.E;
+  return LJ_OK;
+}
+
+int lj_batch_fl_sample(lj_batch* b)
+{
+  LJ_TRY(bsetdev(b));
+  if (!b->fl.active()) return fail(LJ_ERR_STATE, "lj_batch_fl_sample: no counters configured");
+  LJ_HIP(launch_fl_sample(b->stream, b->fl, b->pos, b->vel, 0, b->box, b->pblock, b->nblocks));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_fl_counts(lj_batch* b, int64_t* counts)
+{
+  LJ_TRY(bsetdev(b));
+  if (!counts) return fail(LJ_ERR_ARG, "lj_batch_fl_counts: null output");
+  if (!b->fl.active()) return fail(LJ_ERR_STATE, "lj_batch_fl_counts: no counters configured");
+  LJ_HIP(hipMemcpyAsync(counts, b->fl.last, sizeof(int64_t) * b->nboxes * b->fl.E,
+                        hipMemcpyDeviceToHost, b->stream));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_fl_stats(lj_batch* b, double* stats)
+{
+  LJ_TRY(bsetdev(b));
+  if (!stats) return fail(LJ_ERR_ARG, "lj_batch_fl_stats: null output");
+  if (!b->fl.active()) return fail(LJ_ERR_STATE, "lj_batch_fl_stats: no counters configured");
+  LJ_HIP(hipMemcpyAsync(stats, b->fl.stats, sizeof(double) * kFlStat * b->nboxes * b->fl.E,
+                        hipMemcpyDeviceToHost, b->stream));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_fl_reset(lj_batch* b)
+{
+  LJ_TRY(bsetdev(b));
+  if (!b->fl.active()) return fail(LJ_ERR_STATE, "lj_batch_fl_reset: no counters configured");
+  LJ_HIP(launch_fl_reset(b->stream, b->fl));
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  return LJ_OK;
+}
+
+int lj_batch_set_step_sampling(lj_batch* b, int fluctuations)
+{
+  LJ_TRY(bsetdev(b));
+  if (fluctuations && !b->fl.active())
+    return fail(LJ_ERR_STATE, "lj_batch_set_step_sampling: configure the counters first");
+  LJ_HIP(hipStreamSynchronize(b->stream));
+  bdrop_graph(b);
+  b->smp_fl = fluctuations ? 1 : 0;
   return LJ_OK;
 }
 

@@ -2,6 +2,7 @@
 // shared by the C-ABI implementation files.
 #pragma once
 
+#include "lj_fluct.h"
 #include "lj_internal.h"
 #include "lj_util.h"
 
@@ -56,6 +57,18 @@ struct lj_ctx {
   double4* tfd = nullptr;
   double4* part64 = nullptr;
   double* apart64 = nullptr;
+  // fluctuation counters (lj_fl_*)
+  lj::FlSampler fl;
+  // sampling inside the step graphs (lj_set_step_sampling)
+  int smp_rdf = LJ_SAMPLE_RDF_OFF;
+  int smp_fl = 0;
+  lj::RdfArgs smp_args{};              // RDF pass of each step (hist = rdf_step)
+  unsigned long long* rdf_step = nullptr;
+  int rdf_step_cap = 0;
+  float* step_tab = nullptr;           // bin thresholds of the step's RDF pass
+  int step_tab_cap = 0;
+  int64_t rdf_step_n = 0;              // steps the step histogram holds
+  bool sampling() const { return smp_rdf != LJ_SAMPLE_RDF_OFF || smp_fl != 0; }
 };
 
 namespace lj {

@@ -40,6 +40,21 @@ class LJBatchTuning(C.Structure):
                 ("pair_once", C.c_int)]
 
 
+FL_SLAB_X, FL_SLAB_Y, FL_SLAB_Z, FL_CUBE, FL_VEL_X, FL_VEL_Y, FL_VEL_Z, FL_CUBE_CLOSED = range(8)
+FL_NSTAT = 8
+SAMPLE_RDF_OFF, SAMPLE_RDF_LAST, SAMPLE_RDF_SUM = 0, 1, 2
+
+
+class LJFlLadder(C.Structure):
+    _fields_ = [("type", C.c_int), ("single", C.c_int), ("alpha", C.c_double),
+                ("vmax", C.c_double)]
+
+
+class LJStepSampling(C.Structure):
+    _fields_ = [("rdf", C.c_int), ("rdf_rule", C.c_int), ("rdf_nbins", C.c_int),
+                ("rdf_dr2", C.c_float), ("rdf_rmax", C.c_double), ("fluctuations", C.c_int)]
+
+
 BATCH_NSTAT = 13
 BATCH_STAT_NAMES = ("samples", "U", "U2", "V", "V2", "K", "K2", "P", "P2", "n1", "n2", "n3", "n4")
 
@@ -124,6 +139,22 @@ SIGNATURES = [
     ("lj_batch_get_tuning", _i, [_vp, C.POINTER(LJBatchTuning)]),
     ("lj_batch_time_force", _i, [_vp, _i, C.POINTER(_d)]),
     ("lj_batch_autotune", _i, [_vp, _i, C.POINTER(LJBatchTuning), C.POINTER(_d)]),
+    ("lj_fl_configure", _i, [_vp, _vp, _i]),
+    ("lj_fl_num_entries", _i, [_vp, C.POINTER(_i)]),
+    ("lj_fl_values", _i, [_vp, _vp]),
+    ("lj_fl_sample", _i, [_vp]),
+    ("lj_fl_counts", _i, [_vp, _vp]),
+    ("lj_fl_stats", _i, [_vp, _vp]),
+    ("lj_fl_reset", _i, [_vp]),
+    ("lj_set_step_sampling", _i, [_vp, C.POINTER(LJStepSampling)]),
+    ("lj_step_rdf_counts", _i, [_vp, _vp, C.POINTER(C.c_int64), _i]),
+    ("lj_batch_fl_configure", _i, [_vp, _vp, _i]),
+    ("lj_batch_fl_num_entries", _i, [_vp, C.POINTER(_i)]),
+    ("lj_batch_fl_sample", _i, [_vp]),
+    ("lj_batch_fl_counts", _i, [_vp, _vp]),
+    ("lj_batch_fl_stats", _i, [_vp, _vp]),
+    ("lj_batch_fl_reset", _i, [_vp]),
+    ("lj_batch_set_step_sampling", _i, [_vp, _i]),
 ]
 
 _lib = None

@@ -112,3 +112,33 @@ class LJBatch:
         us = C.c_double()
         check(lib().lj_batch_autotune(self.h, int(reps), C.byref(t), C.byref(us)))
         return (t.ipl, t.jchunk, t.blocks_per_cu, t.pair_once), us.value
+
+    # ---- fluctuation counters of every box (lj_batch_fl_*) ----
+
+    def fl_configure(self, ladders):
+        from .system import _fl_ladders
+        check(lib().lj_batch_fl_configure(self.h, _fl_ladders(ladders), len(ladders)))
+
+    def fl_num_entries(self):
+        e = C.c_int()
+        check(lib().lj_batch_fl_num_entries(self.h, C.byref(e)))
+        return e.value
+
+    def fl_sample(self):
+        check(lib().lj_batch_fl_sample(self.h))
+
+    def fl_counts(self):
+        out = np.zeros((self.nboxes, self.fl_num_entries()), np.int64)
+        check(lib().lj_batch_fl_counts(self.h, ptr(out)))
+        return out
+
+    def fl_stats(self):
+        out = np.zeros((self.nboxes, self.fl_num_entries(), _L.FL_NSTAT), np.float64)
+        check(lib().lj_batch_fl_stats(self.h, ptr(out)))
+        return out
+
+    def fl_reset(self):
+        check(lib().lj_batch_fl_reset(self.h))
+
+    def set_step_sampling(self, fluctuations):
+        check(lib().lj_batch_set_step_sampling(self.h, int(bool(fluctuations))))

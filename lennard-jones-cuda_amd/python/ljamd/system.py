@@ -181,6 +181,61 @@ class LJSystem:
                                      C.c_void_p(stream) if stream else None))
 
 
+    # ---- fluctuation counters (lj_fl_*) and sampling inside the step ----
+
+    def fl_configure(self, ladders):
+        """ladders: (type, single, alpha, vmax) tuples (LJ_FL_* types)."""
+        arr = _fl_ladders(ladders)
+        check(lib().lj_fl_configure(self.h, arr, len(ladders)))
+
+    def fl_num_entries(self):
+        e = C.c_int()
+        check(lib().lj_fl_num_entries(self.h, C.byref(e)))
+        return e.value
+
+    def fl_values(self):
+        out = np.zeros(self.fl_num_entries(), np.float64)
+        check(lib().lj_fl_values(self.h, ptr(out)))
+        return out
+
+    def fl_sample(self):
+        check(lib().lj_fl_sample(self.h))
+
+    def fl_counts(self):
+        out = np.zeros(self.fl_num_entries(), np.int64)
+        check(lib().lj_fl_counts(self.h, ptr(out)))
+        return out
+
+    def fl_stats(self):
+        out = np.zeros((self.fl_num_entries(), _L.FL_NSTAT), np.float64)
+        check(lib().lj_fl_stats(self.h, ptr(out)))
+        return out
+
+    def fl_reset(self):
+        check(lib().lj_fl_reset(self.h))
+
+    def set_step_sampling(self, rdf=_L.SAMPLE_RDF_OFF, rdf_rule=_L.RDF_REF_CPU, rdf_nbins=256,
+                          rdf_dr2=0.0, rdf_rmax=0.0, fluctuations=False):
+        smp = _L.LJStepSampling(int(rdf), int(rdf_rule), int(rdf_nbins), float(rdf_dr2),
+                                float(rdf_rmax), int(bool(fluctuations)))
+        self._rdf_nbins = int(rdf_nbins)
+        check(lib().lj_set_step_sampling(self.h, C.byref(smp)))
+
+    def step_rdf_counts(self, clear=False):
+        """(uint64[nbins] step-sampled RDF counts, steps they hold)."""
+        out = np.zeros(getattr(self, "_rdf_nbins", 256), np.uint64)
+        n = C.c_int64()
+        check(lib().lj_step_rdf_counts(self.h, ptr(out), C.byref(n), int(bool(clear))))
+        return out, n.value
+
+
+def _fl_ladders(ladders):
+    arr = (_L.LJFlLadder * max(len(ladders), 1))()
+    for k, (typ, single, alpha, vmax) in enumerate(ladders):
+        arr[k] = _L.LJFlLadder(int(typ), int(single), float(alpha), float(vmax))
+    return arr
+
+
 # ---- the compatibility set (reference MDSystem.cu:155-299) ----
 
 class CompatArrays:

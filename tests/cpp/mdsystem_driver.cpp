@@ -4,6 +4,11 @@
 // test compares against the oracle:
 //   init U T K V P
 //   step k t U T K V P      (one line per step)
+//   rdfstep <256 counts>     NdNdr2 as the last Integrate left it (every step bins)
+//   posstep <4N floats>      h_Pos after the last step
+//   flucs momN momN2 momN3 momN4 momiters   after Fluctuations(0.3) on that
+//                            state, Integrate(dt), Fluctuations(0.3) (each call
+//                            also prints the reference's "Iteration:" line)
 //   rdf <256 counts>
 //   pos <4N floats>          (h_Pos after the last step)
 //   vel0 / velo0             velocities, then initvelo(3.0, 0.05) of them
@@ -14,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
+#include <iostream>
 
 #include "lj_mdsystem.hpp"
 
@@ -53,6 +59,19 @@ int main(int argc, char** argv)
       s.Integrate(dt);
       std::printf("step %d %.17g %.17g %.17g %.17g %.17g %.17g\n", k, s.t, s.U, s.T, s.K, s.V, s.P);
     }
+    std::printf("rdfstep");
+    for (int c : s.NdNdr2) std::printf(" %d", c);
+    std::printf("\n");
+    print_floats("posstep", s.h_Pos, 4 * cfg.N);
+    std::fflush(stdout);
+    s.Fluctuations(0.3);
+    std::cout.flush();
+    s.Integrate(dt);
+    print_floats("posstep2", s.h_Pos, 4 * cfg.N);
+    std::fflush(stdout);
+    s.Fluctuations(0.3);
+    std::cout.flush();
+    std::printf("flucs %.17g %.17g %.17g %.17g %d\n", s.momN, s.momN2, s.momN3, s.momN4, s.momiters);
     s.CalculateForces();
     std::printf("rdf");
     for (int c : s.NdNdr2) std::printf(" %d", c);

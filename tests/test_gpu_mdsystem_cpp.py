@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from oracle_lib import OracleSystem, lattice_state, rdf, velocity_histogram
+from oracle_lib import ORC, OracleSystem, lattice_state, rdf, velocity_histogram
 
 pytestmark = pytest.mark.gpu
 
@@ -26,8 +26,11 @@ def _run(*args):
     r = subprocess.run([DRIVER, *map(str, args)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    out = {"step": []}
+    out = {"step": [], "iteration_lines": []}
     for line in r.stdout.splitlines():
+        if line.startswith("Iteration:"):  # MDSystem::Fluctuations' progress line
+            out["iteration_lines"].append(line)
+            continue
         key, *vals = line.split()
         if key == "step":
             out["step"].append([float(v) for v in vals])
@@ -60,6 +63,12 @@ def test_mdsystem_class_vs_reference_cpu():
     counts = rdf(p, L, True, 1, 256, dr2=o.s.rdf_dr2)
     assert np.array_equal(np.array(out["rdf"], np.int64), counts)
     assert out["cpu_mode_rejected"] == [1.0]
+    # NdNdr2 after the loop was binned inside the last step, before the
+    # periodic wrap: against the wrapped positions at most a few pairs move
+    rs = np.array(out["rdfstep"], np.int64)
+    re = rdf(np.array(out["posstep"], np.float32), L, True, 1, 256, dr2=o.s.rdf_dr2)
+    assert rs.sum() == re.sum() and np.abs(rs - re).sum() <= 8
+    _flucs_check(out, L, N)
     # speed distribution: initvelo(3.0, 0.05), then updatevelo after 10 steps,
     # against the oracle's counts with the reference's normalisation
     # (MDSystem.cpp:651-694), bit for bit
@@ -79,3 +88,30 @@ def test_mdsystem_class_vs_reference_cpu():
     v = 0.02 * np.arange(m.size)
     assert abs(np.trapezoid(m, v) - 1.0) <= 1e-4
     assert abs(np.trapezoid(v * m, v) - np.sqrt(8 * T / np.pi)) <= 1e-3
+
+
+def _flucs_check(out, L, N):
+    """Fluctuations(0.3) on posstep and on posstep2: the closed-cube counts of
+    the oracle (MDSystem.cpp:708-719) in the reference's moment sums."""
+    n1 = ORC.orc_fl_closed_cube(N, np.array(out["posstep"], np.float32), L, 0.3)
+    n2 = ORC.orc_fl_closed_cube(N, np.array(out["posstep2"], np.float32), L, 0.3)
+    assert out["flucs"] == [float(n1 + n2), float(n1 * n1 + n2 * n2), float(n1 ** 3 + n2 ** 3),
+                            float(n1 ** 4 + n2 ** 4), 2.0]
+    lines = out["iteration_lines"]
+    assert len(lines) == 2 and lines[0].startswith(f"Iteration: 1\t<N> = {n1}\t")
+    assert lines[1].endswith("Binom: 0.7")
+
+
+def test_mdsystem_class_rdf_every_step_and_fluctuations():
+    """Hard walls leave positions in place, so NdNdr2 after Integrate (binned
+    inside the last step, as the reference's CalculateForces does) equals the
+    GPU-rule binning of the positions the step left, bit for bit; Fluctuations
+    keeps the reference's moment sums of the closed-cube count."""
+    N, rho, T0, dt, steps, seed = 400, 0.05, 1.4, 0.005, 30, 5
+    print(f"\n[case] C++ MDSystem N={N} TVN hard wall {steps} steps", flush=True)
+    out = _run(N, rho, T0, 1, 1, dt, steps, seed)
+    L = ORC.orc_box_length(N, rho)
+    p = np.array(out["posstep"], np.float32)
+    exp = rdf(p, L, False, 1, 256, dr2=ORC.orc_rdf_dr2(N))
+    assert np.array_equal(np.array(out["rdfstep"], np.int64), exp)
+    _flucs_check(out, L, N)
