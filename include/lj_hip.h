@@ -415,6 +415,9 @@ int lj_batch_get_array(lj_batch* b, int box, int type, float* host4N);
 int lj_batch_compute_forces(lj_batch* b);
 /* nsteps of the captured batched step */
 int lj_batch_integrate(lj_batch* b, double dt, int nsteps);
+/* the same with the per-step history: hist (host, nsteps * nboxes * 5
+ * doubles, step-major) receives {U, T, K, V, P} of every box after every step */
+int lj_batch_integrate_hist(lj_batch* b, double dt, int nsteps, double* hist);
 int lj_batch_get_observables(lj_batch* b, lj_obs* out /* nboxes */);
 int lj_batch_set_subvolume(lj_batch* b, double fraction);
 int lj_batch_get_stats(lj_batch* b, double* out /* nboxes * LJ_BATCH_NSTAT */);

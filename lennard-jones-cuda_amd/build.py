@@ -5,6 +5,7 @@
 
 Outputs (in-tree, git-ignored, shipped with the tree to the GPU host):
     lennard-jones-cuda_amd/lib/libljhip.so
+    lennard-jones-cuda_amd/lib/run_isotherm, run_fluctuations (task programs)
     oracle/liboracle.so, oracle/_ref/*.so  (checkers; _ref only where the
                                             reference sources are present)
 """
@@ -82,6 +83,23 @@ def build_driver():
     return out
 
 
+TASKS = ["run_isotherm", "run_fluctuations"]
+
+
+def build_tasks():
+    """The task programs (the reference's src/tasks drivers over the C-ABI)."""
+    out = []
+    for name in TASKS:
+        src = os.path.join(PKG, "tasks", name + ".cpp")
+        exe = os.path.join(LIB, name)
+        _run(["g++", "-std=c++11", "-O2", "-Wall", "-Wextra", "-Werror", f"-I{INCLUDE}",
+              f"-I{os.path.join(PKG, 'tasks')}", src, f"-L{LIB}", "-lljhip",
+              "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"])
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 def build_checkers():
     _run(["make", "-s", "-C", ORACLE, "oracle"])
     if os.path.isdir(os.path.join(REFERENCE, "src", "library")):
@@ -91,6 +109,7 @@ def build_checkers():
 def build_all(product_only=False, verbose=False):
     t = build_product(verbose=verbose)
     build_driver()
+    build_tasks()
     if not product_only:
         build_checkers()
     return t

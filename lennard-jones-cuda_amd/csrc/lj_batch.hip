@@ -332,7 +332,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_finalize(BatchArgs a, int adva
   o->Pshear = 0.;
   if (advance) {
     o->t += a.dt;
+    const int64_t k = o->step - (a.hist_base ? a.hist_base[b] : 0);
     o->step += 1;
+    if (a.hist && k >= 0 && k < a.hist_cap) {
+      double* h = a.hist + 5 * ((size_t)k * a.nboxes + b);
+      h[0] = o->U;
+      h[1] = T;
+      h[2] = K;
+      h[3] = o->V;
+      h[4] = P;
+    }
     o->av_iters += 1;
     o->av_U_tot += o->U;
     o->av_p_tot += P;
@@ -353,6 +362,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_finalize(BatchArgs a, int adva
     st[ST_N3] += n * n * n;
     st[ST_N4] += n * n * n * n;
   }
+}
+
+__global__ void k_batch_set_hist_base(const lj_obs* obs, int64_t* base, int nboxes)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nboxes) base[i] = obs[i].step;
 }
 
 __global__ void k_batch_zero_stats(double* stats, int n, lj_obs* obs, int nboxes)
@@ -448,6 +463,13 @@ hipError_t launch_batch_reduce_forces(hipStream_t st, const BatchArgs& a)
 hipError_t launch_batch_kinetic(hipStream_t st, const BatchArgs& a)
 {
   hipLaunchKernelGGL(k_batch_kinetic, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_set_hist_base(hipStream_t st, const BatchArgs& a, int64_t* hist_base)
+{
+  hipLaunchKernelGGL(k_batch_set_hist_base, dim3((a.nboxes + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     st, a.obs, hist_base, a.nboxes);
   return hipGetLastError();
 }
 

@@ -58,6 +58,10 @@ struct lj_batch {
   // step when smp_fl is set
   FlSampler fl;
   int smp_fl = 0;
+  // per-step history of lj_batch_integrate_hist (hist_cap steps of nboxes rows)
+  double* hist = nullptr;
+  int64_t* hist_base = nullptr;
+  int hist_cap = 0;
 };
 
 namespace {
@@ -113,6 +117,9 @@ BatchArgs bargs(lj_batch* b, double dt, bool lean = false)
   if (a.pair_once) a.persistent_grid = 0;
   a.lean = lean && a.pair_once ? 1 : 0;
   a.ew = b->ew;
+  a.hist = b->hist;
+  a.hist_base = b->hist_base;
+  a.hist_cap = b->hist_cap;
   return a;
 }
 
@@ -387,6 +394,8 @@ int lj_batch_destroy(lj_batch* b)
   dev_free(b->ew);
   dev_free(b->counter);
   fl_free(b->fl);
+  dev_free(b->hist);
+  dev_free(b->hist_base);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -568,10 +577,27 @@ int lj_batch_autotune(lj_batch* b, int reps, lj_batch_tuning* best_out, double* 
 
 int lj_batch_integrate(lj_batch* b, double dt, int nsteps)
 {
+  return lj_batch_integrate_hist(b, dt, nsteps, nullptr);
+}
+
+int lj_batch_integrate_hist(lj_batch* b, double dt, int nsteps, double* hist)
+{
   LJ_TRY(bsetdev(b));
   if (nsteps < 0 || !std::isfinite(dt)) return fail(LJ_ERR_ARG, "lj_batch_integrate: bad arguments");
   if (nsteps == 0) return LJ_OK;
   hipStream_t st = b->stream;
+  if (hist && nsteps > b->hist_cap) {
+    // the captured steps hold the history pointer: grow, then capture again
+    LJ_HIP(hipStreamSynchronize(st));
+    bdrop_graph(b);
+    dev_free(b->hist);
+    b->hist_cap = 0;
+    const int cap = std::max(nsteps, 256);
+    LJ_TRY(dev_alloc(&b->hist, (size_t)5 * cap * b->nboxes));
+    if (!b->hist_base) LJ_TRY(dev_alloc(&b->hist_base, b->nboxes));
+    b->hist_cap = cap;
+  }
+  if (b->hist_base) LJ_HIP(launch_batch_set_hist_base(st, bargs(b, dt), b->hist_base));
   if (!b->gexec || b->g_dt != dt) {
     bdrop_graph(b);
     LJ_HIP(hipStreamSynchronize(st));
@@ -586,6 +612,9 @@ int lj_batch_integrate(lj_batch* b, double dt, int nsteps)
   int k = 0;
   for (; k + kGraphSteps <= nsteps; k += kGraphSteps) LJ_HIP(hipGraphLaunch(b->gexec_run, st));
   for (; k < nsteps; ++k) LJ_HIP(hipGraphLaunch(b->gexec, st));
+  if (hist)
+    LJ_HIP(hipMemcpyAsync(hist, b->hist, sizeof(double) * 5 * (size_t)nsteps * b->nboxes,
+                          hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
   return LJ_OK;
 }

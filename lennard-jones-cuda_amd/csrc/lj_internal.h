@@ -230,6 +230,11 @@ struct BatchArgs {
   // each box's item totals
   int lean;
   double* ew;
+  // per-step history (may be null): row k of box b at hist[5 (k nboxes + b)]
+  // = {U, T, K, V, P} for k = obs[b].step - hist_base[b] < hist_cap
+  double* hist;
+  const int64_t* hist_base;
+  int hist_cap;
 };
 hipError_t launch_batch_drift(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_force(hipStream_t st, const BatchArgs& a, bool periodic, bool ties);
@@ -243,6 +248,8 @@ hipError_t launch_batch_prep_q(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_reduce_forces(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_kinetic(hipStream_t st, const BatchArgs& a);
 hipError_t launch_batch_zero_stats(hipStream_t st, const BatchArgs& a);
+// hist_base[b] = obs[b].step (the next step writes history row 0)
+hipError_t launch_batch_set_hist_base(hipStream_t st, const BatchArgs& a, int64_t* hist_base);
 
 // host helpers (lj_host.cpp)
 void lattice_positions(int N, double L, float* pos4);

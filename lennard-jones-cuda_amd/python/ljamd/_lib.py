@@ -131,6 +131,7 @@ SIGNATURES = [
     ("lj_batch_get_array", _i, [_vp, _i, _i, _vp]),
     ("lj_batch_compute_forces", _i, [_vp]),
     ("lj_batch_integrate", _i, [_vp, _d, _i]),
+    ("lj_batch_integrate_hist", _i, [_vp, _d, _i, _vp]),
     ("lj_batch_get_observables", _i, [_vp, _vp]),
     ("lj_batch_set_subvolume", _i, [_vp, _d]),
     ("lj_batch_get_stats", _i, [_vp, _vp]),

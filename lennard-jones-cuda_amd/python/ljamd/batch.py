@@ -113,6 +113,12 @@ class LJBatch:
         check(lib().lj_batch_autotune(self.h, int(reps), C.byref(t), C.byref(us)))
         return (t.ipl, t.jchunk, t.blocks_per_cu, t.pair_once), us.value
 
+    def integrate_hist(self, dt, nsteps):
+        """nsteps batched steps; returns {U, T, K, V, P} [nsteps, nboxes, 5]."""
+        h = np.zeros((max(int(nsteps), 1), self.nboxes, 5), np.float64)
+        check(lib().lj_batch_integrate_hist(self.h, float(dt), int(nsteps), ptr(h)))
+        return h[:nsteps]
+
     # ---- fluctuation counters of every box (lj_batch_fl_*) ----
 
     def fl_configure(self, ladders):
