@@ -21,6 +21,9 @@ from source: tests/test_oracle_pins.py::test_oracle_bitexact_vs_reference and
 the committed reference-run fixtures, tests/test_oracle_golden.py) running the
 same workload for a bounded number of steps on one host thread, as the
 reference does; the same code over 16 host threads beside it.
+precision: the FP32, compensated and FP64 modes on the precision workload
+(N = 131072, hard wall), force-pass time and E_pot / P deviation from FP64
+over the first 100 steps.
 
 Clocks: the GPU's shader clock ramps up over the first ~0.1 s of sustained
 load, so every timed leg first runs force passes for at least SETTLE_MS
@@ -231,6 +234,32 @@ def rdf_leg(reps=20):
             "ref_cpu_256_r2_bins_us": us_c, "ref_cpu_pairs_per_s": pairs / (us_c * 1e-6)}
 
 
+def precision_leg(steps=100):
+    """The precision modes on the precision workload at full size (N = 131072,
+    rho* = 0.3, T* = 1, hard wall, EVN, dt = 0.002, one seeded state): the
+    force pass of each mode (HIP events) and, against the FP64 mode, the
+    largest relative deviation of E_pot and P over the first `steps` steps."""
+    import types
+
+    import numpy as np
+    from ljamd import precision as pr
+
+    a = types.SimpleNamespace(N=131072, rho=0.3, T0=1.0, dt=0.002, steps=steps, chunk=steps,
+                              seed=5)
+    runs = {name: pr.run_mode(name, prec, a, lambda msg: None) for name, prec in pr.MODES}
+    ref = runs["fp64"]["hist"]
+    out = {"workload": "N=131072, rho*=0.3, T*=1.0, hard wall, EVN, dt*=0.002, seed 5",
+           "steps": steps}
+    for name, r in runs.items():
+        d = {"force_kernel_us": r["force_us"], "ms_per_step": r["ms_per_step"]}
+        if name != "fp64":
+            h = r["hist"]
+            d["E_pot_max_rel_vs_fp64"] = float(np.max(np.abs(h[:, 3] - ref[:, 3]) / np.abs(ref[:, 3])))
+            d["P_max_rel_vs_fp64"] = float(np.max(np.abs(h[:, 4] - ref[:, 4]) / np.abs(ref[:, 4])))
+        out[name] = d
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -384,6 +413,10 @@ def main():
                     help="particles of the --profile-force workload (default: the headline N)")
     ap.add_argument("--profile-rdf", action="store_true",
                     help="only the RDF passes of the rdf leg (for rocprofv3)")
+    ap.add_argument("--profile-precision", action="store_true",
+                    help="only the precision modes' leg")
+    ap.add_argument("--profile-adjoint", action="store_true",
+                    help="only the differentiable workload's leg (for rocprofv3)")
     ap.add_argument("--sweep-n", action="store_true",
                     help="only the TVN step and force pass over a range of N (kernel choice)")
     a = ap.parse_args()
@@ -395,6 +428,12 @@ def main():
         return
     if a.profile_rdf:
         print(json.dumps({"profile": "rdf", "N": N, **rdf_leg()}))
+        return
+    if a.profile_precision:
+        print(json.dumps({"profile": "precision", **precision_leg()}))
+        return
+    if a.profile_adjoint:
+        print(json.dumps({"profile": "adjoint", **adjoint_leg()}))
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
@@ -417,6 +456,7 @@ def main():
     batched = None if a.no_batch else batch_leg(a.batch_steps)
     adjoint = adjoint_leg()
     rdf = rdf_leg()
+    precision = precision_leg()
     cpu = None if a.no_cpu else cpu_leg()
     if cpu is not None:
         cpu["all_cores"] = cpu_parallel_leg()
@@ -475,6 +515,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "rdf": rdf,
+        "precision": precision,
         "final_T": g["obs"]["T"],
         "final_u": g["obs"]["U"] / N,
     }

@@ -125,9 +125,13 @@ enum { LJ_EVN = 0, LJ_TVN = 1 };                              /* MDSystem.h:20-2
 enum {
   LJ_RDF_REF_CPU = 0, /* reference CPU rule: r^2 as MDSystem.cpp:269-279 (double min image),
                          bin = floor(r^2/dr2) exact, bins >= nbins dropped, self by index */
-  LJ_RDF_REF_GPU = 1, /* reference GPU rule: MDSystem.cu:26-42 (float lround min image,
-                         r^2 = fma(z,z,fma(y,y,x*x)), floor(fl32(r^2/dr2)), clamp to
-                         nbins-1, pairs with r^2 <= 1e-6 skipped) */
+  LJ_RDF_REF_GPU = 1, /* reference GPU rule, bit pattern assumed (nvcc contraction),
+                         bounded by the CPU rule: MDSystem.cu:26-42 (float lround min
+                         image, r^2 = fma(z,z,fma(y,y,x*x)) -- the contraction nvcc
+                         applies by default, assumed, since the reference's GPU build
+                         cannot run here -- floor(fl32(r^2/dr2)), clamp to nbins-1,
+                         pairs with r^2 <= 1e-6 skipped).  Its counts differ from
+                         LJ_RDF_REF_CPU's only at bin edges and in the clamp bin. */
   LJ_RDF_RBINS = 2    /* r^2 as LJ_RDF_REF_CPU, nbins uniform in r on [0, rmax):
                          bin = max k with edges2[k] <= r^2, edges2[k] =
                          (float)((k*(rmax/nbins))^2) in double; r^2 >= edges2[nbins] dropped */

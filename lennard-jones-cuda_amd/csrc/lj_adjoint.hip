@@ -15,6 +15,9 @@
 //   2 phi' (r.z) r = (24 eps/s^2) (2/s^2) q^2 q3 (4 - 14 q3) (r.z) r
 //   2 psi' r = (48 eps/s^2) q q3 (3 - 12 q3) r
 //   d phi / d s = (24 eps/s^3) q q3 (24 q3 - 6)
+#include <algorithm>
+#include <cstdlib>
+
 #include "lj_adjoint.h"
 #include "lj_device.h"
 
@@ -52,21 +55,105 @@ __device__ __forceinline__ void adj_add(AdjAcc& h, const AdjAcc& t)
   h.s += t.s;
 }
 
-// K2 = 2 u^2 / s^2 (u: length of one internal position unit)
-template <bool PERIODIC, bool GUARD>
-__device__ __forceinline__ void adj_pair(const PosT<PERIODIC>& pi, const PosT<PERIODIC>& pj,
-                                         const float4& zi, const float4& zj, int i, int j,
-                                         float C1, float K2, AdjAcc& t)
+// The lane's particle i in the lane's own mirrored frame.  Periodic boxes:
+// axis a of lane i is mirrored (s_a = -1) when qi_a < 2^31, so that the pair
+// separation in that frame, s_a (qi_a - qj_a), is qj_a - qi_a there and
+// qi_a - qj_a elsewhere: one xor-add, d' = (qj ^ m_a) + o_a, with m_a = ~0,
+// o_a = qi_a + 1 (unmirrored) or m_a = 0, o_a = -qi_a (mirrored).  An exact
+// half-box separation (d' = 2^31) then converts to -2^31 in the lane's frame,
+// i.e. to -L/2 when qi > qj and +L/2 when qi < qj -- the tie rule of the force
+// pass -- without a compare and select per axis; every other separation is
+// the exact negation of the unmirrored one (integer negation and the
+// conversion's round-to-nearest-even are odd).  The adjoint differences
+// z_i - z_j are mirrored alike, s_a z_i - s_a z_j, and every product and sum
+// of the pair then carries the factor s_a exactly, so the accumulated vectors
+// are s_a times the unmirrored ones bit for bit; adj_unmirror takes it out.
+template <bool PERIODIC>
+struct AdjI;
+template <>
+struct AdjI<true> {
+  unsigned mx, my, mz, ox, oy, oz;
+  float sx, sy, sz;  // +-1
+  float zx, zy, zz;  // s_a z_i
+};
+template <>
+struct AdjI<false> {
+  float x, y, z;
+  float zx, zy, zz;
+};
+
+template <bool PERIODIC>
+__device__ __forceinline__ AdjI<PERIODIC> adj_lane(const PosT<PERIODIC>& p, const float4& zi)
 {
-  float dx, dy, dz;
-  if constexpr (PERIODIC) {  // exact half-box ties resolved as in the force pass
-    dx = mi_fixed<true>(pi.x, pj.x);
-    dy = mi_fixed<true>(pi.y, pj.y);
-    dz = mi_fixed<true>(pi.z, pj.z);
+  AdjI<PERIODIC> I;
+  if constexpr (PERIODIC) {
+    const bool fx = p.x < 0x80000000u, fy = p.y < 0x80000000u, fz = p.z < 0x80000000u;
+    I.mx = fx ? 0u : ~0u;
+    I.my = fy ? 0u : ~0u;
+    I.mz = fz ? 0u : ~0u;
+    I.ox = fx ? 0u - p.x : p.x + 1u;
+    I.oy = fy ? 0u - p.y : p.y + 1u;
+    I.oz = fz ? 0u - p.z : p.z + 1u;
+    I.sx = fx ? -1.f : 1.f;
+    I.sy = fy ? -1.f : 1.f;
+    I.sz = fz ? -1.f : 1.f;
+    I.zx = I.sx * zi.x;
+    I.zy = I.sy * zi.y;
+    I.zz = I.sz * zi.z;
   } else {
-    dx = pi.x - pj.x;
-    dy = pi.y - pj.y;
-    dz = pi.z - pj.z;
+    I.x = p.x;
+    I.y = p.y;
+    I.z = p.z;
+    I.zx = zi.x;
+    I.zy = zi.y;
+    I.zz = zi.z;
+  }
+  return I;
+}
+
+// the vector sums back in the common frame; the scalar sum s carries -1/2
+// (see adj_pair)
+template <bool PERIODIC>
+__device__ __forceinline__ void adj_unmirror(const AdjI<PERIODIC>& I, AdjAcc& t)
+{
+  if constexpr (PERIODIC) {
+    t.hx *= I.sx;
+    t.hy *= I.sy;
+    t.hz *= I.sz;
+    t.fx *= I.sx;
+    t.fy *= I.sy;
+    t.fz *= I.sz;
+    t.gx *= I.sx;
+    t.gy *= I.sy;
+    t.gz *= I.sz;
+  }
+  t.s *= -2.f;
+}
+
+// K2 = 2 u^2 / s^2 (u: length of one internal position unit).  The sum s is
+// accumulated as sum G (d.dz), G = q q3 (3 - 12 q3): q q3 (24 q3 - 6) = -2 G,
+// and the factor -2 (exact in binary floating point at every rounding) is
+// applied once, by adj_unmirror.
+template <bool PERIODIC, bool GUARD>
+__device__ __forceinline__ void adj_pair(const AdjI<PERIODIC>& I, const PosT<PERIODIC>& pj,
+                                         const float4& zj, int i, int j, float C1, float K2,
+                                         AdjAcc& t)
+{
+  float dx, dy, dz, ex, ey, ez;
+  if constexpr (PERIODIC) {
+    dx = (float)(int)((pj.x ^ I.mx) + I.ox);
+    dy = (float)(int)((pj.y ^ I.my) + I.oy);
+    dz = (float)(int)((pj.z ^ I.mz) + I.oz);
+    ex = __builtin_fmaf(-I.sx, zj.x, I.zx);
+    ey = __builtin_fmaf(-I.sy, zj.y, I.zy);
+    ez = __builtin_fmaf(-I.sz, zj.z, I.zz);
+  } else {
+    dx = I.x - pj.x;
+    dy = I.y - pj.y;
+    dz = I.z - pj.z;
+    ex = I.zx - zj.x;
+    ey = I.zy - zj.y;
+    ez = I.zz - zj.z;
   }
   const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
   float q = __builtin_amdgcn_rcpf(r2) * C1;
@@ -74,7 +161,6 @@ __device__ __forceinline__ void adj_pair(const PosT<PERIODIC>& pi, const PosT<PE
   const float q3 = q * q * q;
   const float w = q3 * __builtin_fmaf(q3, 2.f, -1.f);  // q3 (2 q3 - 1)
   const float A1 = q * w;                                // force / H z first term
-  const float ex = zi.x - zj.x, ey = zi.y - zj.y, ez = zi.z - zj.z;
   const float ddz = __builtin_fmaf(dz, ez, __builtin_fmaf(dy, ey, dx * ex));
   const float A2 = (K2 * q * q) * q3 * __builtin_fmaf(q3, -14.f, 4.f);
   const float T = A2 * ddz;
@@ -90,14 +176,19 @@ __device__ __forceinline__ void adj_pair(const PosT<PERIODIC>& pi, const PosT<PE
   t.gz = __builtin_fmaf(G, dz, t.gz);
   t.a += q3;
   t.c = __builtin_fmaf(q3, q3, t.c);
-  t.s = __builtin_fmaf(q * q3 * __builtin_fmaf(q3, 24.f, -6.f), ddz, t.s);
+  t.s = __builtin_fmaf(G, ddz, t.s);
 }
 
+// pairs (i, j), j in [jb, je), summed in blocks of kJB j; the j data stream
+// through SGPRs (uniform scalar loads).  Loading each group of four j while
+// the previous group is evaluated (explicit double buffering) measured slower
+// on MI355X (10.96 against 9.61 us at N = 2048): the compiler's own schedule
+// of the unrolled loop is kept.
 template <bool PERIODIC, bool GUARD>
 __device__ __forceinline__ void adj_range(const uint4* __restrict__ q, const float4* __restrict__ pf,
                                           const float4* __restrict__ z, int jb, int je,
-                                          const PosT<PERIODIC>& pi, const float4& zi, int i,
-                                          float C1, float K2, AdjAcc& acc)
+                                          const AdjI<PERIODIC>& I, int i, float C1, float K2,
+                                          AdjAcc& acc)
 {
   for (int b0 = jb; b0 < je; b0 += kJB) {
     const int b1 = min(je, b0 + kJB);
@@ -107,7 +198,7 @@ __device__ __forceinline__ void adj_range(const uint4* __restrict__ q, const flo
     for (int j = b0; j < b1; ++j) {
       const PosT<PERIODIC> pj = load_pos<PERIODIC>(q, pf, j);
       const float4 zj = z[j];
-      adj_pair<PERIODIC, GUARD>(pi, pj, zi, zj, i, j, C1, K2, t);
+      adj_pair<PERIODIC, GUARD>(I, pj, zj, i, j, C1, K2, t);
     }
     adj_add(acc, t);
   }
@@ -130,18 +221,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_pass(const uint4* __restrict__ q
   const int j1 = min(N, j0 + jchunk);
   const int i = i0 + lane;
   const int ic = min(i, N - 1);
-  const PosT<PERIODIC> pi = load_pos<PERIODIC>(q, pf, ic);
-  const float4 zi = z[ic];
+  const AdjI<PERIODIC> I = adj_lane<PERIODIC>(load_pos<PERIODIC>(q, pf, ic), z[ic]);
   AdjAcc acc;
   adj_zero(acc);
   const int d0 = max(j0, i0), d1 = min(j1, i0 + 64);
   if (d0 >= d1) {
-    adj_range<PERIODIC, false>(q, pf, z, j0, j1, pi, zi, i, C1, K2, acc);
+    adj_range<PERIODIC, false>(q, pf, z, j0, j1, I, i, C1, K2, acc);
   } else {
-    adj_range<PERIODIC, false>(q, pf, z, j0, d0, pi, zi, i, C1, K2, acc);
-    adj_range<PERIODIC, true>(q, pf, z, d0, d1, pi, zi, i, C1, K2, acc);
-    adj_range<PERIODIC, false>(q, pf, z, d1, j1, pi, zi, i, C1, K2, acc);
+    adj_range<PERIODIC, false>(q, pf, z, j0, d0, I, i, C1, K2, acc);
+    adj_range<PERIODIC, true>(q, pf, z, d0, d1, I, i, C1, K2, acc);
+    adj_range<PERIODIC, false>(q, pf, z, d1, j1, I, i, C1, K2, acc);
   }
+  adj_unmirror<PERIODIC>(I, acc);
   if (i < N) {
     const size_t o = 3 * ((size_t)s * Npad + i);
     store_part(part + o, acc.hx, acc.hy, acc.hz, acc.a);
@@ -150,54 +241,180 @@ __global__ __launch_bounds__(kBlock) void k_adj_pass(const uint4* __restrict__ q
   }
 }
 
+// The same pass with the j range of each slice split between the JW waves of
+// a workgroup, which share one 64-particle i tile: grid (ceil(N / 64), S).
+// At sizes where the plain grid gives about one wave per SIMD (N = 2048: 8 x
+// 32 workgroups) the pass was latency-bound (16.7 us for 4.2e6 pairs); more
+// waves per SIMD hide the rcp and scalar-load latencies (JW = 4: 9.5 us).  The waves' sums are
+// combined in LDS in wave order (deterministic) and written to the same
+// partial layout, so the reduction kernels are shared.
+template <bool PERIODIC, int JW>
+__global__ __launch_bounds__(64 * JW) void k_adj_pass_split(const uint4* __restrict__ q,
+                                                           const float4* __restrict__ pf,
+                                                           const float4* __restrict__ z,
+                                                           float4* __restrict__ part, int N,
+                                                           int Npad, int jchunk, float C1,
+                                                           float K2)
+{
+  __shared__ float red[JW][12][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = blockIdx.x * 64;
+  if (i0 >= N) return;  // a padding tile: the whole workgroup leaves
+  const int s = blockIdx.y;
+  const int js = s * jchunk;
+  const int je = min(N, js + jchunk);
+  const int jq = (jchunk + JW - 1) / JW;
+  const int j0 = min(je, js + wv * jq);
+  const int j1 = min(je, j0 + jq);
+  const int i = i0 + lane;
+  const int ic = min(i, N - 1);
+  const AdjI<PERIODIC> I = adj_lane<PERIODIC>(load_pos<PERIODIC>(q, pf, ic), z[ic]);
+  AdjAcc acc;
+  adj_zero(acc);
+  const int d0 = max(j0, i0), d1 = min(j1, i0 + 64);
+  if (d0 >= d1) {
+    adj_range<PERIODIC, false>(q, pf, z, j0, j1, I, i, C1, K2, acc);
+  } else {
+    adj_range<PERIODIC, false>(q, pf, z, j0, d0, I, i, C1, K2, acc);
+    adj_range<PERIODIC, true>(q, pf, z, d0, d1, I, i, C1, K2, acc);
+    adj_range<PERIODIC, false>(q, pf, z, d1, j1, I, i, C1, K2, acc);
+  }
+  adj_unmirror<PERIODIC>(I, acc);
+  const float v[12] = {acc.hx, acc.hy, acc.hz, acc.a, acc.fx, acc.fy,
+                       acc.fz, acc.c,  acc.gx, acc.gy, acc.gz, acc.s};
+#pragma unroll
+  for (int k = 0; k < 12; ++k) red[wv][k][lane] = v[k];
+  __syncthreads();
+  // waves 0, 1, 2 each add one float4 of the 12 sums over the JW waves, in
+  // wave order, and store it
+  if (wv >= 3 || i >= N) return;
+  float t[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    t[c] = red[0][4 * wv + c][lane];
+#pragma unroll
+    for (int w = 1; w < JW; ++w) t[c] += red[w][4 * wv + c][lane];
+  }
+  store_part(part + 3 * ((size_t)s * Npad + i) + wv, t[0], t[1], t[2], t[3]);
+}
+
 // per particle: sum slices in FP64, scale, write hz/f/gw (double4), block
-// partials of {sum (c-a), sum (2c-a), sum (24c-6a), sum s, sum f.z}
+// partials of {sum (c-a), sum (2c-a), sum (24c-6a), sum s, sum f.z}.
+// Four lanes share a particle (blocks of 64 particles, one row of blk each):
+// lane r sums the r-th quarter of the slices in slice order with its loads in
+// flight together, then a fixed butterfly adds the quarters.  One thread per
+// particle left the pass bound by load latency (N = 2048: 8 blocks, 32
+// dependent slice loads each).
+constexpr int kAdjLanes = 4;
+constexpr int kAdjRow = kBlock / kAdjLanes;  // particles per block / row
+// the fixed-point coordinate of a double position (q = rint(x qs) mod 2^32)
+__device__ __forceinline__ unsigned fixqd(double x, double qs)
+{
+  return (unsigned)(long long)__builtin_rint(x * qs);
+}
+
+// what the reverse sweep's update of particle i reads besides the pass's sums
+struct AdjPre {
+  double4 u, lv, vp, xn;
+};
+__device__ __forceinline__ void adj_preload(int i, const AdjStep& a, AdjPre& p)
+{
+  if (a.mode != 0) p.u = a.u[i];
+  if (a.mode == 1) p.lv = a.lv[i];
+  if (a.mode == 1 && a.vel_prev) p.vp = a.vel_prev[i];
+  if (a.next_x) p.xn = a.next_x[i];
+}
+__device__ __forceinline__ void adj_update_one(int i, const double4& h, const double4& F,
+                                               const double4& G, const AdjStep& a,
+                                               const AdjPre& p, float4* z);
+
 __global__ __launch_bounds__(kBlock) void k_adj_reduce(const float4* __restrict__ part,
-                                                       const float4* __restrict__ z, int N, int S,
+                                                       float4* __restrict__ z, int N, int S,
                                                        int Npad, AdjScales sc,
                                                        double4* __restrict__ hz,
                                                        double4* __restrict__ f,
                                                        double4* __restrict__ gw,
-                                                       double* __restrict__ blk)
+                                                       double* __restrict__ blk, AdjStep step)
 {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  double v[5] = {0., 0., 0., 0., 0.};
-  if (i < N) {
-    double h[3] = {0., 0., 0.}, ff[3] = {0., 0., 0.}, g[3] = {0., 0., 0.}, a = 0., c = 0., s = 0.;
-    for (int k = 0; k < S; ++k) {
-      const size_t o = 3 * ((size_t)k * Npad + i);
-      const float4 p0 = part[o], p1 = part[o + 1], p2 = part[o + 2];
-      h[0] += p0.x;
-      h[1] += p0.y;
-      h[2] += p0.z;
-      a += p0.w;
-      ff[0] += p1.x;
-      ff[1] += p1.y;
-      ff[2] += p1.z;
-      c += p1.w;
-      g[0] += p2.x;
-      g[1] += p2.y;
-      g[2] += p2.z;
-      s += p2.w;
+  const int r = threadIdx.x % kAdjLanes;
+  const int i = blockIdx.x * kAdjRow + threadIdx.x / kAdjLanes;
+  const int C = (S + kAdjLanes - 1) / kAdjLanes;
+  const int s0 = i < N ? min(S, r * C) : 0;
+  const int s1 = i < N ? min(S, s0 + C) : 0;
+  // the particle's own loads go out ahead of the slice sums (their latencies overlap)
+  const bool own = i < N && r == 0;
+  const float4 zi = own ? z[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  AdjPre pre{};
+  if (own) adj_preload(i, step, pre);
+  double t[12] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  constexpr int B = 4;
+  for (int k0 = s0; k0 < s1; k0 += B) {
+    float4 p[B][3];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int sk = min(k0 + k, s1 - 1);
+      const size_t o = 3 * ((size_t)sk * Npad + i);
+      p[k][0] = part[o];
+      p[k][1] = part[o + 1];
+      p[k][2] = part[o + 2];
     }
-    const double4 F = make_double4(sc.f * ff[0], sc.f * ff[1], sc.f * ff[2], c - a);
-    hz[i] = make_double4(sc.h * h[0], sc.h * h[1], sc.h * h[2], 0.);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      if (k0 + k >= s1) break;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        t[4 * c + 0] += p[k][c].x;
+        t[4 * c + 1] += p[k][c].y;
+        t[4 * c + 2] += p[k][c].z;
+        t[4 * c + 3] += p[k][c].w;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < kAdjLanes; o <<= 1)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) t[k] += __shfl_xor(t[k], o, 64);
+  double v[5] = {0., 0., 0., 0., 0.};
+  if (i < N && r == 0) {
+    // t: (hx, hy, hz, a), (fx, fy, fz, c), (gx, gy, gz, s)
+    const double a = t[3], c = t[7], s = t[11];
+    const double4 F = make_double4(sc.f * t[4], sc.f * t[5], sc.f * t[6], c - a);
+    const double4 H = make_double4(sc.h * t[0], sc.h * t[1], sc.h * t[2], 0.);
+    const double4 G = make_double4(sc.g * t[8], sc.g * t[9], sc.g * t[10], 0.);
+    hz[i] = H;
     f[i] = F;
-    gw[i] = make_double4(sc.g * g[0], sc.g * g[1], sc.g * g[2], 0.);
-    const float4 zi = z[i];
+    gw[i] = G;
     v[0] = c - a;
     v[1] = 2. * c - a;
     v[2] = 24. * c - 6. * a;
     v[3] = s;
     v[4] = F.x * zi.x + F.y * zi.y + F.z * zi.z;
+    if (step.next_x) {  // the next pass's coordinates (k_dprep)
+      const double4 x = pre.xn;
+      if (step.qs > 0.)
+        step.q[i] = make_uint4(fixqd(x.x, step.qs), fixqd(x.y, step.qs), fixqd(x.z, step.qs), 0u);
+      else
+        step.xf[i] = make_float4((float)x.x, (float)x.y, (float)x.z, 0.f);
+    }
+    if (step.mode == 1) {
+      adj_update_one(i, H, F, G, step, pre, z);
+    } else if (step.mode == 2) {  // after the pass at x_0: dL/dx0 = u + H z
+      const double4 U = pre.u;
+      step.dx0[i] = make_double4(U.x + H.x, U.y + H.y, U.z + H.z, 0.);
+    }
   }
   // block totals into row blockIdx.x (5 fields of an 8-double row)
   const int fld[5] = {0, 1, 2, 3, 4};
-  block_sum_store<5>(v, blk + (size_t)blockIdx.x * 8, fld);
+  if (step.rowacc)
+    block_sum_store<5, kWaves, true>(v, step.rowacc + (size_t)blockIdx.x * 8, fld);
+  else
+    block_sum_store<5>(v, blk + (size_t)blockIdx.x * 8, fld);
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_finalize(const double* __restrict__ blk, int nblk,
-                                                         AdjScales sc, double* __restrict__ out)
+                                                         AdjScales sc, double* __restrict__ out,
+                                                         AdjStep step)
 {
   double v[5] = {0., 0., 0., 0., 0.};
   for (int r = threadIdx.x; r < nblk; r += kBlock)
@@ -211,6 +428,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_finalize(const double* __restric
     out[ADJ_W_SIGMA] = 12. * sc.eps / sc.sigma * v[2];
     out[ADJ_FSIGMA_Z] = 0.5 * sc.fsig * v[3];
     out[ADJ_F_Z] = v[4];
+    if (step.g) {
+      // theta accumulation: g[0] = dL/deps, g[1] = dL/dsigma += (dF/deps)^T z
+      // + (dF/dsigma)^T z (+ the direct terms of the loss when direct != 0)
+      double* g = step.g;
+      g[0] += out[ADJ_F_Z] / step.eps;
+      g[1] += out[ADJ_FSIGMA_Z];
+      if (step.direct) {
+        g[0] += step.invM * (out[ADJ_E] / step.eps + step.cw * out[ADJ_W] / step.eps);
+        g[1] += step.invM * (out[ADJ_E_SIGMA] + step.cw * out[ADJ_W_SIGMA]);
+      }
+    }
   }
 }
 
@@ -236,66 +464,33 @@ __global__ __launch_bounds__(kBlock) void k_adj_init(const double4* __restrict__
 //   w  = (dt^2/2) u + (dt/2) lv
 //   lv <- dt u + lv + g_v(m-1)        (g_v(0) = 0: vel_prev may be null)
 //   z  = w + (dt/2) lv  (m-1 >= 1),  z = w  (m-1 == 0)
-__global__ __launch_bounds__(kBlock) void k_adj_update(int N, double invM, double cw, double gv,
-                                                       double dt, int last,
-                                                       const double4* __restrict__ hz,
-                                                       const double4* __restrict__ f,
-                                                       const double4* __restrict__ gw,
-                                                       const double4* __restrict__ vel_prev,
-                                                       double4* __restrict__ u,
-                                                       double4* __restrict__ lv,
-                                                       float4* __restrict__ z)
+// that update for particle i (H z, F, grad W of the pass at x_m)
+__device__ __forceinline__ void adj_update_one(int i, const double4& h, const double4& F,
+                                               const double4& G, const AdjStep& a,
+                                               const AdjPre& p, float4* z)
 {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= N) return;
-  const double4 h = hz[i], F = f[i], G = gw[i];
-  double4 U = u[i], Lv = lv[i];
-  U.x += invM * (-F.x + cw * G.x) + h.x;
-  U.y += invM * (-F.y + cw * G.y) + h.y;
-  U.z += invM * (-F.z + cw * G.z) + h.z;
-  const double hd2 = 0.5 * dt * dt, hd = 0.5 * dt;
+  double4 U = p.u, Lv = p.lv;
+  U.x += a.invM * (-F.x + a.cw * G.x) + h.x;
+  U.y += a.invM * (-F.y + a.cw * G.y) + h.y;
+  U.z += a.invM * (-F.z + a.cw * G.z) + h.z;
+  const double hd2 = 0.5 * a.dt * a.dt, hd = 0.5 * a.dt;
   const double wx = hd2 * U.x + hd * Lv.x, wy = hd2 * U.y + hd * Lv.y, wz = hd2 * U.z + hd * Lv.z;
   double gx = 0., gy = 0., gz = 0.;
-  if (vel_prev) {
-    const double4 v = vel_prev[i];
-    gx = gv * v.x;
-    gy = gv * v.y;
-    gz = gv * v.z;
+  if (a.vel_prev) {
+    const double4 v = p.vp;
+    gx = a.gv * v.x;
+    gy = a.gv * v.y;
+    gz = a.gv * v.z;
   }
-  Lv.x = dt * U.x + Lv.x + gx;
-  Lv.y = dt * U.y + Lv.y + gy;
-  Lv.z = dt * U.z + Lv.z + gz;
-  u[i] = U;
-  lv[i] = Lv;
-  if (last)
+  Lv.x = a.dt * U.x + Lv.x + gx;
+  Lv.y = a.dt * U.y + Lv.y + gy;
+  Lv.z = a.dt * U.z + Lv.z + gz;
+  a.u[i] = U;
+  a.lv[i] = Lv;
+  if (a.last)
     z[i] = make_float4((float)wx, (float)wy, (float)wz, 0.f);
   else
     z[i] = make_float4((float)(wx + hd * Lv.x), (float)(wy + hd * Lv.y), (float)(wz + hd * Lv.z), 0.f);
-}
-
-// after the pass at x_0: dL/dx0 = u + H z
-__global__ __launch_bounds__(kBlock) void k_adj_final(int N, const double4* __restrict__ hz,
-                                                      const double4* __restrict__ u,
-                                                      double4* __restrict__ dx0)
-{
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= N) return;
-  const double4 h = hz[i], U = u[i];
-  dx0[i] = make_double4(U.x + h.x, U.y + h.y, U.z + h.z, 0.);
-}
-
-// theta accumulation on the device: g[0] = dL/deps, g[1] = dL/dsigma
-//   += (dF/deps)^T z + (dF/dsigma)^T z  (+ direct terms when direct != 0)
-__global__ void k_adj_theta(const double* __restrict__ sc, double eps, double invM, double cw3,
-                            int direct, double* __restrict__ g)
-{
-  if (threadIdx.x != 0) return;
-  g[0] += sc[ADJ_F_Z] / eps;
-  g[1] += sc[ADJ_FSIGMA_Z];
-  if (direct) {
-    g[0] += invM * (sc[ADJ_E] / eps + cw3 * sc[ADJ_W] / eps);
-    g[1] += invM * (sc[ADJ_E_SIGMA] + cw3 * sc[ADJ_W_SIGMA]);
-  }
 }
 
 __global__ void k_adj_zero2(double* g)
@@ -318,11 +513,6 @@ __global__ __launch_bounds__(kBlock) void k_dstate_init(const float4* __restrict
   const float4 p = pos[i], v = vel[i];
   xd[i] = make_double4(p.x, p.y, p.z, 0.);
   vd[i] = make_double4(v.x, v.y, v.z, 0.);
-}
-
-__device__ __forceinline__ unsigned fixqd(double x, double qs)
-{
-  return (unsigned)(long long)__builtin_rint(x * qs);
 }
 
 // internal coordinates of double positions: fixed point (qs > 0) or float
@@ -365,12 +555,17 @@ __global__ __launch_bounds__(kBlock) void k_dkick(const float4* __restrict__ fpa
                                                   const float* __restrict__ apart, int N, int S,
                                                   int Npad, double FS, double dt,
                                                   double4* __restrict__ fd, double4* __restrict__ vd,
-                                                  double* __restrict__ blk)
+                                                  double* __restrict__ blk,
+                                                  const double4* __restrict__ xd,
+                                                  double4* __restrict__ slot_x,
+                                                  double4* __restrict__ slot_v)
 {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  // four lanes share a particle (blocks of 64 particles), as in k_adj_reduce
+  const int r = threadIdx.x % kAdjLanes;
+  const int i = blockIdx.x * kAdjRow + threadIdx.x / kAdjLanes;
+  const Partial p = sum_slices_lanes<kAdjLanes, true>(fpart, apart, i < N ? i : -1, S, Npad, r);
   double v3[3] = {0., 0., 0.};
-  if (i < N) {
-    const Partial p = sum_slices(fpart, apart, i, S, Npad);
+  if (i < N && r == 0) {
     const double4 f = make_double4(FS * p.fx, FS * p.fy, FS * p.fz, 0.5 * (p.b - p.a));
     fd[i] = f;
     double4 v = vd[i];
@@ -379,6 +574,10 @@ __global__ __launch_bounds__(kBlock) void k_dkick(const float4* __restrict__ fpa
     v.y += h * f.y;
     v.z += h * f.z;
     vd[i] = v;
+    if (slot_x) {
+      slot_x[i] = xd[i];
+      slot_v[i] = v;
+    }
     v3[0] = f.w;
     v3[1] = p.b;
     v3[2] = v.x * v.x + v.y * v.y + v.z * v.z;
@@ -416,13 +615,76 @@ __global__ void k_dzero4(double* acc)
 
 inline int nb(int n) { return (n + kBlock - 1) / kBlock; }
 
+// the j-split passes run one 64-particle i tile per workgroup; the tile count
+// is padded to a multiple of the 8 XCDs, so that tile t is written by
+// workgroups of XCD t mod 8 (workgroups go to the XCDs round robin) whatever
+// their slice, the XCD whose L2 serves the row-t block of the reduction
+// kernels that read the tile's partials (tiles past N exit at once)
+inline unsigned split_tiles(int N) { return (unsigned)((((N + 63) / 64) + 7) / 8 * 8); }
+
 }  // namespace
+
+// the j-split pass serves plans whose plain grid holds under four waves per
+// SIMD of the device (1024 SIMDs); LJ_ADJ_SPLIT=0/1 forces it off or on
+bool adj_split_waves(const ForcePlan& pl)
+{
+  static const char* e = std::getenv("LJ_ADJ_SPLIT");
+  if (e && e[0] == '0') return false;
+  if (e && e[0] == '1') return true;
+  return (long)pl.gx * kWaves * pl.S < 4L * 1024;
+}
+
+// waves sharing a tile in the j-split pass (LJ_ADJ_SPLIT_WAVES=4 for four)
+int adj_split_jw()
+{
+  static const char* e = std::getenv("LJ_ADJ_SPLIT_WAVES");
+  return (e && e[0] == '4') ? 4 : 8;
+}
+
+// The force pass's sliced plan where its grid fills the device.  Otherwise
+// the j-split pass: its waves split each slice, so the slices are fewer --
+// about 2048 waves in all, at least 32 j per wave (N = 2048: 8 slices of 256
+// j, 8 waves of 32 j per 64-particle tile, where the force plan's 32 slices
+// of 64 j left 8 j per wave to amortise each wave's set-up and LDS
+// reduction; measured on MI355X, the reverse sweep took 2.84 ms against 3.28
+// ms, and 2.96 ms at 16 slices).  LJ_ADJ_SLICES overrides the slice count.
+ForcePlan adj_force_plan(int N)
+{
+  const ForcePlan pl = make_force_plan(N, 0, 1);
+  if (!adj_split_waves(pl)) return pl;
+  const int jw = adj_split_jw();
+  const int tiles = (N + 63) / 64;
+  int S = std::max(1, std::min(2048 / (tiles * jw), N / (32 * jw)));
+  static const char* e = std::getenv("LJ_ADJ_SLICES");
+  if (e && std::atoi(e) > 0) S = std::atoi(e);
+  return make_force_plan(N, S, 1);
+}
 
 hipError_t launch_adj_pass(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, const float4* z, float4* part)
 {
   const double u = ps.periodic ? ps.L / 4294967296.0 : 1.0;
   const float K2 = (float)(2. * u * u / (ps.sigma * ps.sigma));
+  if (adj_split_waves(pl)) {
+    dim3 grid(split_tiles(pl.N), pl.S);
+    const bool w8 = adj_split_jw() == 8;
+    if (ps.periodic) {
+      if (w8)
+        hipLaunchKernelGGL((k_adj_pass_split<true, 8>), grid, dim3(512), 0, st, q, posf, z, part,
+                           pl.N, pl.Npad, pl.jchunk, ps.C1, K2);
+      else
+        hipLaunchKernelGGL((k_adj_pass_split<true, 4>), grid, dim3(256), 0, st, q, posf, z, part,
+                           pl.N, pl.Npad, pl.jchunk, ps.C1, K2);
+    } else {
+      if (w8)
+        hipLaunchKernelGGL((k_adj_pass_split<false, 8>), grid, dim3(512), 0, st, q, posf, z, part,
+                           pl.N, pl.Npad, pl.jchunk, ps.C1, K2);
+      else
+        hipLaunchKernelGGL((k_adj_pass_split<false, 4>), grid, dim3(256), 0, st, q, posf, z, part,
+                           pl.N, pl.Npad, pl.jchunk, ps.C1, K2);
+    }
+    return hipGetLastError();
+  }
   dim3 grid(pl.gx, pl.S);
   if (ps.periodic)
     hipLaunchKernelGGL(k_adj_pass<true>, grid, dim3(kBlock), 0, st, q, posf, z, part, pl.N, pl.Npad,
@@ -448,18 +710,18 @@ AdjScales make_adj_scales(const PairScalars& ps)
 }
 
 hipError_t launch_adj_reduce(hipStream_t st, const ForcePlan& pl, const AdjScales& sc,
-                             const float4* part, const float4* z, double4* hz, double4* f,
-                             double4* gw, double* blk)
+                             const float4* part, float4* z, double4* hz, double4* f,
+                             double4* gw, double* blk, const AdjStep& step)
 {
-  hipLaunchKernelGGL(k_adj_reduce, dim3(nb(pl.N)), dim3(kBlock), 0, st, part, z, pl.N, pl.S, pl.Npad,
-                     sc, hz, f, gw, blk);
+  hipLaunchKernelGGL(k_adj_reduce, dim3(adj_rows(pl.N)), dim3(kBlock), 0, st, part, z, pl.N, pl.S, pl.Npad,
+                     sc, hz, f, gw, blk, step);
   return hipGetLastError();
 }
 
 hipError_t launch_adj_finalize(hipStream_t st, const double* blk, int nblk, const AdjScales& sc,
-                               double* out)
+                               double* out, const AdjStep& step)
 {
-  hipLaunchKernelGGL(k_adj_finalize, dim3(1), dim3(kBlock), 0, st, blk, nblk, sc, out);
+  hipLaunchKernelGGL(k_adj_finalize, dim3(1), dim3(kBlock), 0, st, blk, nblk, sc, out, step);
   return hipGetLastError();
 }
 
@@ -467,29 +729,6 @@ hipError_t launch_adj_init(hipStream_t st, const double4* vel, int N, double gv,
                            double4* u, double4* lv, float4* z)
 {
   hipLaunchKernelGGL(k_adj_init, dim3(nb(N)), dim3(kBlock), 0, st, vel, N, gv, dt, u, lv, z);
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_update(hipStream_t st, int N, double invM, double cw, double gv, double dt,
-                             int last, const double4* hz, const double4* f, const double4* gw,
-                             const double4* vel_prev, double4* u, double4* lv, float4* z)
-{
-  hipLaunchKernelGGL(k_adj_update, dim3(nb(N)), dim3(kBlock), 0, st, N, invM, cw, gv, dt, last, hz,
-                     f, gw, vel_prev, u, lv, z);
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_final(hipStream_t st, int N, const double4* hz, const double4* u,
-                            double4* dx0)
-{
-  hipLaunchKernelGGL(k_adj_final, dim3(nb(N)), dim3(kBlock), 0, st, N, hz, u, dx0);
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_theta(hipStream_t st, const double* sc, double eps, double invM, double cw3,
-                            int direct, double* g)
-{
-  hipLaunchKernelGGL(k_adj_theta, dim3(1), dim3(64), 0, st, sc, eps, invM, cw3, direct, g);
   return hipGetLastError();
 }
 
@@ -521,10 +760,10 @@ hipError_t launch_ddrift(hipStream_t st, double4* xd, double4* vd, const double4
 
 hipError_t launch_dkick(hipStream_t st, const ForcePlan& pl, double FS, double dt,
                         const float4* fpart, const float* apart, double4* fd, double4* vd,
-                        double* blk)
+                        double* blk, const double4* xd, double4* slot_x, double4* slot_v)
 {
-  hipLaunchKernelGGL(k_dkick, dim3(nb(pl.N)), dim3(kBlock), 0, st, fpart, apart, pl.N, pl.S,
-                     pl.Npad, FS, dt, fd, vd, blk);
+  hipLaunchKernelGGL(k_dkick, dim3(adj_rows(pl.N)), dim3(kBlock), 0, st, fpart, apart, pl.N, pl.S,
+                     pl.Npad, FS, dt, fd, vd, blk, xd, slot_x, slot_v);
   return hipGetLastError();
 }
 

@@ -33,10 +33,12 @@ using namespace lj;
 struct AdjState {
   int N = 0, nblk = 0;
   int seg_cap = 0, ck_cap = 0;
+  ForcePlan plan;  // pair pass geometry (adj_force_plan)
   float4* part = nullptr;
   float4* z = nullptr;
   double4 *hz = nullptr, *f = nullptr, *gw = nullptr, *u = nullptr, *lv = nullptr, *dx0 = nullptr;
   double* blk = nullptr;
+  double* rowacc = nullptr;  // block partials added over the reverse sweep's states
   double* scal = nullptr;
   double* theta = nullptr;
   double* acc = nullptr;                                // {sum E, sum P, E, P}
@@ -48,9 +50,26 @@ struct AdjState {
   hipGraphExec_t gexec = nullptr;
   double g_dt = 0., g_eps = 0., g_sigma = 0.;
   int g_bc = -1;  // boundary the captured step was built for (periodic flag, image scale)
+  // runs of g_k steps captured together (same key as the one-step graph):
+  // the forward's, and the recompute's that also stores each state in its
+  // segment slot 1..g_k
+  hipGraph_t graph_k = nullptr, graph_seg = nullptr;
+  hipGraphExec_t gexec_k = nullptr, gexec_seg = nullptr;
+  int g_k = 0;
 };
 
 namespace {
+
+void adj_drop_kgraphs(AdjState* a)
+{
+  if (a->gexec_k) (void)hipGraphExecDestroy(a->gexec_k);
+  if (a->graph_k) (void)hipGraphDestroy(a->graph_k);
+  if (a->gexec_seg) (void)hipGraphExecDestroy(a->gexec_seg);
+  if (a->graph_seg) (void)hipGraphDestroy(a->graph_seg);
+  a->gexec_k = a->gexec_seg = nullptr;
+  a->graph_k = a->graph_seg = nullptr;
+  a->g_k = 0;
+}
 
 void adj_drop_graph(AdjState* a)
 {
@@ -58,6 +77,7 @@ void adj_drop_graph(AdjState* a)
   if (a->graph) (void)hipGraphDestroy(a->graph);
   a->gexec = nullptr;
   a->graph = nullptr;
+  adj_drop_kgraphs(a);
 }
 
 int adj_ensure(lj_ctx* c, int ncheck, int seg)
@@ -66,9 +86,10 @@ int adj_ensure(lj_ctx* c, int ncheck, int seg)
   if (!a) {
     a = new AdjState();
     a->N = c->cfg.N;
-    a->nblk = c->nblk;
+    a->nblk = adj_rows(c->cfg.N);  // rows of k_adj_reduce / k_dkick
     const int N = a->N;
-    LJ_TRY(dev_alloc(&a->part, (size_t)3 * c->plan.S * c->plan.Npad));
+    a->plan = adj_force_plan(N);
+    LJ_TRY(dev_alloc(&a->part, (size_t)3 * a->plan.S * a->plan.Npad));
     LJ_TRY(dev_alloc(&a->z, N));
     LJ_TRY(dev_alloc(&a->hz, N));
     LJ_TRY(dev_alloc(&a->f, N));
@@ -77,6 +98,7 @@ int adj_ensure(lj_ctx* c, int ncheck, int seg)
     LJ_TRY(dev_alloc(&a->lv, N));
     LJ_TRY(dev_alloc(&a->dx0, N));
     LJ_TRY(dev_alloc(&a->blk, (size_t)a->nblk * 8));
+    LJ_TRY(dev_alloc(&a->rowacc, (size_t)a->nblk * 8));
     LJ_TRY(dev_alloc(&a->scal, kAdjScalars));
     LJ_TRY(dev_alloc(&a->theta, 2));
     LJ_TRY(dev_alloc(&a->acc, 4));
@@ -94,6 +116,7 @@ int adj_ensure(lj_ctx* c, int ncheck, int seg)
   }
   if (seg > a->seg_cap) {
     LJ_HIP(hipStreamSynchronize(c->stream));
+    adj_drop_kgraphs(a);  // the recompute graph holds the old slots
     dev_free(a->seg);
     a->seg_cap = 0;
     LJ_TRY(dev_alloc(&a->seg, (size_t)2 * seg * a->N));
@@ -120,15 +143,20 @@ int dforces(lj_ctx* c, hipStream_t st, const double4* x)
   return LJ_OK;
 }
 
-// one FP64 step: drift, forces, kick, observables accumulated into acc
-int enqueue_dstep(lj_ctx* c, hipStream_t st, double dt)
+// one FP64 step: drift, forces, kick, observables accumulated into acc.
+// slot (the recompute, may be null): the state goes to slot (x) and slot + N
+// (v) from the kick kernel, and the observables, which only the forward
+// sweep's loss reads, are left out
+int enqueue_dstep(lj_ctx* c, hipStream_t st, double dt, double4* slot = nullptr)
 {
   AdjState* a = c->adj;
   const double vol = std::pow(c->L, 3.);
-  LJ_HIP(launch_ddrift(st, a->xd, a->vd, a->fd, c->cfg.N, dt, qscale_of(c), c->q, a->xf));
+  const int N = c->cfg.N;
+  LJ_HIP(launch_ddrift(st, a->xd, a->vd, a->fd, N, dt, qscale_of(c), c->q, a->xf));
   LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, a->xf, c->fpart, c->apart, false, false));
-  LJ_HIP(launch_dkick(st, fplan(c), c->ps.FS, dt, c->fpart, c->apart, a->fd, a->vd, a->blk));
-  LJ_HIP(launch_dfinal(st, a->blk, a->nblk, c->cfg.eps, vol, 1, a->acc));
+  LJ_HIP(launch_dkick(st, fplan(c), c->ps.FS, dt, c->fpart, c->apart, a->fd, a->vd, a->blk, a->xd,
+                      slot, slot ? slot + N : nullptr));
+  if (!slot) LJ_HIP(launch_dfinal(st, a->blk, a->nblk, c->cfg.eps, vol, 1, a->acc));
   return LJ_OK;
 }
 
@@ -159,6 +187,48 @@ int ensure_dgraph(lj_ctx* c, double dt)
   return LJ_OK;
 }
 
+// captures `steps` FP64 steps back to back; with_slots: step s also writes
+// its state into segment slot s (1..steps), without the loss observables
+int capture_dsteps(lj_ctx* c, double dt, int steps, bool with_slots, hipGraph_t* graph,
+                   hipGraphExec_t* exec)
+{
+  AdjState* a = c->adj;
+  const int N = c->cfg.N;
+  hipStream_t st = c->stream;
+  LJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = LJ_OK;
+  for (int s = 1; s <= steps && rc == LJ_OK; ++s)
+    rc = enqueue_dstep(c, st, dt, with_slots ? a->seg + (size_t)2 * s * N : nullptr);
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(st, &g);
+  if (rc != LJ_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  LJ_HIP(ee);
+  *graph = g;
+  LJ_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return LJ_OK;
+}
+
+// the k-step runs of the one-step graph's key (call after ensure_dgraph);
+// the segment buffer must hold k + 1 slots
+int ensure_kgraphs(lj_ctx* c, double dt, int k)
+{
+  AdjState* a = c->adj;
+  if (a->gexec_k && a->gexec_seg && a->g_k == k) return LJ_OK;
+  adj_drop_kgraphs(a);
+  LJ_HIP(hipStreamSynchronize(c->stream));
+  int rc = capture_dsteps(c, dt, k, false, &a->graph_k, &a->gexec_k);
+  if (rc == LJ_OK) rc = capture_dsteps(c, dt, k, true, &a->graph_seg, &a->gexec_seg);
+  if (rc != LJ_OK) {
+    adj_drop_kgraphs(a);
+    return rc;
+  }
+  a->g_k = k;
+  return LJ_OK;
+}
+
 int check_traj_args(lj_ctx* c, double dt, int nsteps)
 {
   if (nsteps < 1) return fail(LJ_ERR_ARG, "trajectory: nsteps must be >= 1");
@@ -181,13 +251,22 @@ int forward(lj_ctx* c, double dt, int M, int k, double pt, double wp, double* ou
   LJ_TRY(dforces(c, st, a->xd));
   LJ_HIP(launch_dzero4(st, a->acc));
   LJ_TRY(ensure_dgraph(c, dt));
+  if (k > 1 && a->seg_cap >= k + 1) LJ_TRY(ensure_kgraphs(c, dt, k));
   if (k > 0) {
     LJ_TRY(dcopy(c, a->ck, a->xd));
     LJ_TRY(dcopy(c, a->ck + N, a->vd));
     LJ_TRY(dcopy(c, a->ck + 2 * N, a->fd));
   }
-  for (int m = 1; m <= M; ++m) {
-    LJ_HIP(hipGraphLaunch(a->gexec, st));
+  // whole checkpoint intervals replay the k-step graph, the rest single steps
+  const bool runs = k > 1 && a->gexec_k && a->g_k == k;
+  for (int m = 0; m < M;) {
+    if (runs && m % k == 0 && m + k <= M) {
+      LJ_HIP(hipGraphLaunch(a->gexec_k, st));
+      m += k;
+    } else {
+      LJ_HIP(hipGraphLaunch(a->gexec, st));
+      m += 1;
+    }
     if (k > 0 && m % k == 0 && m < M) {
       double4* slot = a->ck + (size_t)3 * (m / k) * N;
       LJ_TRY(dcopy(c, slot, a->xd));
@@ -205,15 +284,18 @@ int forward(lj_ctx* c, double dt, int M, int k, double pt, double wp, double* ou
   return LJ_OK;
 }
 
-// adjoint pass at FP64 positions x with vector a->z
-int adj_pass_d(lj_ctx* c, const double4* x, hipStream_t st)
+// adjoint pass at FP64 positions x (prep: their internal coordinates are not
+// yet in c->q / a->xf) with vector a->z, followed by the reverse sweep's
+// per-particle update (`step`, fused into the reduction); the finaliser runs
+// when the step's block partials are not accumulated (step.rowacc null)
+int adj_pass_d(lj_ctx* c, const double4* x, bool prep, hipStream_t st, const AdjStep& step)
 {
   AdjState* a = c->adj;
-  LJ_HIP(launch_dprep(st, x, c->cfg.N, qscale_of(c), c->q, a->xf));
-  LJ_HIP(launch_adj_pass(st, c->plan, c->ps, c->q, a->xf, a->z, a->part));
+  if (prep) LJ_HIP(launch_dprep(st, x, c->cfg.N, qscale_of(c), c->q, a->xf));
+  LJ_HIP(launch_adj_pass(st, a->plan, c->ps, c->q, a->xf, a->z, a->part));
   const AdjScales sc = make_adj_scales(c->ps);
-  LJ_HIP(launch_adj_reduce(st, c->plan, sc, a->part, a->z, a->hz, a->f, a->gw, a->blk));
-  LJ_HIP(launch_adj_finalize(st, a->blk, a->nblk, sc, a->scal));
+  LJ_HIP(launch_adj_reduce(st, a->plan, sc, a->part, a->z, a->hz, a->f, a->gw, a->blk, step));
+  if (!step.rowacc) LJ_HIP(launch_adj_finalize(st, a->blk, a->nblk, sc, a->scal, step));
   return LJ_OK;
 }
 
@@ -234,6 +316,7 @@ void adj_free(lj_ctx* c)
   dev_free(a->lv);
   dev_free(a->dx0);
   dev_free(a->blk);
+  dev_free(a->rowacc);
   dev_free(a->scal);
   dev_free(a->theta);
   dev_free(a->acc);
@@ -298,9 +381,11 @@ int lj_adjoint_pass_device(lj_ctx* c, const float* d_pos4, const float* d_z4, do
   const float4* p = reinterpret_cast<const float4*>(d_pos4);
   const float4* z = reinterpret_cast<const float4*>(d_z4);
   if (c->ps.periodic) LJ_HIP(launch_prep_q(st, p, c->q, c->cfg.N, c->ps.qscale));
-  LJ_HIP(launch_adj_pass(st, c->plan, c->ps, c->q, p, z, a->part));
+  LJ_HIP(launch_adj_pass(st, a->plan, c->ps, c->q, p, z, a->part));
   const AdjScales sc = make_adj_scales(c->ps);
-  LJ_HIP(launch_adj_reduce(st, c->plan, sc, a->part, z, a->hz, a->f, a->gw, a->blk));
+  // the plain pass (no fused update) only reads z
+  LJ_HIP(launch_adj_reduce(st, a->plan, sc, a->part, const_cast<float4*>(z), a->hz, a->f, a->gw,
+                           a->blk));
   LJ_HIP(launch_adj_finalize(st, a->blk, a->nblk, sc, a->scal));
   const size_t b4 = sizeof(double4) * c->cfg.N;
   if (d_hz4) LJ_HIP(hipMemcpyAsync(d_hz4, a->hz, b4, hipMemcpyDeviceToDevice, st));
@@ -351,7 +436,12 @@ int lj_trajectory_grad(lj_ctx* c, double dt, int nsteps, int checkpoint_every, d
     const double cw = cP / (3. * vol);               // dP/dx = gradW / (3 vol)
     const double gv = invM * cP * (2. / 3.) / vol;  // g_v = (1/M) cP (2/3) v / vol
     check(launch_adj_zero2(st, a->theta));
+    // the block partials of the states m >= 1 add up in rowacc; one finaliser
+    // turns their total into the eps / sigma terms at the end of the sweep
+    check(launch_fill_u64(st, reinterpret_cast<unsigned long long*>(a->rowacc),
+                          (int)((size_t)a->nblk * 8)));
     bool first = true;
+    bool prepped = false;  // the next pass's coordinates prepared by the last reduction
     for (int cseg = ncheck - 1; cseg >= 0 && rc == LJ_OK; --cseg) {
       const int s0 = cseg * k, s1 = std::min(M, s0 + k);
       // recompute the segment's states s0..s1 from its checkpoint
@@ -360,10 +450,14 @@ int lj_trajectory_grad(lj_ctx* c, double dt, int nsteps, int checkpoint_every, d
       if ((rc = dcopy(c, a->xd, ckp)) || (rc = dcopy(c, a->vd, ckp + N)) || (rc = dcopy(c, a->fd, ckp + 2 * N)))
         break;
       if ((rc = dcopy(c, a->seg, a->xd)) || (rc = dcopy(c, a->seg + N, a->vd))) break;
-      for (int m = s0 + 1; m <= s1 && rc == LJ_OK; ++m) {
-        if (!check(hipGraphLaunch(a->gexec, st))) break;
-        double4* slot = a->seg + (size_t)2 * (m - s0) * N;
-        if ((rc = dcopy(c, slot, a->xd)) || (rc = dcopy(c, slot + N, a->vd))) break;
+      if (s1 - s0 == k && a->gexec_seg && a->g_k == k) {
+        check(hipGraphLaunch(a->gexec_seg, st));  // k steps, each state into its slot
+      } else {
+        for (int m = s0 + 1; m <= s1 && rc == LJ_OK; ++m) {
+          if (!check(hipGraphLaunch(a->gexec, st))) break;
+          double4* slot = a->seg + (size_t)2 * (m - s0) * N;
+          if ((rc = dcopy(c, slot, a->xd)) || (rc = dcopy(c, slot + N, a->vd))) break;
+        }
       }
       if (rc || !check(hipEventRecord(ev[3], st)) || !check(hipEventSynchronize(ev[3]))) break;
       float ms = 0.f;
@@ -377,16 +471,45 @@ int lj_trajectory_grad(lj_ctx* c, double dt, int nsteps, int checkpoint_every, d
           if (!check(launch_adj_init(st, xm + N, N, gv, dt, a->u, a->lv, a->z))) break;
           first = false;
         }
-        if ((rc = adj_pass_d(c, xm, st)) != LJ_OK) break;
-        if (!check(launch_adj_theta(st, a->scal, c->cfg.eps, invM, cw, m >= 1, a->theta))) break;
-        if (m >= 1) {
-          const double4* vprev = (m - 1 >= 1) ? a->seg + (size_t)2 * (m - 1 - s0) * N + N : nullptr;
-          check(launch_adj_update(st, N, invM, cw, gv, dt, m - 1 == 0, a->hz, a->f, a->gw, vprev,
-                                  a->u, a->lv, a->z));
-        } else {
-          check(launch_adj_final(st, N, a->hz, a->u, a->dx0));
+        // the pass, then (fused into its reduction and finaliser) the update of
+        // u, lambda and z for state m - 1 (or dL/dx0 at m = 0) and the eps /
+        // sigma terms
+        AdjStep step;
+        step.mode = m >= 1 ? 1 : 2;
+        step.invM = invM;
+        step.cw = cw;
+        step.gv = gv;
+        step.dt = dt;
+        step.last = m - 1 == 0;
+        step.vel_prev = (m - 1 >= 1) ? a->seg + (size_t)2 * (m - 1 - s0) * N + N : nullptr;
+        step.u = a->u;
+        step.lv = a->lv;
+        step.dx0 = a->dx0;
+        step.eps = c->cfg.eps;
+        step.direct = m >= 1;
+        step.g = a->theta;
+        step.rowacc = m >= 1 ? a->rowacc : nullptr;
+        // the next state of this segment (the next segment's are recomputed
+        // first, which overwrites the coordinates)
+        if (m - 1 >= mlow) {
+          step.next_x = a->seg + (size_t)2 * (m - 1 - s0) * N;
+          step.qs = qscale_of(c);
+          step.q = c->q;
+          step.xf = a->xf;
         }
+        if ((rc = adj_pass_d(c, xm, !prepped, st, step)) != LJ_OK) break;
+        prepped = step.next_x != nullptr;
       }
+      prepped = false;
+    }
+    if (rc == LJ_OK) {
+      AdjStep fin;
+      fin.eps = c->cfg.eps;
+      fin.invM = invM;
+      fin.cw = cw;
+      fin.direct = 1;
+      fin.g = a->theta;
+      check(launch_adj_finalize(st, a->rowacc, a->nblk, make_adj_scales(c->ps), a->scal, fin));
     }
   }
   if (rc == LJ_OK && check(hipEventRecord(ev[3], st)) &&

@@ -672,7 +672,7 @@ int prepare_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, int accu
   a.est_scale = rule == LJ_RDF_RBINS ? (float)(nbins / rmax) : (float)(1.0 / dr2);
   std::vector<float> e(nbins + 1);
   rdf_bin_thresholds(rule, nbins, dr2, rmax, e.data());
-  a.far = rdf_far_test(a.periodic != 0, c->L, rule == LJ_RDF_REF_GPU ? e[nbins - 1] : e[nbins]);
+  a.end = rule == LJ_RDF_REF_GPU ? e[nbins - 1] : e[nbins];
   if (nbins + 1 > c->edges_cap) {
     LJ_HIP(hipStreamSynchronize(st));
     dev_free(c->edges2);
@@ -1005,7 +1005,7 @@ int compat_forces(float* Pos, float* Force, float* host_pressure, int N, float h
     a.tab = s->rdf_tab;
     a.est_scale = 1.f / host_dr2;
     gpu_minimage_thresholds((double)host_L, a.thr);
-    a.far = rdf_far_test(a.periodic != 0, (double)host_L, s->tab_last);
+    a.end = s->tab_last;
     a.hist = s->rdf;
     LJ_HIP(launch_rdf(st, a));
   }

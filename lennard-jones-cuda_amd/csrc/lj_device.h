@@ -140,8 +140,9 @@ __device__ __forceinline__ void j_range(const uint4* __restrict__ q, const float
 
 // ---------------------------------------------------------------------------
 // deterministic block reduction of NV doubles: butterfly inside the wave, then
-// the NW wave results in wave order.  Row `out` receives the block totals.
-template <int NV, int NW = kWaves>
+// the NW wave results in wave order.  Row `out` receives the block totals
+// (ACCUM: they are added to the row, which only this block touches).
+template <int NV, int NW = kWaves, bool ACCUM = false>
 __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out, const int (&field)[NV])
 {
   __shared__ double sh[NW][NV];
@@ -161,7 +162,10 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out, co
 #pragma unroll
     for (int k = 0; k < NV; ++k)
       if (k == (int)threadIdx.x) f = field[k];
-    out[f] = s;
+    if constexpr (ACCUM)
+      out[f] += s;
+    else
+      out[f] = s;
   }
 }
 

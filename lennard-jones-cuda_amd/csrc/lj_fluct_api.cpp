@@ -134,7 +134,7 @@ int lj_set_step_sampling(lj_ctx* c, const lj_step_sampling* s)
       gpu_minimage_thresholds(c->L, a.thr);
     else
       cpu_minimage_thresholds(c->L, a.thr);
-    a.far = rdf_far_test(a.periodic != 0, c->L, rule == LJ_RDF_REF_GPU ? e[nbins - 1] : e[nbins]);
+    a.end = rule == LJ_RDF_REF_GPU ? e[nbins - 1] : e[nbins];
     a.hist = c->rdf_step;
     LJ_HIP(launch_fill_u64(st, c->rdf_step, nbins));
     LJ_HIP(hipStreamSynchronize(st));

@@ -156,13 +156,6 @@ float first_float(float guess, P pred)
 }
 }  // namespace
 
-bool rdf_far_test(bool periodic, double L, double rmax2)
-{
-  if (!periodic || !(L > 0.) || !(rmax2 >= 0.)) return false;
-  const double r = std::sqrt(rmax2);
-  if (r >= 0.5 * L) return false;
-  return 4.18879020478639098 * r * r * r < 0.3 * L * L * L;  // (4/3) pi r^3 < 0.3 L^3
-}
 
 void gpu_minimage_thresholds(double L, float thr[4])
 {

@@ -160,13 +160,9 @@ struct RdfArgs {
   const float* tab;  // rdf_bin_thresholds (nbins + 1)
   float est_scale;   // bin estimate: r^2 est_scale (REF rules), sqrt(r^2) est_scale (RBINS)
   float thr[4];      // minimum-image thresholds of the rule
-  int far;           // reject far pairs first (rdf_far_test)
+  double end;        // the counted range's end, tab[nbins - 1] (REF_GPU) or tab[nbins]
   unsigned long long* hist;
 };
-// whether the RDF pass should reject far pairs by a fixed-point test first:
-// periodic boxes whose counted range (r^2 < rmax2) holds under 30% of the
-// pairs (the sphere of radius sqrt(rmax2) against the box volume)
-bool rdf_far_test(bool periodic, double L, double rmax2);
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);
 // speed histogram (MDSystem.cpp:651-694); hist is added to (clear it first)

@@ -21,6 +21,8 @@
 // Integrator (MDSystem.cpp:438-510), BCs (:406-436), observables (:325-359):
 // per-particle kernels with the reference's double/float rounding pattern and
 // deterministic FP64 block reductions.
+#include <cstdlib>
+
 #include "lj_device.h"
 
 namespace lj {
@@ -85,6 +87,76 @@ __global__ __launch_bounds__(kBlock) void k_force(const uint4* __restrict__ q,
       }
     }
   }
+}
+
+// The same pass with the j range of each slice split between the four waves
+// of a workgroup, which share one 64-particle i tile: grid (ceil(N / 64), S).
+// At small N the plain grid holds about one wave per SIMD or less (N = 2048:
+// 8 x 32 workgroups) and the pass was latency-bound; here every slice runs as
+// four waves.  Their sums are combined in LDS in wave order (deterministic)
+// and written to the same partial layout.
+template <bool PERIODIC, bool GUARDALL, bool COMP, bool TIES>
+__global__ __launch_bounds__(kBlock) void k_force_split(const uint4* __restrict__ q,
+                                                        const float4* __restrict__ pf,
+                                                        float4* __restrict__ fpart,
+                                                        float* __restrict__ apart, int N, int Npad,
+                                                        int jchunk, float C1, float thr)
+{
+  __shared__ float red[kWaves - 1][5][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = blockIdx.x * 64;
+  if (i0 >= N) return;  // a padding tile: the whole workgroup leaves
+  const int s = blockIdx.y;
+  const int js = s * jchunk;
+  const int je = min(N, js + jchunk);
+  const int jq = (jchunk + kWaves - 1) / kWaves;
+  const int j0 = min(je, js + wv * jq);
+  const int j1 = min(je, j0 + jq);
+  PosT<PERIODIC> pi[1];
+  int ii[1] = {i0 + lane};
+  Acc hi[1], lo[1];
+  pi[0] = load_pos<PERIODIC>(q, pf, min(ii[0], N - 1));
+  acc_zero(hi[0]);
+  acc_zero(lo[0]);
+  if constexpr (GUARDALL) {
+    j_range<PERIODIC, 2, 1, COMP, TIES>(q, pf, j0, j1, pi, ii, C1, thr, hi, lo);
+  } else {
+    const int d0 = max(j0, i0), d1 = min(j1, i0 + 64);
+    if (d0 >= d1) {
+      j_range<PERIODIC, 0, 1, COMP, TIES>(q, pf, j0, j1, pi, ii, C1, thr, hi, lo);
+    } else {
+      j_range<PERIODIC, 0, 1, COMP, TIES>(q, pf, j0, d0, pi, ii, C1, thr, hi, lo);
+      j_range<PERIODIC, 1, 1, COMP, TIES>(q, pf, d0, d1, pi, ii, C1, thr, hi, lo);
+      j_range<PERIODIC, 0, 1, COMP, TIES>(q, pf, d1, j1, pi, ii, C1, thr, hi, lo);
+    }
+  }
+  float v[5];
+  if constexpr (COMP) {
+    v[0] = hi[0].fx + lo[0].fx;
+    v[1] = hi[0].fy + lo[0].fy;
+    v[2] = hi[0].fz + lo[0].fz;
+    v[3] = hi[0].b + lo[0].b;
+    v[4] = hi[0].a + lo[0].a;
+  } else {
+    v[0] = hi[0].fx;
+    v[1] = hi[0].fy;
+    v[2] = hi[0].fz;
+    v[3] = hi[0].b;
+    v[4] = hi[0].a;
+  }
+  if (wv > 0)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[wv - 1][k][lane] = v[k];
+  __syncthreads();
+  if (wv != 0 || ii[0] >= N) return;
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int w = 0; w < kWaves - 1; ++w) v[k] += red[w][k][lane];
+  const size_t o = (size_t)s * Npad + ii[0];
+  store_part(fpart + o, v[0], v[1], v[2], v[3]);
+  store_part(apart + o, v[4]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_prep_q(const float4* __restrict__ pos,
@@ -462,49 +534,111 @@ __device__ __forceinline__ float mi_gpu1(float d, float Lf, float t0)
 // histogram per block and the j particles read from global memory, so that
 // the LDS stays within 64 KiB at the largest nbins.  Bin divisions and the
 // image rounding are table lookups (host-bisected thresholds), with the
-// reference's float/double arithmetic reproduced.
+// reference's float/double arithmetic reproduced.  Periodic boxes with SMALL
+// histograms take k_rdf_fx, which decides almost every pair from the
+// fixed-point image with a proven error bound and runs the reference
+// arithmetic only near bin edges; the others take k_rdf.
 constexpr int kRdfWaveCopy = 1024;
 
-// FAR (periodic, SMALL): a pair whose squared distance from the exact
-// fixed-point minimum image (the force pass's, q = x 2^32 / L) lies beyond the
-// counted range by a margin is dropped (REF_CPU, RBINS) or put in the last bin
-// (REF_GPU) without the reference arithmetic.  The two squared distances agree
-// to about 1e-6 relative plus 1e-6 L^2 (float positions, one fixed-point unit
-// L / 2^32 per axis); the margin is 1e-4 relative plus 1e-5 L^2, so no pair
-// changes bin; pairs inside or near the range take the exact path.
-// The lanes of a wave pair different i with one j, so the exact path runs
-// whenever one lane needs it: FAR pays off only when few pairs are in range
-// (the launcher enables it below a 30% in-range volume fraction).
-template <int RULE, bool PERIODIC, bool SMALL, bool FAR>
+// The reference arithmetic for one pair: image, r^2, bin (MDSystem.cpp:269-285
+// for REF_CPU and RBINS, MDSystem.cu:28-42 for REF_GPU).  Returns the bin, or
+// -1 for a pair that is not counted.
+template <int RULE, bool PERIODIC>
+__device__ __forceinline__ int rdf_exact_bin(const float4& pi, const float4& pj, double L, float Lf,
+                                             const float (&thr)[4], const float* T, int nbins,
+                                             float Tmax, float Tlast, float est_scale)
+{
+#pragma clang fp contract(off)
+  float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+  if constexpr (PERIODIC) {
+    // |r| < thr[1] on every axis (any position within half a box of the box):
+    // the image index is -1, 0 or 1, from the first threshold alone
+    const float am = fmaxf(fabsf(rx), fmaxf(fabsf(ry), fabsf(rz)));
+    if constexpr (RULE == LJ_RDF_REF_GPU) {
+      if (am < thr[1]) {
+        rx = mi_gpu1(rx, Lf, thr[0]);
+        ry = mi_gpu1(ry, Lf, thr[0]);
+        rz = mi_gpu1(rz, Lf, thr[0]);
+      } else {
+        rx = mi_gpu(rx, Lf, thr);
+        ry = mi_gpu(ry, Lf, thr);
+        rz = mi_gpu(rz, Lf, thr);
+      }
+    } else {
+      if (am < thr[1]) {
+        rx = mi_cpu1(rx, L, thr[0]);
+        ry = mi_cpu1(ry, L, thr[0]);
+        rz = mi_cpu1(rz, L, thr[0]);
+      } else {
+        rx = mi_cpu(rx, L, thr);
+        ry = mi_cpu(ry, L, thr);
+        rz = mi_cpu(rz, L, thr);
+      }
+    }
+  }
+  float r2;
+  if constexpr (RULE == LJ_RDF_REF_GPU) {
+    r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));  // MDSystem.cu:35
+    if (!(r2 > 1e-6f)) return -1;                                    // MDSystem.cu:37
+  } else {
+    r2 = rx * rx + ry * ry + rz * rz;  // float, no contraction (MDSystem.cpp:279)
+    if (!(r2 < Tmax)) return -1;       // beyond the last bin: dropped
+  }
+  if (r2 >= Tlast) return nbins - 1;  // the GPU rule clamps every larger r2 here
+  const float e = RULE == LJ_RDF_RBINS ? __builtin_amdgcn_sqrtf(r2) * est_scale : r2 * est_scale;
+  return rdf_bin(r2, T, nbins, (int)e);
+}
+
+// The block's counts, one unit per unordered pair (doubled into the ordered
+// counts when flushed), in one LDS histogram per wave, h[wave][bin].  Pairs of
+// the last bin go to a register first: with the GPU rule every pair beyond the
+// range is clamped there, which would put most lanes of a wave on one LDS
+// address.  (A 32-column histogram per workgroup, h[bin][lane & 31], whose
+// atomics never share a bank, measured slower on MI355X: 254 against 130 us
+// at 500 bins, N = 16384, from 2 workgroups per CU at 64 KiB.)
+__device__ __forceinline__ unsigned int* rdf_hist_init(unsigned int* hs, int nbins)
+{
+  for (int b = threadIdx.x; b < kWaves * nbins; b += kBlock) hs[b] = 0u;
+  return hs + (int)(threadIdx.x >> 6) * nbins;
+}
+
+// adds the block's counts (with the waves' last-bin registers) to hist
+__device__ __forceinline__ void rdf_flush(const unsigned int* hs, unsigned int* hw, int ncopy,
+                                          int nbins, unsigned int last,
+                                          unsigned long long* __restrict__ hist)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) last += __shfl_xor(last, o, 64);
+  if ((threadIdx.x & 63) == 0 && last) atomicAdd(&hw[nbins - 1], last);
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += kBlock) {
+    unsigned int c = 0u;
+    for (int k = 0; k < ncopy; ++k) c += hs[k * nbins + b];
+    if (c) atomicAdd(&hist[b], 2ull * c);
+  }
+}
+
+template <int RULE, bool PERIODIC, bool SMALL>
 __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, int N, double L,
                                                 int nbins, const float* __restrict__ tab,
                                                 float est_scale, float t0, float t1, float t2,
-                                                float t3, double qscale,
-                                                unsigned long long* __restrict__ hist)
+                                                float t3, unsigned long long* __restrict__ hist)
 {
-#pragma clang fp contract(off)
-  static_assert(!FAR || (PERIODIC && SMALL), "FAR: periodic boxes, small histograms");
   const int g = blockIdx.x, hgrp = blockIdx.y;
   if (hgrp < g) return;  // uniform, before any barrier
   extern __shared__ float4 sh_rdf4[];
-  constexpr int ncopy = SMALL ? kWaves : 1;
-  float4* pj_s = sh_rdf4;                                   // [kBlock] when SMALL
-  uint4* qj_s = reinterpret_cast<uint4*>(sh_rdf4 + kBlock);  // [kBlock] when FAR
-  unsigned int* hs = reinterpret_cast<unsigned int*>(sh_rdf4 + (SMALL ? kBlock : 0) + (FAR ? kBlock : 0));
-  float* T = reinterpret_cast<float*>(hs + ncopy * nbins);
-  unsigned int* hw = hs + (SMALL ? (int)(threadIdx.x >> 6) * nbins : 0);
-  for (int b = threadIdx.x; b < ncopy * nbins; b += kBlock) hs[b] = 0u;
+  float4* pj_s = sh_rdf4;  // [kBlock] when SMALL
+  unsigned int* hs = reinterpret_cast<unsigned int*>(sh_rdf4 + (SMALL ? kBlock : 0));
+  float* T = reinterpret_cast<float*>(hs + (SMALL ? kWaves : 1) * nbins);
+  unsigned int* hw = hs;
+  if constexpr (SMALL) hw = rdf_hist_init(hs, nbins);
+  else
+    for (int b = threadIdx.x; b < nbins; b += kBlock) hs[b] = 0u;
   for (int b = threadIdx.x; b <= nbins; b += kBlock) T[b] = tab[b];
   const int j0 = hgrp * kBlock;
   const int nj = min(kBlock, N - j0);
-  if constexpr (SMALL) {
-    if ((int)threadIdx.x < nj) {
-      const float4 p = pos[j0 + threadIdx.x];
-      pj_s[threadIdx.x] = p;
-      if constexpr (FAR)
-        qj_s[threadIdx.x] = make_uint4(fixq(p.x, qscale), fixq(p.y, qscale), fixq(p.z, qscale), 0u);
-    }
-  }
+  if constexpr (SMALL)
+    if ((int)threadIdx.x < nj) pj_s[threadIdx.x] = pos[j0 + threadIdx.x];
   const float4* PJ = SMALL ? pj_s : pos + j0;
   __syncthreads();
 
@@ -512,84 +646,248 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
   const bool valid = i < N;
   const float4 pi = pos[min(i, N - 1)];
   const bool diag = g == hgrp;
-  const float Lf = (float)L;
   const float thr[4] = {t0, t1, t2, t3};
-  const float Tmax = T[nbins];
-  const float Tlast = T[nbins - 1];
-  uint4 qi = make_uint4(0u, 0u, 0u, 0u);
-  if constexpr (FAR) qi = make_uint4(fixq(pi.x, qscale), fixq(pi.y, qscale), fixq(pi.z, qscale), 0u);
-  const float qs2 = (float)((L / 4294967296.) * (L / 4294967296.));  // (fixed-point unit)^2
-  const float far = (float)((double)(RULE == LJ_RDF_REF_GPU ? Tlast : Tmax) * (1. + 1e-4) + 1e-5 * L * L);
-  // pairs of the last bin are counted in a register: with the GPU rule every
-  // pair beyond the range is clamped into it, which would otherwise put most
-  // lanes of a wave on one LDS address
+  const float Tmax = T[nbins], Tlast = T[nbins - 1];
   unsigned int last = 0u;
-  // the reference arithmetic for pair (i, j0 + jj): image, r^2, bin
-  auto exact = [&](int jj) {
-    const float4 pj = PJ[jj];
-    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-    if constexpr (PERIODIC) {
-      // |r| < thr[1] on every axis (any position within half a box of the box):
-      // the image index is -1, 0 or 1, from the first threshold alone
-      const float am = fmaxf(fabsf(rx), fmaxf(fabsf(ry), fabsf(rz)));
-      if constexpr (RULE == LJ_RDF_REF_GPU) {
-        if (am < thr[1]) {
-          rx = mi_gpu1(rx, Lf, thr[0]);
-          ry = mi_gpu1(ry, Lf, thr[0]);
-          rz = mi_gpu1(rz, Lf, thr[0]);
-        } else {
-          rx = mi_gpu(rx, Lf, thr);
-          ry = mi_gpu(ry, Lf, thr);
-          rz = mi_gpu(rz, Lf, thr);
-        }
-      } else {
-        if (am < thr[1]) {
-          rx = mi_cpu1(rx, L, thr[0]);
-          ry = mi_cpu1(ry, L, thr[0]);
-          rz = mi_cpu1(rz, L, thr[0]);
-        } else {
-          rx = mi_cpu(rx, L, thr);
-          ry = mi_cpu(ry, L, thr);
-          rz = mi_cpu(rz, L, thr);
-        }
-      }
+#pragma unroll 4
+  for (int jj = 0; jj < nj; ++jj) {
+    if (!valid || (diag && jj <= (int)threadIdx.x)) continue;
+    const int bin =
+        rdf_exact_bin<RULE, PERIODIC>(pi, PJ[jj], L, (float)L, thr, T, nbins, Tmax, Tlast, est_scale);
+    if (bin == nbins - 1) ++last;
+    else if (bin >= 0) atomicAdd(&hw[bin], 1u);
+  }
+  rdf_flush(hs, hw, SMALL ? kWaves : 1, nbins, last, hist);  // !SMALL: the waves share one
+}
+
+// Periodic boxes, small histograms: every pair is first classified from the
+// exact fixed-point minimum image (the force pass's, q = x 2^32 / L).  Its
+// squared distance r2 (fixed-point units, S = 2^32 / L per length) differs
+// from the reference's (float positions, the rule's image and roundings, in
+// S^2 units) by at most
+//   delta = Kr r + K2 r2 + K0,
+//   Kr = 2 sqrt(3) (2 A + 3 L) u S,  K2 = 16 u,  K0 = 3 u^2 (2 A + 5 L)^2 S^2 + Kr^2,
+// u = 2^-24, r = sqrt(r2), A a bound of |x_i - x_j| per axis (here the span
+// of the coordinates of the pair's two particle groups).  Per axis the two
+// differences agree within e = u (2 A + 3 L + 2 |r_a|): the reference's
+// float difference rounds at u A, its image shift at u |r_a| (and the GPU
+// rule's float L is off by u L); the fixed-point difference is within L 2^-32
+// of the exact one and converts at u |r_a|; where the rule and the fixed
+// point take different images (|r_a| within u (A + L) of L/2), the two
+// magnitudes differ by at most 2 u (A + L) + 2 L 2^-32.  Squares differ by
+// e (2 |r_a| + e), sum_a |r_a| <= sqrt(3) r; the reference's r^2 rounds at
+// 3 u r2, ours at 3 u r2, the interval ends and the table's units at u r2
+// each (K2: 16 against 4 + 3 + 3 + 2 + 1); the Kr^2 in K0 covers evaluating
+// Kr r at the fixed-point r rather than the exact one.  Each constant is
+// taken 1% up.  A pair whose interval [r2 - delta, r2 + delta] lies inside
+// one bin (or wholly beyond the counted range) takes that bin without the
+// reference arithmetic; the others -- about 2 delta / bin width of the
+// pairs, 1e-3 at 500 r-bins -- are marked in a bit mask per lane and walked
+// through the reference arithmetic after each run of 32 j.
+//   RBINS (edges T[k] = fl((k dr)^2)): the bin is decided from x = r / dr
+// alone: k = floor(x) when x >= 1 and frac(x) lies in [m, 1 - m],
+// m = es Kr + es^2 K0 + (K2 + 6 u) x (es = 1 / (dr S)): the first terms are
+// delta / (2 r) in bins, the last covers T's rounding, the sqrt (1 ulp) and
+// the products forming x.  No table read.
+//   Other rules: k from the estimate, then T[k] <= r2 - delta and
+// r2 + delta < T[k + 1] from one read of (T[k], T[k + 1]).
+//   FAR (ranges that hold a small part of the box, as the reference rules'
+// r2 < 25.6 at liquid densities): a first pass over 32 j marks the pairs
+// that may lie in the range by one comparison of r2 with a bound X beyond
+// which r2 - delta >= the range's end (r2 - delta(r2) grows with r2;
+// X = end + 2 delta(sqrt(2 end)) <= 2 end); each lane then walks its marked
+// pairs through the classification.
+template <int RULE, bool FAR>
+__global__ __launch_bounds__(kBlock) void k_rdf_fx(const float4* __restrict__ pos, int N, double L,
+                                                   int nbins, const float* __restrict__ tab,
+                                                   float est_scale, float t0, float t1, float t2,
+                                                   float t3, double qscale,
+                                                   unsigned long long* __restrict__ hist)
+{
+  constexpr bool TAB = RULE != LJ_RDF_RBINS;  // the classification reads (T[k], T[k + 1])
+  const int g = blockIdx.x, hgrp = blockIdx.y;
+  if (hgrp < g) return;  // uniform, before any barrier
+  // the j group apart from the histograms, so that the loads of a batch of j
+  // may pass the atomics of the previous one
+  __shared__ float4 pj_s[kBlock];
+  __shared__ uint4 qj_s[kBlock];
+  __shared__ float bw[2][kWaves];
+  // dynamic LDS: T2 [nbins] when TAB: (T[k], T[k + 1]) S^2; T [nbins + 1]; the
+  // per-wave histograms
+  extern __shared__ float2 sh_rdf2[];
+  float2* T2 = sh_rdf2;
+  float* T = reinterpret_cast<float*>(T2 + (TAB ? nbins : 0));
+  unsigned int* hs = reinterpret_cast<unsigned int*>(T + nbins + 1);
+  const int wv = (int)(threadIdx.x >> 6);
+  unsigned int* hw = rdf_hist_init(hs, nbins);
+  const double S2 = qscale * qscale;
+  for (int b = threadIdx.x; b <= nbins; b += kBlock) {
+    const float t = tab[b];
+    T[b] = t;
+    if constexpr (TAB) {
+      const float tq = (float)((double)t * S2);
+      if (b < nbins) T2[b].x = tq;
+      if (b > 0) T2[b - 1].y = tq;
     }
-    float r2;
-    if constexpr (RULE == LJ_RDF_REF_GPU) {
-      r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));  // MDSystem.cu:35
-      if (!(r2 > 1e-6f)) return;                                       // MDSystem.cu:37
-    } else {
-      r2 = rx * rx + ry * ry + rz * rz;  // float, no contraction (MDSystem.cpp:279)
-      if (!(r2 < Tmax)) return;          // beyond the last bin: dropped
-    }
-    if (r2 >= Tlast) {
-      ++last;
-      return;
-    }
-    const float e = RULE == LJ_RDF_RBINS ? __builtin_amdgcn_sqrtf(r2) * est_scale : r2 * est_scale;
-    atomicAdd(&hw[rdf_bin(r2, T, nbins, (int)e)], 2u);
+  }
+  const int j0 = hgrp * kBlock;
+  const int nj = min(kBlock, N - j0);
+  float cmax = -3.0e38f, cmin = 3.0e38f;
+  if ((int)threadIdx.x < nj) {
+    const float4 p = pos[j0 + threadIdx.x];
+    pj_s[threadIdx.x] = p;
+    qj_s[threadIdx.x] = make_uint4(fixq(p.x, qscale), fixq(p.y, qscale), fixq(p.z, qscale), 0u);
+    cmax = fmaxf(p.x, fmaxf(p.y, p.z));
+    cmin = fminf(p.x, fminf(p.y, p.z));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    cmax = fmaxf(cmax, __shfl_xor(cmax, o, 64));
+    cmin = fminf(cmin, __shfl_xor(cmin, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    bw[0][wv] = cmax;
+    bw[1][wv] = cmin;
+  }
+  __syncthreads();
+
+  const int i = g * kBlock + threadIdx.x;
+  const float4 pi = pos[min(i, N - 1)];
+  const uint4 qi = make_uint4(fixq(pi.x, qscale), fixq(pi.y, qscale), fixq(pi.z, qscale), 0u);
+  // this lane's pairs: j > jlo (the diagonal group counts each pair once)
+  const int jlo = i < N ? (g == hgrp ? (int)threadIdx.x : -1) : kBlock;
+  const float thr[4] = {t0, t1, t2, t3};
+  const float Tmax = T[nbins], Tlast = T[nbins - 1];
+  // the upper end of the counted range, in fixed-point units: dropped
+  // (REF_CPU, RBINS) or the clamped last bin (REF_GPU) from there on; the GPU
+  // rule counts r2 > 1e-6 only
+  const double Tend_d = (double)(RULE == LJ_RDF_REF_GPU ? Tlast : Tmax) * S2;
+  const float Tend = (float)Tend_d;
+  const float Tlow = (float)(1e-6 * S2);
+  // the error bound's constants for this lane and block, in fixed-point units
+  float hmax = fmaxf(pi.x, fmaxf(pi.y, pi.z)), hmin = fminf(pi.x, fminf(pi.y, pi.z));
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    hmax = fmaxf(hmax, bw[0][w]);
+    hmin = fminf(hmin, bw[1][w]);
+  }
+  const double A = (double)hmax - (double)hmin;
+  const double u = 0x1p-24;
+  const double Krd = 1.01 * 3.4641016151377546 * (2. * A + 3. * L) * u * qscale;
+  const double K2d = 1.01 * 16. * u;
+  const double K0d = 1.01 * (3. * u * u * (2. * A + 5. * L) * (2. * A + 5. * L) * S2 + Krd * Krd);
+  const float Kr = (float)Krd, K2 = (float)K2d, K0 = (float)K0d;
+  const double esd = RULE == LJ_RDF_RBINS ? est_scale / qscale : est_scale / S2;
+  const float es = (float)esd;
+  // RBINS: m = M0 + M1 x (bins)
+  const float M0 = (float)(1.01 * (esd * Krd + esd * esd * K0d));
+  const float M1 = (float)(1.01 * (K2d + 6. * u));
+  const float kmax = (float)(nbins - 1), fnb = (float)nbins;
+  unsigned int last = 0u;
+
+  // the fixed-point r2 of pair (i, j)
+  auto r2_of = [&](const uint4& qj) {
+    const float dx = (float)(int)(qi.x - qj.x), dy = (float)(int)(qi.y - qj.y),
+                dz = (float)(int)(qi.z - qj.z);
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
   };
-  if constexpr (FAR) {
-    // 32 j at a time: the fixed-point test marks each lane's near pairs in a
-    // bit mask, then the lanes walk their own set bits, so the exact path runs
-    // as often as the busiest lane needs it, not once per j
-    for (int c0 = 0; c0 < nj; c0 += 32) {
-      const int cn = min(32, nj - c0);
-      unsigned int mask = 0u;
+  // the reference arithmetic for pair (i, jj), counted
+  auto exact = [&](int jj) {
+    const int bin = rdf_exact_bin<RULE, true>(pi, pj_s[jj], L, (float)L, thr, T, nbins, Tmax, Tlast,
+                                              est_scale);
+    if (bin == nbins - 1) ++last;
+    else if (bin >= 0) atomicAdd(&hw[bin], 1u);
+  };
+  // Classification state of one pair: k (the bin if decided), dec (decided
+  // and counted in bin k), gone (decided: not counted, or REF_GPU's clamp).
+  // RBINS, from r2 alone:
+  auto class_rb = [&](float r2, int& k, bool& dec, bool& gone) {
+    const float x = __builtin_amdgcn_sqrtf(r2) * es;
+    const float m = __builtin_fmaf(M1, x, M0);
+    const float f = __builtin_amdgcn_fractf(x);
+    k = (int)x;
+    dec = (x >= 1.f) & (f >= m) & (f <= 1.f - m) & (k < nbins);
+    gone = x - m >= fnb;
+  };
+  // the other rules: the bin estimate and interval; t = T2[k] is read between
+  auto measure_tab = [&](float r2, int& k, float& lo, float& hi) {
+    const float r = __builtin_bit_cast(float, (__builtin_bit_cast(unsigned, r2) >> 1) + 0x1fc00000u);
+    const float dl = __builtin_fmaf(Kr, r, __builtin_fmaf(K2, r2, K0));
+    lo = r2 - dl;
+    hi = r2 + dl;
+    k = (int)__builtin_amdgcn_fmed3f(r2 * es, 0.f, kmax);
+  };
+  auto decide_tab = [&](int k, float lo, float hi, float2 t, bool& dec, bool& gone) {
+    gone = lo >= Tend;
+    dec = (t.x <= lo) & (hi < t.y);
+    if constexpr (RULE == LJ_RDF_REF_GPU) dec = dec & (lo > Tlow) & (k < nbins - 1);
+  };
+  // one pair, decided or walked to the exact path at once (the walks)
+  auto one = [&](int jj) {
+    const float r2 = r2_of(qj_s[jj]);
+    int k;
+    bool dec, gone;
+    if constexpr (TAB) {
+      float lo, hi;
+      measure_tab(r2, k, lo, hi);
+      decide_tab(k, lo, hi, T2[k], dec, gone);
+    } else {
+      class_rb(r2, k, dec, gone);
+    }
+    if (dec) atomicAdd(&hw[k], 1u);
+    else if (gone) {
+      if constexpr (RULE == LJ_RDF_REF_GPU) ++last;
+    } else {
+      exact(jj);
+    }
+  };
+
+  for (int c0 = 0; c0 < nj; c0 += 32) {  // kBlock is a multiple of 32
+    unsigned int mask = 0u;
+    if constexpr (FAR) {
+      const double dl2 = Krd * __builtin_sqrt(2. * Tend_d) + K2d * 2. * Tend_d + K0d;
+      const float X = (float)((Tend_d + 2. * dl2) * (1. + 0x1p-20));
 #pragma unroll 8
-      for (int b = 0; b < cn; ++b) {
+      for (int b = 0; b < 32; ++b) {
         const int jj = c0 + b;
-        const uint4 qj = qj_s[jj];
-        const float dx = (float)(int)(qi.x - qj.x), dy = (float)(int)(qi.y - qj.y),
-                    dz = (float)(int)(qi.z - qj.z);
-        const float r2q = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) * qs2;
-        const bool ok = valid && !(diag && jj <= (int)threadIdx.x);
-        if (ok) {
-          if (r2q > far) {
-            if constexpr (RULE == LJ_RDF_REF_GPU) ++last;
-          } else {
-            mask |= 1u << b;
+        const float r2 = r2_of(qj_s[jj]);
+        const bool ok = (jj > jlo) & (jj < nj);
+        const bool far = r2 >= X;
+        if constexpr (RULE == LJ_RDF_REF_GPU) last += (ok & far) ? 1u : 0u;
+        mask |= (ok & !far) ? (1u << b) : 0u;
+      }
+      while (mask) {
+        const int b = __builtin_ctz(mask);
+        mask &= mask - 1u;
+        one(c0 + b);
+      }
+    } else {
+      constexpr int B = 8;
+#pragma unroll 1
+      for (int b0 = 0; b0 < 32; b0 += B) {
+        int kk[B];
+        bool dec[B], gone[B];
+        if constexpr (TAB) {
+          float lo[B], hi[B];
+          float2 t[B];
+#pragma unroll
+          for (int b = 0; b < B; ++b) {
+            measure_tab(r2_of(qj_s[c0 + b0 + b]), kk[b], lo[b], hi[b]);
+            t[b] = T2[kk[b]];
           }
+#pragma unroll
+          for (int b = 0; b < B; ++b) decide_tab(kk[b], lo[b], hi[b], t[b], dec[b], gone[b]);
+        } else {
+#pragma unroll
+          for (int b = 0; b < B; ++b) class_rb(r2_of(qj_s[c0 + b0 + b]), kk[b], dec[b], gone[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          const int jj = c0 + b0 + b;
+          const bool ok = (jj > jlo) & (jj < nj);
+          if constexpr (RULE == LJ_RDF_REF_GPU) last += (ok & gone[b]) ? 1u : 0u;
+          if (ok & dec[b]) atomicAdd(&hw[kk[b]], 1u);
+          mask |= (ok & !dec[b] & !gone[b]) ? (1u << (b0 + b)) : 0u;
         }
       }
       while (mask) {
@@ -598,23 +896,8 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
         exact(c0 + b);
       }
     }
-  } else {
-#pragma unroll 4
-    for (int jj = 0; jj < nj; ++jj) {
-      if (!valid || (diag && jj <= (int)threadIdx.x)) continue;
-      exact(jj);
-    }
   }
-  // the wave's last-bin count: one LDS atomic per wave
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) last += __shfl_xor(last, o, 64);
-  if ((threadIdx.x & 63) == 0 && last) atomicAdd(&hw[nbins - 1], 2u * last);
-  __syncthreads();
-  for (int b = threadIdx.x; b < nbins; b += kBlock) {
-    unsigned int c = 0u;
-    for (int k = 0; k < ncopy; ++k) c += hs[k * nbins + b];
-    if (c) atomicAdd(&hist[b], (unsigned long long)c);
-  }
+  rdf_flush(hs, hw, kWaves, nbins, last, hist);
 }
 
 // speed histogram of MDSystem::initvelo / updatevelo (MDSystem.cpp:651-694):
@@ -647,10 +930,32 @@ __global__ void k_fill_u64(unsigned long long* p, int n)
 
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
 
+// the j-split pass serves one-particle-per-lane plans whose plain grid holds
+// under four waves per SIMD (1024 SIMDs); LJ_FORCE_SPLIT=0/1 forces it off or on
+bool force_split_waves(const ForcePlan& pl)
+{
+  static const char* e = std::getenv("LJ_FORCE_SPLIT");
+  if (pl.ipl != 1 || pl.n3) return false;
+  if (e && e[0] == '0') return false;
+  if (e && e[0] == '1') return true;
+  return (long)pl.gx * kWaves * pl.S < 4L * 1024;
+}
+
 template <bool PERIODIC, bool GUARDALL, int IPL, bool COMP, bool TIES>
 hipError_t launch_force_t(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                           const uint4* q, const float4* posf, float4* fpart, float* apart)
 {
+  if constexpr (IPL == 1) {
+    if (force_split_waves(pl)) {
+      // tiles padded to a multiple of the 8 XCDs: tile t is written on XCD
+      // t mod 8 by every slice's workgroup, as the 64-particle row t of the
+      // integrator kernels that read its partials is scheduled (round robin)
+      dim3 grid((unsigned)((((pl.N + 63) / 64) + 7) / 8 * 8), pl.S);
+      hipLaunchKernelGGL((k_force_split<PERIODIC, GUARDALL, COMP, TIES>), grid, dim3(kBlock), 0, st,
+                         q, posf, fpart, apart, pl.N, pl.Npad, pl.jchunk, ps.C1, ps.thr);
+      return hipGetLastError();
+    }
+  }
   dim3 grid(pl.gx, pl.S);
   hipLaunchKernelGGL((k_force<PERIODIC, GUARDALL, IPL, COMP, TIES>), grid, dim3(kBlock), 0, st, q,
                      posf, fpart, apart, pl.N, pl.Npad, pl.jchunk, ps.C1, ps.thr);
@@ -889,23 +1194,38 @@ hipError_t launch_rdf(hipStream_t st, const RdfArgs& a)
 {
   const int ng = nblocks(a.N);
   const bool small = a.nbins <= kRdfWaveCopy;
-  const bool far = small && a.periodic && a.far;
-  const size_t lds = (small ? sizeof(float4) * kBlock + sizeof(unsigned) * kWaves * a.nbins
-                            : sizeof(unsigned) * a.nbins) +
-                     (far ? sizeof(uint4) * kBlock : 0) + sizeof(float) * (a.nbins + 1);
   const dim3 grid(ng, ng), blk(kBlock);
+  const size_t tab = sizeof(float) * (a.nbins + 1);
+  if (small && a.periodic) {
+    const size_t lds = (a.rule != LJ_RDF_RBINS ? sizeof(float2) * a.nbins : 0) + tab +
+                       sizeof(unsigned) * kWaves * a.nbins;
+    // the far-pair pass where the counted range holds under 30% of the box
+    // (a sphere of radius sqrt(end) against L^3)
+    const double re = std::sqrt(std::fmax(a.end, 0.));
+    const bool far = re < 0.5 * a.L && 4.18879020478639098 * re * re * re < 0.3 * a.L * a.L * a.L;
+#define LJ_RDF_FX_ARGS a.pos, a.N, a.L, a.nbins, a.tab, a.est_scale, a.thr[0], a.thr[1], a.thr[2], \
+                       a.thr[3], 4294967296. / a.L, a.hist
+#define LJ_RDF_FX(R)                                                                       \
+  do {                                                                                     \
+    if (far) hipLaunchKernelGGL((k_rdf_fx<R, true>), grid, blk, lds, st, LJ_RDF_FX_ARGS); \
+    else hipLaunchKernelGGL((k_rdf_fx<R, false>), grid, blk, lds, st, LJ_RDF_FX_ARGS);    \
+  } while (0)
+    if (a.rule == LJ_RDF_REF_CPU) LJ_RDF_FX(LJ_RDF_REF_CPU);
+    else if (a.rule == LJ_RDF_REF_GPU) LJ_RDF_FX(LJ_RDF_REF_GPU);
+    else LJ_RDF_FX(LJ_RDF_RBINS);
+#undef LJ_RDF_FX
+#undef LJ_RDF_FX_ARGS
+    return hipGetLastError();
+  }
+  const size_t lds =
+      (small ? sizeof(float4) * kBlock + sizeof(unsigned) * kWaves * a.nbins : sizeof(unsigned) * a.nbins) + tab;
 #define LJ_RDF_ARGS a.pos, a.N, a.L, a.nbins, a.tab, a.est_scale, a.thr[0], a.thr[1], a.thr[2], \
-                    a.thr[3], 4294967296. / a.L, a.hist
+                    a.thr[3], a.hist
 #define LJ_RDF_LAUNCH(R)                                                                             \
   do {                                                                                               \
-    if (far) hipLaunchKernelGGL((k_rdf<R, true, true, true>), grid, blk, lds, st, LJ_RDF_ARGS);     \
-    else if (small) {                                                                                \
-      if (a.periodic) hipLaunchKernelGGL((k_rdf<R, true, true, false>), grid, blk, lds, st, LJ_RDF_ARGS); \
-      else hipLaunchKernelGGL((k_rdf<R, false, true, false>), grid, blk, lds, st, LJ_RDF_ARGS);     \
-    } else {                                                                                         \
-      if (a.periodic) hipLaunchKernelGGL((k_rdf<R, true, false, false>), grid, blk, lds, st, LJ_RDF_ARGS); \
-      else hipLaunchKernelGGL((k_rdf<R, false, false, false>), grid, blk, lds, st, LJ_RDF_ARGS);    \
-    }                                                                                                \
+    if (small) hipLaunchKernelGGL((k_rdf<R, false, true>), grid, blk, lds, st, LJ_RDF_ARGS);        \
+    else if (a.periodic) hipLaunchKernelGGL((k_rdf<R, true, false>), grid, blk, lds, st, LJ_RDF_ARGS); \
+    else hipLaunchKernelGGL((k_rdf<R, false, false>), grid, blk, lds, st, LJ_RDF_ARGS);             \
   } while (0)
   if (a.rule == LJ_RDF_REF_CPU) LJ_RDF_LAUNCH(LJ_RDF_REF_CPU);
   else if (a.rule == LJ_RDF_REF_GPU) LJ_RDF_LAUNCH(LJ_RDF_REF_GPU);

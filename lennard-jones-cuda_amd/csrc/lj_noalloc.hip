@@ -51,7 +51,7 @@ namespace {
 struct NoAllocDevice {
   unsigned char* ptr = nullptr;
   float tab_dr2 = 0.f;
-  float tab_last = 0.f;
+  float tab_last = 0.f;  // the table's last-bin threshold (host copy)
 };
 std::mutex g_mu;
 std::vector<NoAllocDevice> g_dev;
@@ -131,7 +131,7 @@ int noalloc_forces(float* Pos, float* Force, float* host_pressure, int N, float 
     a.tab = tab;
     a.est_scale = 1.f / host_dr2;
     gpu_minimage_thresholds((double)host_L, a.thr);
-    a.far = rdf_far_test(a.periodic != 0, (double)host_L, d->tab_last);
+    a.end = d->tab_last;
     a.hist = hist;
     LJ_HIP(launch_rdf(st, a));
   }

@@ -125,3 +125,32 @@ def test_precision_workload_first_steps_vs_oracle():
           f"V {hist[-1, 3]:.12g} vs {obs[-1, 0]:.12g}")
     assert np.max(np.abs(x - xo)) <= 1e-11 and dF <= 1e-12
     assert np.all(np.abs(hist[:, 3] - obs[:, 0]) <= 1e-12 * np.abs(obs[:, 0]))
+
+
+def test_precision_workload_modes_full_size():
+    """FP32 and compensated against the FP64 mode on the precision study's
+    workload at full size (N = 131072, rho* = 0.3, hard wall, EVN,
+    dt = 0.002), over its first 100 steps from one seeded state: the first
+    state's forces, E_pot and P over every step.  Measured on MI355X: forces
+    7.7e-7 / 4.6e-7 (global), E_pot 3.8e-7 / 2.4e-7, P 1.1e-5 / 8.7e-6; the
+    tolerances are about 3x those."""
+    import types
+
+    from ljamd import precision as pr
+
+    a = types.SimpleNamespace(N=131072, rho=0.3, T0=1.0, dt=0.002, steps=100, chunk=100, seed=5)
+    print(f"\n[case] precision workload N={a.N}, 3 modes x {a.steps} steps", flush=True)
+    runs = {name: pr.run_mode(name, prec, a, lambda msg: None) for name, prec in pr.MODES}
+    ref = runs["fp64"]
+    errs = {}
+    for name in ("fp32", "compensated"):
+        r = runs[name]
+        fe = force_errors(r["F0"], ref["F0"])["global_rel"]
+        dE = np.max(np.abs(r["hist"][:, 3] - ref["hist"][:, 3]) / np.abs(ref["hist"][:, 3]))
+        dP = np.max(np.abs(r["hist"][:, 4] - ref["hist"][:, 4]) / np.abs(ref["hist"][:, 4]))
+        errs[name] = fe
+        print(f"  {name}: forces {fe:.2e}  E_pot {dE:.2e}  P {dP:.2e}  "
+              f"force pass {r['force_us']:.0f} us (fp64 {ref['force_us']:.0f} us)")
+        assert fe <= 2.5e-6 and dE <= 1.2e-6 and dP <= 3.5e-5
+    # the compensated per-particle sums are the more accurate ones
+    assert errs["compensated"] <= errs["fp32"]
