@@ -413,6 +413,8 @@ def main():
                     help="particles of the --profile-force workload (default: the headline N)")
     ap.add_argument("--profile-rdf", action="store_true",
                     help="only the RDF passes of the rdf leg (for rocprofv3)")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="only the batched-ensemble leg")
     ap.add_argument("--profile-precision", action="store_true",
                     help="only the precision modes' leg")
     ap.add_argument("--profile-adjoint", action="store_true",
@@ -428,6 +430,9 @@ def main():
         return
     if a.profile_rdf:
         print(json.dumps({"profile": "rdf", "N": N, **rdf_leg()}))
+        return
+    if a.profile_batch:
+        print(json.dumps({"profile": "batch", **batch_leg(a.batch_steps)}))
         return
     if a.profile_precision:
         print(json.dumps({"profile": "precision", **precision_leg()}))

@@ -391,9 +391,11 @@ typedef struct lj_batch lj_batch;
  *   jchunk         j particles per work item (waves per box = tiles x slices)
  *   blocks_per_cu  persistent grid of blocks_per_cu workgroups per CU pulling
  *                  items from a queue; 0 = one wave per item (no queue)
- *   pair_once      1: each unordered pair once, one 4-wave workgroup per pair
- *                  of 256-particle groups of a box (ipl, jchunk and
- *                  blocks_per_cu are then unused); 0: the tiling above */
+ *   pair_once      1: each unordered pair once, items = pairs of 256-particle
+ *                  groups of a box, run by 4-wave workgroups: one per item
+ *                  (blocks_per_cu 0) or a persistent grid of blocks_per_cu
+ *                  workgroups per CU draining the item queue (ipl and jchunk
+ *                  are then unused); 0: the tiling above */
 typedef struct lj_batch_tuning {
   int ipl;
   int jchunk;

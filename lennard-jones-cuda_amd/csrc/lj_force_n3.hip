@@ -39,6 +39,7 @@
 // The time-step variant's loop is 24 VALU instructions per unordered pair
 // (12.4 per ordered pair over the whole kernel, measured) against 21.5 per
 // ordered pair for the sliced kernel, which evaluates each pair twice.
+#include <algorithm>
 #include <cstdlib>
 
 #include "lj_device.h"
@@ -659,19 +660,45 @@ __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8)))
 // batched ensemble: item = (box, g, h) over all boxes; each box's partials at
 // its own rows (pbase, S = groups of 256, Npad); ENERGY: per-particle energies
 // kept (single evaluations), else the time step's variant with the item's
-// (E, W) totals at ew[2 e]
+// (E, W) totals at ew[2 e].
+// PERSIST: a grid of resident workgroups drains the item queue (longest items
+// first): each takes the next item from the queue head until the head passes
+// nitems, which every workgroup reaches; the head is reset by the kernel that
+// precedes every force pass (prep, drift, or the previous step's kick).
+// Otherwise workgroup b runs item b.
 template <bool PERIODIC, bool TIES, bool ENERGY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_batch_force_n3(
-    const uint4* __restrict__ q, const float4* __restrict__ pf, float4* __restrict__ fpart,
-    float* __restrict__ apart, const BoxParam* __restrict__ box, const BatchItem* __restrict__ items,
-    double* __restrict__ ew)
+__device__ __forceinline__ void batch_n3_item(N3Lds<2>& lds, const uint4* __restrict__ q,
+                                              const float4* __restrict__ pf,
+                                              float4* __restrict__ fpart, float* __restrict__ apart,
+                                              const BoxParam* __restrict__ box,
+                                              const BatchItem& w, double* __restrict__ ew)
 {
-  __shared__ N3Lds<2> lds;
-  const BatchItem w = items[blockIdx.x];
   const BoxParam bp = box[w.box];
   n3_item<PERIODIC, TIES, 2, ENERGY>(lds, q + bp.off, pf + bp.off, fpart + bp.pbase, apart + bp.pbase,
                                      ENERGY ? nullptr : ew + 2 * (size_t)w.e, bp.N, bp.Npad, bp.C1,
                                      w.i0, w.s);
+}
+
+template <bool PERIODIC, bool TIES, bool ENERGY, bool PERSIST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_batch_force_n3(
+    const uint4* __restrict__ q, const float4* __restrict__ pf, float4* __restrict__ fpart,
+    float* __restrict__ apart, const BoxParam* __restrict__ box, const BatchItem* __restrict__ items,
+    double* __restrict__ ew, unsigned* __restrict__ counter, int nitems)
+{
+  __shared__ N3Lds<2> lds;
+  if constexpr (PERSIST) {
+    __shared__ unsigned s_item;
+    for (;;) {
+      if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+      __syncthreads();  // also: every thread is done with the previous item's LDS
+      const unsigned it = s_item;
+      __syncthreads();  // s_item read by all before thread 0 writes it again
+      if (it >= (unsigned)nitems) break;  // uniform in the workgroup
+      batch_n3_item<PERIODIC, TIES, ENERGY>(lds, q, pf, fpart, apart, box, items[it], ew);
+    }
+  } else {
+    batch_n3_item<PERIODIC, TIES, ENERGY>(lds, q, pf, fpart, apart, box, items[blockIdx.x], ew);
+  }
 }
 
 }  // namespace
@@ -804,29 +831,32 @@ hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& p
 
 }  // namespace
 
+template <bool PERIODIC, bool TIES, bool ENERGY>
+void launch_batch_n3_t(hipStream_t st, const BatchArgs& a)
+{
+  const int pg = a.persistent_grid > 0 ? std::min(a.persistent_grid, a.nitems) : 0;
+  if (pg > 0)
+    hipLaunchKernelGGL((k_batch_force_n3<PERIODIC, TIES, ENERGY, true>), dim3(pg), dim3(256), 0, st,
+                       a.q, a.pos, a.fpart, a.apart, a.box, a.items, a.ew, a.counter, a.nitems);
+  else
+    hipLaunchKernelGGL((k_batch_force_n3<PERIODIC, TIES, ENERGY, false>), dim3(a.nitems),
+                       dim3(256), 0, st, a.q, a.pos, a.fpart, a.apart, a.box, a.items, a.ew,
+                       a.counter, a.nitems);
+}
+
 hipError_t launch_batch_force_n3(hipStream_t st, const BatchArgs& a, bool periodic, bool ties)
 {
-  dim3 grid(a.nitems), block(256);
   if (a.lean) {  // the time step (no exact-tie rule there)
     if (ties || !a.ew) return hipErrorInvalidValue;
-    if (periodic)
-      hipLaunchKernelGGL((k_batch_force_n3<true, false, false>), grid, block, 0, st, a.q, a.pos,
-                         a.fpart, a.apart, a.box, a.items, a.ew);
-    else
-      hipLaunchKernelGGL((k_batch_force_n3<false, false, false>), grid, block, 0, st, a.q, a.pos,
-                         a.fpart, a.apart, a.box, a.items, a.ew);
+    if (periodic) launch_batch_n3_t<true, false, false>(st, a);
+    else launch_batch_n3_t<false, false, false>(st, a);
     return hipGetLastError();
   }
   if (periodic) {
-    if (ties)
-      hipLaunchKernelGGL((k_batch_force_n3<true, true, true>), grid, block, 0, st, a.q, a.pos,
-                         a.fpart, a.apart, a.box, a.items, a.ew);
-    else
-      hipLaunchKernelGGL((k_batch_force_n3<true, false, true>), grid, block, 0, st, a.q, a.pos,
-                         a.fpart, a.apart, a.box, a.items, a.ew);
+    if (ties) launch_batch_n3_t<true, true, true>(st, a);
+    else launch_batch_n3_t<true, false, true>(st, a);
   } else {
-    hipLaunchKernelGGL((k_batch_force_n3<false, false, true>), grid, block, 0, st, a.q, a.pos,
-                       a.fpart, a.apart, a.box, a.items, a.ew);
+    launch_batch_n3_t<false, false, true>(st, a);
   }
   return hipGetLastError();
 }

@@ -50,7 +50,7 @@ def test_batch_forces_vs_oracle(tuning):
 def test_batch_tilings_agree_and_persistent_is_bitwise():
     N, rho, T0 = _small_batch(seed=5)
     outs = {}
-    for tuning in [(1, 512, 0), (1, 512, 8), (2, 1024, 4), (1, 512, 0, 1)]:
+    for tuning in [(1, 512, 0), (1, 512, 8), (2, 1024, 4), (1, 512, 0, 1), (1, 512, 8, 1)]:
         with ljamd.LJBatch(N, rho, T0, seed=11) as b:
             b.set_tuning(*tuning)
             b.compute_forces()
@@ -63,6 +63,27 @@ def test_batch_tilings_agree_and_persistent_is_bitwise():
     assert force_errors(F2, F1)["global_rel"] <= 2e-6
     F3 = xyz(outs[(1, 512, 0, 1)])
     assert force_errors(F3, F1)["global_rel"] <= 2e-6
+    # pair-once items drained by a persistent grid: the same items and slots
+    assert np.array_equal(outs[(1, 512, 0, 1)], outs[(1, 512, 8, 1)])
+
+
+def test_batch_pair_once_persistent_trajectory_bitwise():
+    """The time step with the pair-once items run one workgroup each and
+    drained by a persistent grid (the queue head reset by each step's kick):
+    the same bits after 20 steps, EVN and TVN."""
+    N, rho, T0 = _small_batch(nb=8, seed=29)
+    for canon in (ljamd.EVN, ljamd.TVN):
+        outs = []
+        for tuning in [(1, 512, 0, 1), (1, 512, 8, 1), (1, 512, 1, 1)]:
+            with ljamd.LJBatch(N, rho, T0, canonical=canon, seed=4) as b:
+                b.set_tuning(*tuning)
+                b.integrate(0.005, 20)
+                obs = b.observables()
+                outs.append((b.get_array(ljamd.ARRAY_POS), b.get_array(ljamd.ARRAY_VEL),
+                             [[o[k] for k in ("U", "V", "P")] for o in obs]))
+        for o in outs[1:]:
+            for x, y in zip(outs[0], o):
+                assert np.array_equal(np.asarray(x), np.asarray(y)), canon
 
 
 @pytest.mark.parametrize("canon", [ljamd.EVN, ljamd.TVN])
@@ -124,7 +145,8 @@ def test_batch_pair_once_step_totals(canon, bc):
 
 
 @pytest.mark.parametrize("canon,tuning", [(ljamd.EVN, (1, 512, 8, 0)), (ljamd.EVN, (1, 512, 0, 1)),
-                                          (ljamd.TVN, (1, 512, 0, 1)), (ljamd.TVN, (2, 1024, 4, 0))])
+                                          (ljamd.TVN, (1, 512, 0, 1)), (ljamd.TVN, (2, 1024, 4, 0)),
+                                          (ljamd.TVN, (1, 512, 8, 1))])
 def test_batch_graph_replay_deterministic(canon, tuning):
     """20 steps as 8 + 8 + 4 x 1 (8-step graphs whose kick kernels do the next
     drift and reset the work queue) and as 7 x 1 + (8 + 5 x 1): the same bits
