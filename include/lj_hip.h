@@ -186,6 +186,16 @@ float lj_rdf_dr2_default(int N);
  * (MDSystem.cpp:179-180).  Computed on the host, uploaded; then forces and
  * observables are evaluated as MDSystem::Reinitialize does (MDSystem.cpp:101-103). */
 int lj_sample_initial_state(lj_ctx* ctx, uint64_t seed);
+/* The same with a choice of velocity distribution:
+ *   LJ_INIT_GAUSSIAN  as lj_sample_initial_state;
+ *   LJ_INIT_MAXWELL   the reference's own sampler (MDSystem.cpp:36-53,170-177):
+ *                     Maxwell speeds sqrt(2 T0) t, t drawn by the reference's
+ *                     rejection rule (density ~ t^2 exp(-t^2)), directions
+ *                     with cos(theta) and phi uniform, from two SplitMix64
+ *                     streams of the seed instead of the clock-seeded MTRand;
+ * then the momentum is removed and T rescaled to exactly T0 (MDSystem.cpp:179-180). */
+enum { LJ_INIT_GAUSSIAN = 0, LJ_INIT_MAXWELL = 1 };
+int lj_sample_initial_state_ex(lj_ctx* ctx, uint64_t seed, int velocities);
 
 /* host float[4N] <-> device state           MDSystem.cpp:586-630 */
 int lj_set_array(lj_ctx* ctx, int type, const float* host4N);

@@ -12,7 +12,10 @@
  *     observables, as the reference's GPU branch leaves them; setHostSync(false)
  *     skips the array copies (fields are still refreshed);
  *   - initial velocities come from a seeded generator (m_config.seed), not
- *     from the clock-seeded MTRand (MersenneTwister.h:262);
+ *     from the clock-seeded MTRand (MersenneTwister.h:262): by default the
+ *     benchmark configurations' Gaussian draw, or with
+ *     m_config.initialVelocities = LJ_INIT_MAXWELL the reference's own
+ *     Maxwell-speed rejection sampler with isotropic directions;
  *   - setArray writes the device state, which here is authoritative (on the
  *     reference's GPU branch the next CalculateForces overwrites it from h_Pos);
  *   - NdNdr2 is refreshed by CalculateForces() and, as in the reference,
@@ -58,9 +61,13 @@ class MDSystem {
     int precision;           // LJ_PREC_*
     int device;
     uint64_t seed;
+    // LJ_INIT_GAUSSIAN (the benchmark configurations' initial velocities) or
+    // LJ_INIT_MAXWELL (the reference class's own sampler, MDSystem.cpp:147-181)
+    int initialVelocities;
     MDSystemConfiguration()
         : N(128), T0(1.5), rho(0.2), canonical(false), boundaryConditions(0), useCUDA(true),
-          CUDABlockSize(256), eps(1.0), sigma(1.0), precision(LJ_PREC_FP32), device(0), seed(1)
+          CUDABlockSize(256), eps(1.0), sigma(1.0), precision(LJ_PREC_FP32), device(0), seed(1),
+          initialVelocities(LJ_INIT_GAUSSIAN)
     {
     }
   };
@@ -130,6 +137,7 @@ class MDSystem {
     step_rdf_on_ = false;
     if (rdf_every_step_ && config.precision != LJ_PREC_FP64) enable_step_rdf(true);
     seed_ = config.seed;
+    init_velocities_ = config.initialVelocities;
     SampleInitialConditions();
     CalculateForces();
     CalculateParameters();
@@ -138,11 +146,11 @@ class MDSystem {
     resetAveraging();
   }
 
-  // MDSystem.cpp:147-181 (lattice, then seeded Gaussian velocities with the
-  // momentum removed and T rescaled to T0)
+  // MDSystem.cpp:147-181 (lattice, then seeded Gaussian or Maxwell velocities,
+  // m_config.initialVelocities, with the momentum removed and T rescaled to T0)
   void SampleInitialConditions()
   {
-    check(lj_sample_initial_state(ctx_, seed_++));
+    check(lj_sample_initial_state_ex(ctx_, seed_++, init_velocities_));
     t = 0;
     pull_fields();
     pull_arrays();
@@ -389,6 +397,7 @@ class MDSystem {
   bool step_rdf_on_ = false;
   double fl_fraction_ = -1.;  // fraction the device counter is set up for
   uint64_t seed_ = 1;
+  int init_velocities_ = LJ_INIT_GAUSSIAN;
 
   void enable_step_rdf(bool on)
   {

@@ -586,11 +586,19 @@ int lj_reset_averaging(lj_ctx* c)
 
 int lj_sample_initial_state(lj_ctx* c, uint64_t seed)
 {
+  return lj_sample_initial_state_ex(c, seed, LJ_INIT_GAUSSIAN);
+}
+
+int lj_sample_initial_state_ex(lj_ctx* c, uint64_t seed, int velocities)
+{
   LJ_TRY(ctx_setdev(c));
+  if (velocities != LJ_INIT_GAUSSIAN && velocities != LJ_INIT_MAXWELL)
+    return fail(LJ_ERR_ARG, "lj_sample_initial_state_ex: velocities must be LJ_INIT_GAUSSIAN or LJ_INIT_MAXWELL");
   const int N = c->cfg.N;
   std::vector<float> p((size_t)4 * N), v((size_t)4 * N);
   lattice_positions(N, c->L, p.data());
-  gaussian_velocities(N, c->cfg.T0, seed, v.data());
+  if (velocities == LJ_INIT_MAXWELL) maxwell_velocities(N, c->cfg.T0, seed, v.data());
+  else gaussian_velocities(N, c->cfg.T0, seed, v.data());
   LJ_TRY(lj_set_array(c, LJ_ARRAY_POS, p.data()));
   LJ_TRY(lj_set_array(c, LJ_ARRAY_VEL, v.data()));
   LJ_HIP(launch_reset_clock(c->stream, c->obs));

@@ -70,6 +70,37 @@ void lattice_positions(int N, double L, float* pos4)  // MDSystem.cpp:150-168
   }
 }
 
+namespace {
+// CorrectTotalMomentum (MDSystem.cpp:183-202), then RenormalizeVelocities(true)
+// (MDSystem.cpp:375-385, KineticTemperature :361-373), on float velocities
+void momentum_and_temperature(int N, double T0, float* vel4)
+{
+  double px = 0., py = 0., pz = 0., mass = 0.;
+  for (int k = 0; k < N; ++k) {
+    px += vel4[4 * k];
+    py += vel4[4 * k + 1];
+    pz += vel4[4 * k + 2];
+    mass += 1.;
+  }
+  for (int k = 0; k < N; ++k) {
+    vel4[4 * k] = (float)(vel4[4 * k] + -px / mass);
+    vel4[4 * k + 1] = (float)(vel4[4 * k + 1] + -py / mass);
+    vel4[4 * k + 2] = (float)(vel4[4 * k + 2] + -pz / mass);
+  }
+  double tk = 0.;
+  for (int k = 0; k < N; ++k) {
+    const float* v = vel4 + 4 * k;
+    const float s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    tk += s;
+  }
+  tk *= 1. / 3. / N;
+  for (int k = 0; k < 3 * N; ++k) {
+    float& v = vel4[4 * (k / 3) + k % 3];
+    v = (float)(v * std::sqrt(T0 / tk));
+  }
+}
+}  // namespace
+
 void gaussian_velocities(int N, double T0, uint64_t seed, float* vel4)
 {
   // component m = 3k + c of particle k; Box-Muller pairs in m order
@@ -84,31 +115,42 @@ void gaussian_velocities(int N, double T0, uint64_t seed, float* vel4)
     if (m + 1 < M) vel4[4 * ((m + 1) / 3) + (m + 1) % 3] = (float)(sT * (rad * std::sin(2.0 * kPi * u2)));
   }
   for (int k = 0; k < N; ++k) vel4[4 * k + 3] = 0.f;
-  // CorrectTotalMomentum (MDSystem.cpp:183-202)
-  double px = 0., py = 0., pz = 0., mass = 0.;
+  momentum_and_temperature(N, T0, vel4);
+}
+
+void maxwell_velocities(int N, double T0, uint64_t seed, float* vel4)
+{
+  // MDSystem.cpp:36-53, 170-177: the speed v = sqrt(2 T) t with t = -ln(alpha),
+  // alpha uniform in (0, 1) accepted when eta < p(alpha), eta uniform in
+  // [0, 1.2) (the generator's probmax, MDSystem.h:51), p(alpha) = t^2 exp(-t^2)
+  // / alpha -- t then has density proportional to t^2 exp(-t^2), the Maxwell
+  // speed distribution at temperature T; the direction has cos(theta) uniform
+  // in [-1, 1) and phi uniform in [0, 2 pi).  As in the reference the speed
+  // sampler has a generator of its own (`sp`) beside the one of the
+  // directions (`dr`); both are SplitMix64 streams of the seed.
+  uint64_t sp = seed, dr = seed ^ 0x5DEECE66DA3B9F1Full;
+  const double s2T = std::sqrt(2. * T0);
   for (int k = 0; k < N; ++k) {
-    px += vel4[4 * k];
-    py += vel4[4 * k + 1];
-    pz += vel4[4 * k + 2];
-    mass += 1.;
+    double t = 0.;
+    for (;;) {
+      const double alpha = ((double)(splitmix_next(sp) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+      const double eta = uniform01(sp) * 1.2;
+      const double tl = -std::log(alpha);
+      if (eta < tl * tl * std::exp(-tl * tl) / alpha) {
+        t = tl;
+        break;
+      }
+    }
+    const double v = s2T * t;
+    const double cth = 2. * uniform01(dr) - 1.;
+    const double ph = 2. * kPi * uniform01(dr);
+    const double sth = std::sqrt(1. - cth * cth);
+    vel4[4 * k] = (float)(v * sth * std::cos(ph));
+    vel4[4 * k + 1] = (float)(v * sth * std::sin(ph));
+    vel4[4 * k + 2] = (float)(v * cth);
+    vel4[4 * k + 3] = 0.f;
   }
-  for (int k = 0; k < N; ++k) {
-    vel4[4 * k] = (float)(vel4[4 * k] + -px / mass);
-    vel4[4 * k + 1] = (float)(vel4[4 * k + 1] + -py / mass);
-    vel4[4 * k + 2] = (float)(vel4[4 * k + 2] + -pz / mass);
-  }
-  // RenormalizeVelocities(true) (MDSystem.cpp:375-385, KineticTemperature :361-373)
-  double tk = 0.;
-  for (int k = 0; k < N; ++k) {
-    const float* v = vel4 + 4 * k;
-    const float s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-    tk += s;
-  }
-  tk *= 1. / 3. / N;
-  for (int k = 0; k < 3 * N; ++k) {
-    float& v = vel4[4 * (k / 3) + k % 3];
-    v = (float)(v * std::sqrt(T0 / tk));
-  }
+  momentum_and_temperature(N, T0, vel4);
 }
 
 void cpu_minimage_thresholds(double L, float thr[4])

@@ -250,6 +250,9 @@ hipError_t launch_batch_set_hist_base(hipStream_t st, const BatchArgs& a, int64_
 // host helpers (lj_host.cpp)
 void lattice_positions(int N, double L, float* pos4);
 void gaussian_velocities(int N, double T0, uint64_t seed, float* vel4);
+// the reference's SampleInitialConditions velocities (Maxwell speeds by
+// rejection, isotropic directions), seeded; momentum removed, T rescaled
+void maxwell_velocities(int N, double T0, uint64_t seed, float* vel4);
 double box_length(int N, double rho);
 float rdf_dr2_rule(int N);
 // smallest positive float x with fast_round((float)(x / L)) >= m, m = 1..4

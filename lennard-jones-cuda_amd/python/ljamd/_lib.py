@@ -13,6 +13,7 @@ LJ_OK = 0
 LJ_ERR_ARG, LJ_ERR_DEVICE, LJ_ERR_ALLOC, LJ_ERR_STATE, LJ_ERR_UNSUPPORTED = -1, -2, -3, -4, -5
 BC_PERIODIC, BC_WALL, BC_NONE = 0, 1, 2
 EVN, TVN = 0, 1
+INIT_GAUSSIAN, INIT_MAXWELL = 0, 1
 RDF_REF_CPU, RDF_REF_GPU, RDF_RBINS = 0, 1, 2
 PREC_FP32, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2
 ARRAY_POS, ARRAY_VEL, ARRAY_FORCE = 0, 1, 2
@@ -95,6 +96,7 @@ SIGNATURES = [
     ("lj_box_length", _d, [_vp]),
     ("lj_rdf_dr2_default", _f, [_i]),
     ("lj_sample_initial_state", _i, [_vp, C.c_uint64]),
+    ("lj_sample_initial_state_ex", _i, [_vp, C.c_uint64, _i]),
     ("lj_set_array", _i, [_vp, _i, _vp]),
     ("lj_get_array", _i, [_vp, _i, _vp]),
     ("lj_set_array_f64", _i, [_vp, _i, _vp]),

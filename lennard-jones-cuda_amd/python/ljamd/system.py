@@ -53,8 +53,14 @@ class LJSystem:
     def L(self):
         return lib().lj_box_length(self.h)
 
-    def sample_initial_state(self, seed):
-        check(lib().lj_sample_initial_state(self.h, int(seed)))
+    def sample_initial_state(self, seed, velocities=None):
+        """Lattice positions and seeded velocities: the Gaussian draw of the
+        benchmark configurations (default) or INIT_MAXWELL, the reference's
+        own Maxwell-speed sampler."""
+        if velocities is None:
+            check(lib().lj_sample_initial_state(self.h, int(seed)))
+        else:
+            check(lib().lj_sample_initial_state_ex(self.h, int(seed), int(velocities)))
 
     def set_array(self, kind, a4):
         a4 = np.ascontiguousarray(a4, np.float32).reshape(-1)

@@ -235,3 +235,40 @@ def test_force_plan_choice(N, pair_once, slots, split_env):
         os.environ.pop(k, None)
     with ljamd.LJSystem(N, 0.85, 1.0, canonical=ljamd.TVN) as s:
         assert s.force_plan() == (pair_once, slots)
+
+
+def test_maxwell_initial_state():
+    """The reference's own initial-velocity sampler (MDSystem.cpp:36-53,
+    147-181), seeded: zero momentum and T = T0 exactly (float rounding), speeds
+    Maxwell-distributed at T0 (Kolmogorov-Smirnov against the chi distribution
+    with 3 degrees of freedom, scale sqrt(T0)), isotropic directions, the
+    lattice positions unchanged, reproducible per seed."""
+    from scipy import stats
+
+    N, rho, T0 = 16384, 0.85, 1.3
+    with ljamd.LJSystem(N, rho, T0) as s:
+        s.sample_initial_state(5, ljamd.INIT_MAXWELL)
+        pos = s.get_array(ljamd.ARRAY_POS).reshape(N, 4)
+        v = s.get_array(ljamd.ARRAY_VEL).reshape(N, 4)[:, :3].astype(np.float64)
+        s.sample_initial_state(5, ljamd.INIT_MAXWELL)
+        v_again = s.get_array(ljamd.ARRAY_VEL).reshape(N, 4)[:, :3]
+        s.sample_initial_state(6, ljamd.INIT_MAXWELL)
+        v_other = s.get_array(ljamd.ARRAY_VEL).reshape(N, 4)[:, :3]
+        s.sample_initial_state(5)
+        pos_g = s.get_array(ljamd.ARRAY_POS).reshape(N, 4)
+        v_g = s.get_array(ljamd.ARRAY_VEL).reshape(N, 4)[:, :3]
+        with pytest.raises(ljamd.LJError):
+            s.sample_initial_state(5, 7)
+    assert np.array_equal(v, v_again) and not np.array_equal(v, v_other)
+    assert np.array_equal(pos, pos_g) and not np.array_equal(v[:, :3].astype(np.float32), v_g)
+    assert np.all(np.abs(v.sum(axis=0)) <= 1e-3)  # float momentum of 16384 particles
+    assert abs(np.sum(v * v) / (3 * N) - T0) <= 1e-6 * T0
+    speed = np.linalg.norm(v, axis=1)
+    ks = stats.kstest(speed, stats.chi(df=3, scale=np.sqrt(T0)).cdf)
+    print(f"\n  KS statistic {ks.statistic:.4f} (p = {ks.pvalue:.3f})")
+    assert ks.pvalue > 1e-3
+    cth = v[:, 2] / speed
+    assert abs(np.mean(cth)) < 0.03 and abs(np.mean(cth ** 2) - 1 / 3) < 0.02
+    # the Gaussian draw, for contrast, is Maxwell-distributed too (same law)
+    assert stats.kstest(np.linalg.norm(v_g.astype(np.float64), axis=1),
+                        stats.chi(df=3, scale=np.sqrt(T0)).cdf).pvalue > 1e-3
