@@ -37,7 +37,7 @@ HIPFLAGS = [
 ]
 
 SOURCES = ["lj_kernels.hip", "lj_force_n3.hip", "lj_batch.hip", "lj_adjoint.hip", "lj_fp64.hip",
-           "lj_noalloc.hip", "lj_fluct.hip", "lj_host.cpp",
+           "lj_noalloc.hip", "lj_fluct.hip", "lj_rdf_cells.hip", "lj_host.cpp",
            "lj_api.cpp", "lj_batch_api.cpp", "lj_adjoint_api.cpp", "lj_fluct_api.cpp"]
 
 

@@ -371,6 +371,7 @@ int lj_destroy(lj_ctx* c)
   fl_free(c->fl);
   dev_free(c->rdf_step);
   dev_free(c->step_tab);
+  rdf_cells_free(c->rdf_cells);
   if (c->e0) (void)hipEventDestroy(c->e0);
   if (c->e1) (void)hipEventDestroy(c->e1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -681,6 +682,7 @@ int prepare_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, int accu
   std::vector<float> e(nbins + 1);
   rdf_bin_thresholds(rule, nbins, dr2, rmax, e.data());
   a.end = rule == LJ_RDF_REF_GPU ? e[nbins - 1] : e[nbins];
+  LJ_TRY(attach_rdf_cells(c, a));
   if (nbins + 1 > c->edges_cap) {
     LJ_HIP(hipStreamSynchronize(st));
     dev_free(c->edges2);
@@ -708,6 +710,19 @@ int lj_rdf(lj_ctx* c, int rule, int nbins, float dr2, double rmax, uint64_t* cou
   LJ_TRY(prepare_rdf(c, rule, nbins, dr2, rmax, accumulate, "lj_rdf", &a));
   hipStream_t st = c->stream;
   LJ_HIP(launch_rdf(st, a));
+  if (rdf_uses_cells(a)) {
+    // coordinates beyond [-L, 2 L): the cell-list pass counted nothing; the
+    // all-pairs pass counts instead
+    unsigned bad = 0u;
+    LJ_HIP(hipMemcpyAsync(&bad, a.cells->result, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    LJ_HIP(hipStreamSynchronize(st));
+    if (bad) {
+      LJ_HIP(hipMemsetAsync(a.cells->result, 0, sizeof(unsigned), st));
+      RdfArgs b = a;
+      b.cells = nullptr;
+      LJ_HIP(launch_rdf(st, b));
+    }
+  }
   LJ_HIP(hipMemcpyAsync(counts, c->rdf, sizeof(uint64_t) * a.nbins, hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
   return LJ_OK;

@@ -68,6 +68,7 @@ struct lj_ctx {
   float* step_tab = nullptr;           // bin thresholds of the step's RDF pass
   int step_tab_cap = 0;
   int64_t rdf_step_n = 0;              // steps the step histogram holds
+  lj::RdfCellsWs rdf_cells;            // workspace of the cell-list RDF pass
   bool sampling() const { return smp_rdf != LJ_SAMPLE_RDF_OFF || smp_fl != 0; }
 };
 
@@ -89,4 +90,17 @@ inline bool is_fp64(const lj_ctx* c) { return c->cfg.precision == LJ_PREC_FP64; 
 double4* state64(lj_ctx* c, int type);
 // plan of the force pass that fills fpart/apart
 inline const ForcePlan& fplan(const lj_ctx* c) { return c->plan3.n3 ? c->plan3 : c->plan; }
+// gives an RDF pass of the context the cell-list workspace where that pass
+// applies (periodic, at most 1024 bins, 3 or more cells per axis), grown first
+inline int attach_rdf_cells(lj_ctx* c, RdfArgs& a)
+{
+  a.cells = nullptr;
+  if (!a.periodic || a.nbins > 1024) return LJ_OK;
+  const int nc = rdf_cells_per_axis(a.L, a.end);
+  if (nc <= 0) return LJ_OK;
+  const int rc = rdf_cells_ensure(c->rdf_cells, a.N, nc * nc * nc, a.nbins);
+  if (rc != LJ_OK) return rc;
+  a.cells = &c->rdf_cells;
+  return LJ_OK;
+}
 }  // namespace lj

@@ -150,6 +150,29 @@ hipError_t launch_store_vw(hipStream_t st, const double* ev, lj_obs* obs);
 hipError_t launch_reset_clock(hipStream_t st, lj_obs* obs);
 
 // RDF
+// Device workspace of the cell-list RDF pass (lj_rdf_cells.hip), owned by a
+// context and grown by rdf_cells_ensure before any capture
+constexpr int kRdfReps = 64;       // replicas of the workgroup histograms
+constexpr int kRdfMaxCells = 64;   // cells per axis at most
+constexpr int kCellScanMax = 1 << 16;  // (cells x blocks) of the LDS-counted sort
+struct RdfCellsWs {
+  int* count = nullptr;            // [cap_cells]: particles per cell (cleared by the scan)
+  int* start = nullptr;            // [cap_cells + 1]
+  int* cellof = nullptr;           // [cap_n]
+  int* rank = nullptr;             // [cap_n]
+  uint4* sorted = nullptr;         // [cap_n]: (q.x, q.y, q.z, particle)
+  unsigned* reps = nullptr;        // [kRdfReps][cap_bins]
+  unsigned long long* other = nullptr;  // [kRdfReps]
+  unsigned* flag = nullptr;        // coordinates beyond [-L, 2 L) in the current pass
+  unsigned* result = nullptr;      // the last pass's flag, for the host
+  int* bcnt = nullptr;             // [cap_m]: per-block cell counts, block-major
+  int cap_n = 0, cap_cells = 0, cap_bins = 0, cap_m = 0;
+};
+// cells per axis for a periodic range ending at r^2 = end, or 0 (fewer than 3)
+int rdf_cells_per_axis(double L, double end);
+int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins);  // LJ_OK or a status
+void rdf_cells_free(RdfCellsWs& w);
+
 struct RdfArgs {
   const float4* pos;
   int N;
@@ -162,8 +185,12 @@ struct RdfArgs {
   float thr[4];      // minimum-image thresholds of the rule
   double end;        // the counted range's end, tab[nbins - 1] (REF_GPU) or tab[nbins]
   unsigned long long* hist;
+  const RdfCellsWs* cells;  // may be null: no cell-list pass
 };
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
+hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a, int nc);
+// whether launch_rdf takes the cell-list pass for these arguments
+bool rdf_uses_cells(const RdfArgs& a);
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);
 // speed histogram (MDSystem.cpp:651-694); hist is added to (clear it first)
 hipError_t launch_velocity_hist(hipStream_t st, const float4* vel, int N, double step, int nbins,

@@ -24,12 +24,14 @@
 #include <cstdlib>
 
 #include "lj_device.h"
+#include "lj_rdf_device.h"
 
 namespace lj {
 
 namespace {
 
 using namespace dev;
+using namespace rdfdev;
 
 // grid: (gx, S); block: kBlock.  fpart[s*Npad + i] = (fx, fy, fz, b), apart = a,
 // in internal units (force = FS * f).
@@ -473,57 +475,6 @@ __global__ void k_reset_averaging(lj_obs* obs)
 // ---------------------------------------------------------------------------
 // RDF
 
-__device__ __forceinline__ float mi_cpu(float r, double L, const float (&t)[4])
-{
-#pragma clang fp contract(off)
-  // k = fast_round((float)(r / L)) (MDSystem.cpp:274-276, :732-739) as a step
-  // function of r with host-computed thresholds t[m-1] = min{x : k(x) >= m}
-  int k = (r >= t[0]) + (r >= t[1]) + (r >= t[2]) + (r >= t[3]);
-  k -= (r <= -t[0]) + (r <= -t[1]) + (r <= -t[2]) + (r <= -t[3]);
-  return (float)((double)r - (double)k * L);
-}
-
-__device__ __forceinline__ float mi_gpu(float d, float Lf, const float (&t)[4])
-{
-#pragma clang fp contract(off)
-  // MDSystem.cu:31-33: r += -L * lround(r / L) with a correctly rounded
-  // division; k = lround(r / L) as a step function of r with host-computed
-  // thresholds t[m-1] = min{x : k(x) >= m}
-  int k = (d >= t[0]) + (d >= t[1]) + (d >= t[2]) + (d >= t[3]);
-  k -= (d <= -t[0]) + (d <= -t[1]) + (d <= -t[2]) + (d <= -t[3]);
-  return d - Lf * (float)k;
-}
-
-// bin of r2 in the threshold table T (T[k] = least r^2 of bin >= k, T[0] = 0
-// <= r2, and r2 < T[nbins - 1] here): the largest k < nbins with T[k] <= r2.
-// The estimate (r2 / dr2, or r / dr, in float with an approximate square
-// root) is off by at most one bin: its relative error is below 2^-21 on bin
-// indices below 2^13, so one step down and one step up on the table give the
-// bin exactly.
-__device__ __forceinline__ int rdf_bin(float r2, const float* T, int nbins, int est)
-{
-  int k = min(max(est, 0), nbins - 1);
-  k -= T[k] > r2 ? 1 : 0;
-  k += T[k + 1] <= r2 ? 1 : 0;  // k + 1 <= nbins - 1 < nbins + 1 entries
-  return k;
-}
-
-// minimum image for |r| < the second threshold (image index -1, 0 or 1):
-// k L is exact, so one fused rounding of r - k L in double equals the
-// reference's product-then-difference
-__device__ __forceinline__ float mi_cpu1(float r, double L, float t0)
-{
-  const int k = (r >= t0 ? 1 : 0) - (r <= -t0 ? 1 : 0);
-  return (float)__builtin_fma(-(double)k, L, (double)r);
-}
-
-__device__ __forceinline__ float mi_gpu1(float d, float Lf, float t0)
-{
-#pragma clang fp contract(off)
-  const float k = (d >= t0 ? 1.f : 0.f) - (d <= -t0 ? 1.f : 0.f);
-  return d - Lf * k;
-}
-
 // RDF pass.  Every binning rule is symmetric in i and j (the coordinate
 // differences of the two orders are exact negatives, the image shift rounds
 // half away from zero, r^2 squares them), so each unordered pair is binned
@@ -539,55 +490,6 @@ __device__ __forceinline__ float mi_gpu1(float d, float Lf, float t0)
 // fixed-point image with a proven error bound and runs the reference
 // arithmetic only near bin edges; the others take k_rdf.
 constexpr int kRdfWaveCopy = 1024;
-
-// The reference arithmetic for one pair: image, r^2, bin (MDSystem.cpp:269-285
-// for REF_CPU and RBINS, MDSystem.cu:28-42 for REF_GPU).  Returns the bin, or
-// -1 for a pair that is not counted.
-template <int RULE, bool PERIODIC>
-__device__ __forceinline__ int rdf_exact_bin(const float4& pi, const float4& pj, double L, float Lf,
-                                             const float (&thr)[4], const float* T, int nbins,
-                                             float Tmax, float Tlast, float est_scale)
-{
-#pragma clang fp contract(off)
-  float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-  if constexpr (PERIODIC) {
-    // |r| < thr[1] on every axis (any position within half a box of the box):
-    // the image index is -1, 0 or 1, from the first threshold alone
-    const float am = fmaxf(fabsf(rx), fmaxf(fabsf(ry), fabsf(rz)));
-    if constexpr (RULE == LJ_RDF_REF_GPU) {
-      if (am < thr[1]) {
-        rx = mi_gpu1(rx, Lf, thr[0]);
-        ry = mi_gpu1(ry, Lf, thr[0]);
-        rz = mi_gpu1(rz, Lf, thr[0]);
-      } else {
-        rx = mi_gpu(rx, Lf, thr);
-        ry = mi_gpu(ry, Lf, thr);
-        rz = mi_gpu(rz, Lf, thr);
-      }
-    } else {
-      if (am < thr[1]) {
-        rx = mi_cpu1(rx, L, thr[0]);
-        ry = mi_cpu1(ry, L, thr[0]);
-        rz = mi_cpu1(rz, L, thr[0]);
-      } else {
-        rx = mi_cpu(rx, L, thr);
-        ry = mi_cpu(ry, L, thr);
-        rz = mi_cpu(rz, L, thr);
-      }
-    }
-  }
-  float r2;
-  if constexpr (RULE == LJ_RDF_REF_GPU) {
-    r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));  // MDSystem.cu:35
-    if (!(r2 > 1e-6f)) return -1;                                    // MDSystem.cu:37
-  } else {
-    r2 = rx * rx + ry * ry + rz * rz;  // float, no contraction (MDSystem.cpp:279)
-    if (!(r2 < Tmax)) return -1;       // beyond the last bin: dropped
-  }
-  if (r2 >= Tlast) return nbins - 1;  // the GPU rule clamps every larger r2 here
-  const float e = RULE == LJ_RDF_RBINS ? __builtin_amdgcn_sqrtf(r2) * est_scale : r2 * est_scale;
-  return rdf_bin(r2, T, nbins, (int)e);
-}
 
 // The block's counts, one unit per unordered pair (doubled into the ordered
 // counts when flushed), in one LDS histogram per wave, h[wave][bin].  Pairs of
@@ -661,42 +563,17 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
 }
 
 // Periodic boxes, small histograms: every pair is first classified from the
-// exact fixed-point minimum image (the force pass's, q = x 2^32 / L).  Its
-// squared distance r2 (fixed-point units, S = 2^32 / L per length) differs
-// from the reference's (float positions, the rule's image and roundings, in
-// S^2 units) by at most
-//   delta = Kr r + K2 r2 + K0,
-//   Kr = 2 sqrt(3) (2 A + 3 L) u S,  K2 = 16 u,  K0 = 3 u^2 (2 A + 5 L)^2 S^2 + Kr^2,
-// u = 2^-24, r = sqrt(r2), A a bound of |x_i - x_j| per axis (here the span
-// of the coordinates of the pair's two particle groups).  Per axis the two
-// differences agree within e = u (2 A + 3 L + 2 |r_a|): the reference's
-// float difference rounds at u A, its image shift at u |r_a| (and the GPU
-// rule's float L is off by u L); the fixed-point difference is within L 2^-32
-// of the exact one and converts at u |r_a|; where the rule and the fixed
-// point take different images (|r_a| within u (A + L) of L/2), the two
-// magnitudes differ by at most 2 u (A + L) + 2 L 2^-32.  Squares differ by
-// e (2 |r_a| + e), sum_a |r_a| <= sqrt(3) r; the reference's r^2 rounds at
-// 3 u r2, ours at 3 u r2, the interval ends and the table's units at u r2
-// each (K2: 16 against 4 + 3 + 3 + 2 + 1); the Kr^2 in K0 covers evaluating
-// Kr r at the fixed-point r rather than the exact one.  Each constant is
-// taken 1% up.  A pair whose interval [r2 - delta, r2 + delta] lies inside
-// one bin (or wholly beyond the counted range) takes that bin without the
-// reference arithmetic; the others -- about 2 delta / bin width of the
-// pairs, 1e-3 at 500 r-bins -- are marked in a bit mask per lane and walked
-// through the reference arithmetic after each run of 32 j.
-//   RBINS (edges T[k] = fl((k dr)^2)): the bin is decided from x = r / dr
-// alone: k = floor(x) when x >= 1 and frac(x) lies in [m, 1 - m],
-// m = es Kr + es^2 K0 + (K2 + 6 u) x (es = 1 / (dr S)): the first terms are
-// delta / (2 r) in bins, the last covers T's rounding, the sqrt (1 ulp) and
-// the products forming x.  No table read.
-//   Other rules: k from the estimate, then T[k] <= r2 - delta and
-// r2 + delta < T[k + 1] from one read of (T[k], T[k + 1]).
+// exact fixed-point minimum image (the force pass's, q = x 2^32 / L) under the
+// error bound of RdfClass (lj_rdf_device.h), with A the span of the
+// coordinates of the pair's two particle groups.  The pairs it leaves
+// undecided -- about 2 delta / bin width of them, 1e-3 at 500 r-bins -- are
+// marked in a bit mask per lane and walked through the reference arithmetic
+// after each run of 32 j, so the exact path runs as often as the busiest lane
+// needs it.
 //   FAR (ranges that hold a small part of the box, as the reference rules'
 // r2 < 25.6 at liquid densities): a first pass over 32 j marks the pairs
-// that may lie in the range by one comparison of r2 with a bound X beyond
-// which r2 - delta >= the range's end (r2 - delta(r2) grows with r2;
-// X = end + 2 delta(sqrt(2 end)) <= 2 end); each lane then walks its marked
-// pairs through the classification.
+// that may lie in the range by one comparison of r2 with RdfClass's far
+// bound; each lane then walks its marked pairs through the classification.
 template <int RULE, bool FAR>
 __global__ __launch_bounds__(kBlock) void k_rdf_fx(const float4* __restrict__ pos, int N, double L,
                                                    int nbins, const float* __restrict__ tab,
@@ -758,39 +635,17 @@ __global__ __launch_bounds__(kBlock) void k_rdf_fx(const float4* __restrict__ po
   const int jlo = i < N ? (g == hgrp ? (int)threadIdx.x : -1) : kBlock;
   const float thr[4] = {t0, t1, t2, t3};
   const float Tmax = T[nbins], Tlast = T[nbins - 1];
-  // the upper end of the counted range, in fixed-point units: dropped
-  // (REF_CPU, RBINS) or the clamped last bin (REF_GPU) from there on; the GPU
-  // rule counts r2 > 1e-6 only
-  const double Tend_d = (double)(RULE == LJ_RDF_REF_GPU ? Tlast : Tmax) * S2;
-  const float Tend = (float)Tend_d;
-  const float Tlow = (float)(1e-6 * S2);
-  // the error bound's constants for this lane and block, in fixed-point units
+  // the bound's span A: the coordinates of this lane's particle and the j group
   float hmax = fmaxf(pi.x, fmaxf(pi.y, pi.z)), hmin = fminf(pi.x, fminf(pi.y, pi.z));
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) {
     hmax = fmaxf(hmax, bw[0][w]);
     hmin = fminf(hmin, bw[1][w]);
   }
-  const double A = (double)hmax - (double)hmin;
-  const double u = 0x1p-24;
-  const double Krd = 1.01 * 3.4641016151377546 * (2. * A + 3. * L) * u * qscale;
-  const double K2d = 1.01 * 16. * u;
-  const double K0d = 1.01 * (3. * u * u * (2. * A + 5. * L) * (2. * A + 5. * L) * S2 + Krd * Krd);
-  const float Kr = (float)Krd, K2 = (float)K2d, K0 = (float)K0d;
-  const double esd = RULE == LJ_RDF_RBINS ? est_scale / qscale : est_scale / S2;
-  const float es = (float)esd;
-  // RBINS: m = M0 + M1 x (bins)
-  const float M0 = (float)(1.01 * (esd * Krd + esd * esd * K0d));
-  const float M1 = (float)(1.01 * (K2d + 6. * u));
-  const float kmax = (float)(nbins - 1), fnb = (float)nbins;
+  RdfClass<RULE> cl;
+  cl.init((double)hmax - (double)hmin, L, qscale, est_scale, nbins, Tlast, Tmax);
   unsigned int last = 0u;
 
-  // the fixed-point r2 of pair (i, j)
-  auto r2_of = [&](const uint4& qj) {
-    const float dx = (float)(int)(qi.x - qj.x), dy = (float)(int)(qi.y - qj.y),
-                dz = (float)(int)(qi.z - qj.z);
-    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-  };
   // the reference arithmetic for pair (i, jj), counted
   auto exact = [&](int jj) {
     const int bin = rdf_exact_bin<RULE, true>(pi, pj_s[jj], L, (float)L, thr, T, nbins, Tmax, Tlast,
@@ -798,41 +653,17 @@ __global__ __launch_bounds__(kBlock) void k_rdf_fx(const float4* __restrict__ po
     if (bin == nbins - 1) ++last;
     else if (bin >= 0) atomicAdd(&hw[bin], 1u);
   };
-  // Classification state of one pair: k (the bin if decided), dec (decided
-  // and counted in bin k), gone (decided: not counted, or REF_GPU's clamp).
-  // RBINS, from r2 alone:
-  auto class_rb = [&](float r2, int& k, bool& dec, bool& gone) {
-    const float x = __builtin_amdgcn_sqrtf(r2) * es;
-    const float m = __builtin_fmaf(M1, x, M0);
-    const float f = __builtin_amdgcn_fractf(x);
-    k = (int)x;
-    dec = (x >= 1.f) & (f >= m) & (f <= 1.f - m) & (k < nbins);
-    gone = x - m >= fnb;
-  };
-  // the other rules: the bin estimate and interval; t = T2[k] is read between
-  auto measure_tab = [&](float r2, int& k, float& lo, float& hi) {
-    const float r = __builtin_bit_cast(float, (__builtin_bit_cast(unsigned, r2) >> 1) + 0x1fc00000u);
-    const float dl = __builtin_fmaf(Kr, r, __builtin_fmaf(K2, r2, K0));
-    lo = r2 - dl;
-    hi = r2 + dl;
-    k = (int)__builtin_amdgcn_fmed3f(r2 * es, 0.f, kmax);
-  };
-  auto decide_tab = [&](int k, float lo, float hi, float2 t, bool& dec, bool& gone) {
-    gone = lo >= Tend;
-    dec = (t.x <= lo) & (hi < t.y);
-    if constexpr (RULE == LJ_RDF_REF_GPU) dec = dec & (lo > Tlow) & (k < nbins - 1);
-  };
   // one pair, decided or walked to the exact path at once (the walks)
   auto one = [&](int jj) {
-    const float r2 = r2_of(qj_s[jj]);
+    const float r2 = rdf_r2q(qi, qj_s[jj]);
     int k;
     bool dec, gone;
     if constexpr (TAB) {
       float lo, hi;
-      measure_tab(r2, k, lo, hi);
-      decide_tab(k, lo, hi, T2[k], dec, gone);
+      cl.measure(r2, k, lo, hi);
+      cl.decide(k, lo, hi, T2[k], dec, gone);
     } else {
-      class_rb(r2, k, dec, gone);
+      cl.rbins(r2, k, dec, gone);
     }
     if (dec) atomicAdd(&hw[k], 1u);
     else if (gone) {
@@ -845,12 +676,11 @@ __global__ __launch_bounds__(kBlock) void k_rdf_fx(const float4* __restrict__ po
   for (int c0 = 0; c0 < nj; c0 += 32) {  // kBlock is a multiple of 32
     unsigned int mask = 0u;
     if constexpr (FAR) {
-      const double dl2 = Krd * __builtin_sqrt(2. * Tend_d) + K2d * 2. * Tend_d + K0d;
-      const float X = (float)((Tend_d + 2. * dl2) * (1. + 0x1p-20));
+      const float X = cl.far_bound();
 #pragma unroll 8
       for (int b = 0; b < 32; ++b) {
         const int jj = c0 + b;
-        const float r2 = r2_of(qj_s[jj]);
+        const float r2 = rdf_r2q(qi, qj_s[jj]);
         const bool ok = (jj > jlo) & (jj < nj);
         const bool far = r2 >= X;
         if constexpr (RULE == LJ_RDF_REF_GPU) last += (ok & far) ? 1u : 0u;
@@ -872,14 +702,14 @@ __global__ __launch_bounds__(kBlock) void k_rdf_fx(const float4* __restrict__ po
           float2 t[B];
 #pragma unroll
           for (int b = 0; b < B; ++b) {
-            measure_tab(r2_of(qj_s[c0 + b0 + b]), kk[b], lo[b], hi[b]);
+            cl.measure(rdf_r2q(qi, qj_s[c0 + b0 + b]), kk[b], lo[b], hi[b]);
             t[b] = T2[kk[b]];
           }
 #pragma unroll
-          for (int b = 0; b < B; ++b) decide_tab(kk[b], lo[b], hi[b], t[b], dec[b], gone[b]);
+          for (int b = 0; b < B; ++b) cl.decide(kk[b], lo[b], hi[b], t[b], dec[b], gone[b]);
         } else {
 #pragma unroll
-          for (int b = 0; b < B; ++b) class_rb(r2_of(qj_s[c0 + b0 + b]), kk[b], dec[b], gone[b]);
+          for (int b = 0; b < B; ++b) cl.rbins(rdf_r2q(qi, qj_s[c0 + b0 + b]), kk[b], dec[b], gone[b]);
         }
 #pragma unroll
         for (int b = 0; b < B; ++b) {
@@ -1196,6 +1026,7 @@ hipError_t launch_rdf(hipStream_t st, const RdfArgs& a)
   const bool small = a.nbins <= kRdfWaveCopy;
   const dim3 grid(ng, ng), blk(kBlock);
   const size_t tab = sizeof(float) * (a.nbins + 1);
+  if (small && a.periodic && rdf_uses_cells(a)) return launch_rdf_cells(st, a, rdf_cells_per_axis(a.L, a.end));
   if (small && a.periodic) {
     const size_t lds = (a.rule != LJ_RDF_RBINS ? sizeof(float2) * a.nbins : 0) + tab +
                        sizeof(unsigned) * kWaves * a.nbins;

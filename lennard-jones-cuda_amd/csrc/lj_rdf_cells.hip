@@ -1,0 +1,556 @@
+// lj_rdf_cells.hip -- the RDF pass over a cell list, for periodic boxes whose
+// counted range is short against the box (the reference rules' r^2 < 25.6 at
+// liquid densities: N = 16384, rho* = 0.85 has 5 cells per axis).
+//
+// The particles are sorted into nc^3 cells of side s >= sqrt(X), X the
+// classification's far bound (lj_rdf_device.h): every pair whose fixed-point
+// r2 is below X lies in the same or adjacent cells, and every other pair is,
+// by the error bound, beyond the counted range in the reference's arithmetic
+// too -- dropped (CPU rule, r-bins) or in the clamped last bin (GPU rule).  A
+// work item is (cell c, forward offset o): the cell's own pairs once (o = 0)
+// or its pairs with one of the 13 forward neighbours (half a shell), so every
+// unordered pair of adjacent cells is met once (nc >= 3 makes the 27
+// neighbours distinct).  Its pairs are classified as in k_rdf_fx, with the
+// reference arithmetic for the pairs near a bin edge.  The GPU rule's clamp bin then holds the N (N - 1) / 2 pairs
+// less those counted elsewhere (or excluded at r^2 <= 1e-6).
+//
+// The bound holds for coordinate spans up to A = 3 L (all coordinates in
+// [-L, 2 L), as the periodic wrap keeps them); the assignment kernel flags
+// any other coordinate, the pass then counts nothing, and the flag is left
+// for the host (lj_rdf falls back to the all-pairs pass).
+//
+// Kernels: the sort -- up to 4096 cells, k_cells_assign_lds (cell and rank
+// in block-private LDS counts) and k_cells_place (the cells' starts, the
+// sorted fixed-point positions with their index); else assign (global
+// counters), scan (one block) and scatter -- then the pairs and the
+// reduction.  Counts are integers added by atomics (order-independent), so the
+// histogram is exactly the reference's whatever the order within a cell.
+// Workgroup histograms go to one of kRdfReps replicas, which the reduction
+// sums, doubles into ordered-pair counts, adds to hist and clears for the
+// next pass (no memset node in a captured step).
+#include <cmath>
+
+#include <cstdlib>
+
+#include "lj_internal.h"
+#include "lj_rdf_device.h"
+#include "lj_util.h"
+
+namespace lj {
+
+namespace {
+
+using namespace dev;
+using namespace rdfdev;
+
+constexpr int kCellBlock = 256;
+constexpr int kJStage = 512;  // neighbour-cell particles staged in LDS at a time
+
+__device__ __forceinline__ int cell_of(const uint4& q, int nc)
+{
+  const int cx = (int)(((unsigned long long)q.x * (unsigned)nc) >> 32);
+  const int cy = (int)(((unsigned long long)q.y * (unsigned)nc) >> 32);
+  const int cz = (int)(((unsigned long long)q.z * (unsigned)nc) >> 32);
+  return cx + nc * (cy + nc * cz);
+}
+
+__global__ __launch_bounds__(kCellBlock) void k_cells_assign(const float4* __restrict__ pos, int N,
+                                                             double L, double qs, int nc,
+                                                             int* __restrict__ count,
+                                                             int* __restrict__ cellof,
+                                                             int* __restrict__ rank,
+                                                             unsigned* __restrict__ flag)
+{
+  const int i = blockIdx.x * kCellBlock + threadIdx.x;
+  if (i >= N) return;
+  const float4 p = pos[i];
+  const uint4 q = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), 0u);
+  const int c = cell_of(q, nc);
+  cellof[i] = c;
+  rank[i] = atomicAdd(&count[c], 1);
+  const double lo = -L, hi = 2. * L;
+  if (!((double)p.x >= lo && (double)p.x < hi && (double)p.y >= lo && (double)p.y < hi &&
+        (double)p.z >= lo && (double)p.z < hi))
+    atomicOr(flag, 1u);
+}
+
+// The same with the counts in LDS (ncell <= kCellLds): each block ranks its
+// particles in a private histogram and writes its count of every cell to
+// bcnt[block * ncell + c], from which k_cells_place finds the cells' starts
+// and each block's offset within a cell -- no global atomics, which on a few
+// hundred counters serialised (28 us at N = 16384 over 125 cells, against 3 us).
+constexpr int kCellLds = 4096;
+constexpr int kAssignPer = 4;  // particles per thread of the LDS-counted assignment
+__global__ __launch_bounds__(kCellBlock) void k_cells_assign_lds(const float4* __restrict__ pos, int N,
+                                                                 double L, double qs, int nc,
+                                                                 int ncell, int nblk,
+                                                                 int* __restrict__ bcnt,
+                                                                 int* __restrict__ cellof,
+                                                                 int* __restrict__ rank,
+                                                                 unsigned* __restrict__ flag)
+{
+  __shared__ int h[kCellLds];
+  for (int c = threadIdx.x; c < ncell; c += kCellBlock) h[c] = 0;
+  __syncthreads();
+  for (int k = 0; k < kAssignPer; ++k) {
+    const int i = blockIdx.x * (kCellBlock * kAssignPer) + k * kCellBlock + (int)threadIdx.x;
+    if (i >= N) break;
+    const float4 p = pos[i];
+    const uint4 q = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), 0u);
+    const int c = cell_of(q, nc);
+    cellof[i] = c;
+    rank[i] = atomicAdd(&h[c], 1);
+    const double lo = -L, hi = 2. * L;
+    if (!((double)p.x >= lo && (double)p.x < hi && (double)p.y >= lo && (double)p.y < hi &&
+          (double)p.z >= lo && (double)p.z < hi))
+      atomicOr(flag, 1u);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < ncell; c += kCellBlock) bcnt[(size_t)blockIdx.x * ncell + c] = h[c];
+}
+
+// After k_cells_assign_lds, one launch over the same blocks: each block sums
+// the per-block counts into the cells' totals, scans them into the cells'
+// starts (block 0 also writes them to start), adds the counts of the blocks
+// before it, and scatters its own particles.  The scan is repeated by every
+// block (ncell x nblk reads each, from L2) instead of a separate one-block
+// launch.
+__global__ __launch_bounds__(kCellBlock) void k_cells_place(const float4* __restrict__ pos, int N,
+                                                            double qs, int ncell, int nblk,
+                                                            const int* __restrict__ bcnt,
+                                                            const int* __restrict__ cellof,
+                                                            const int* __restrict__ rank,
+                                                            int* __restrict__ start,
+                                                            uint4* __restrict__ sorted)
+{
+  __shared__ int cs[kCellLds];
+  __shared__ int wsum[kWaves];
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < ncell; base += kCellBlock) {
+    const int k = base + (int)threadIdx.x;
+    int tot = 0, mine = 0;
+    if (k < ncell) {
+#pragma unroll 8
+      for (int t = 0; t < nblk; ++t) {
+        const int v = bcnt[(size_t)t * ncell + k];
+        tot += v;
+        mine += t < (int)blockIdx.x ? v : 0;
+      }
+    }
+    int x = tot;  // inclusive scan inside the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int before = carry, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      before += w < wv ? wsum[w] : 0;
+      total += wsum[w];
+    }
+    if (k < ncell) {
+      const int ex = before + x - tot;
+      cs[k] = ex + mine;
+      if (blockIdx.x == 0) start[k] = ex;
+    }
+    carry += total;
+    __syncthreads();  // wsum is rewritten by the next chunk; cs is complete after the last
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) start[ncell] = carry;
+  for (int u = 0; u < kAssignPer; ++u) {
+    const int i = blockIdx.x * (kCellBlock * kAssignPer) + u * kCellBlock + (int)threadIdx.x;
+    if (i >= N) break;
+    const float4 p = pos[i];
+    sorted[cs[cellof[i]] + rank[i]] =
+        make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), (unsigned)i);
+  }
+}
+
+// exclusive scan of count[0, n) into start[0, n], one block; count is
+// cleared for the next pass
+__global__ __launch_bounds__(1024) void k_cells_scan(int* __restrict__ count, int* __restrict__ start,
+                                                     int n)
+{
+  // per chunk of 1024: an inclusive scan inside each wave (shuffles), the
+  // 16 wave totals scanned by wave 0, then each value offset by its wave's
+  // exclusive total and the chunks' running carry
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    const int k = base + (int)threadIdx.x;
+    const int v = k < n ? count[k] : 0;
+    if (k < n) count[k] = 0;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (wv == 0) {
+      int t = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        const int y = __shfl_up(t, o, 64);
+        if (lane >= o) t += y;
+      }
+      if (lane < 16) wsum[lane] = t;  // inclusive
+    }
+    __syncthreads();
+    const int before = carry + (wv > 0 ? wsum[wv - 1] : 0);
+    if (k < n) start[k] = before + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += wsum[15];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) start[n] = carry;
+}
+
+// nblk > 0: start holds the blocks' offsets per cell (k_cells_assign_lds);
+// else the cells' starts with global ranks
+__global__ __launch_bounds__(kCellBlock) void k_cells_scatter(const float4* __restrict__ pos, int N,
+                                                              double qs,
+                                                              const int* __restrict__ cellof,
+                                                              const int* __restrict__ rank,
+                                                              const int* __restrict__ start,
+                                                              uint4* __restrict__ sorted)
+{
+  const int i = blockIdx.x * kCellBlock + threadIdx.x;
+  if (i >= N) return;
+  const float4 p = pos[i];
+  sorted[start[cellof[i]] + rank[i]] = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), (unsigned)i);
+}
+
+// the 13 forward neighbours of a cell (dz > 0, or dz = 0 and dy > 0, or
+// dz = dy = 0 and dx > 0), after the cell itself (offset 0)
+__device__ __forceinline__ void shell_offset(int o, int& dx, int& dy, int& dz)
+{
+  // o = 1..13 enumerate the 26 neighbours' forward half in lexicographic
+  // (dz, dy, dx) order
+  const int k = o + 13;  // index of (dx, dy, dz) in [-1, 1]^3, 13 = the cell itself
+  dx = k % 3 - 1;
+  dy = (k / 3) % 3 - 1;
+  dz = k / 9 - 1;
+}
+
+// One workgroup's pairs: the particles of cell blockIdx.x against those of
+// its neighbour blockIdx.y (0..13: the cell itself, then 13 forward
+// neighbours), counted into replica (blockIdx.x + ncell blockIdx.y) % kRdfReps.
+// Returns (uniformly over the workgroup) with nothing counted when either cell
+// is empty or the flag is set.
+template <int RULE>
+__device__ __forceinline__ void rdf_cells_pairs(
+    const uint4* __restrict__ sorted, const int* __restrict__ start, const float4* __restrict__ pos,
+    int nc, double L, int nbins, const float* __restrict__ tab, float est_scale, float t0, float t1,
+    float t2, float t3, double qscale, const unsigned* __restrict__ flag,
+    unsigned* __restrict__ reps, unsigned long long* __restrict__ other)
+{
+  constexpr bool TAB = RULE != LJ_RDF_RBINS;
+  constexpr bool GPU = RULE == LJ_RDF_REF_GPU;
+  if (*flag) return;  // coordinates beyond the bound's span: no count
+  const int c = blockIdx.x, o = blockIdx.y;
+  const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
+  int dx, dy, dz;
+  shell_offset(o, dx, dy, dz);
+  const int c2 = (cx + dx + nc) % nc + nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc));
+  const int i0 = start[c], i1 = start[c + 1];
+  const int j0 = start[c2], j1 = start[c2 + 1];
+  if (i0 == i1 || j0 == j1) return;  // uniform, before any barrier
+
+  // dynamic LDS: T2 [nbins] when TAB: (T[k], T[k + 1]) S^2; T [nbins + 1]; the
+  // per-wave histograms
+  extern __shared__ float2 sh_cells[];
+  float2* T2 = sh_cells;
+  float* T = reinterpret_cast<float*>(T2 + (TAB ? nbins : 0));
+  unsigned int* hs = reinterpret_cast<unsigned int*>(T + nbins + 1);
+  __shared__ unsigned long long s_other;
+  __shared__ uint4 sj[kJStage];
+  for (int b = threadIdx.x; b < kWaves * nbins; b += kCellBlock) hs[b] = 0u;
+  unsigned int* hw = hs + (int)(threadIdx.x >> 6) * nbins;
+  const double S2 = qscale * qscale;
+  for (int b = threadIdx.x; b <= nbins; b += kCellBlock) {
+    const float t = tab[b];
+    T[b] = t;
+    if constexpr (TAB) {
+      const float tq = (float)((double)t * S2);
+      if (b < nbins) T2[b].x = tq;
+      if (b > 0) T2[b - 1].y = tq;
+    }
+  }
+  if (threadIdx.x == 0) s_other = 0ull;
+  __syncthreads();
+
+  const float thr[4] = {t0, t1, t2, t3};
+  const float Tmax = T[nbins], Tlast = T[nbins - 1];
+  RdfClass<RULE> cl;
+  cl.init(3. * L, L, qscale, est_scale, nbins, Tlast, Tmax);
+  // GPU rule: the clamp bin takes every pair not counted elsewhere; the pass
+  // counts the pairs of the other bins (its histogram's total, taken at the
+  // flush) and the excluded pairs r2 <= 1e-6 (this lane's, from the exact path)
+  unsigned int nexcl = 0u;
+
+  // The neighbour cell's particles are staged in LDS, kJStage at a time (so
+  // that the walks below gather them from LDS, not global memory); each wave
+  // takes its own chunks of 64 i (a cell of ~130 particles keeps three waves
+  // busy; a wave without a chunk idles to the next barrier).
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  for (int jb = j0; jb < j1; jb += kJStage) {
+    const int jn = min(kJStage, j1 - jb);
+    __syncthreads();  // the previous chunk is no longer read
+    for (int t = threadIdx.x; t < jn; t += kCellBlock) sj[t] = sorted[jb + t];
+    __syncthreads();
+    for (int ib = i0 + 64 * wave; ib < i1; ib += kCellBlock) {
+      const int ii = ib + lane;
+      const bool vi = ii < i1;
+      const uint4 qi = sorted[vi ? ii : i1 - 1];
+      // the lane's pairs: j in the chunk, and j > ii within the cell itself
+      const int jlo = vi ? (o == 0 ? ii : jb - 1) : j1;
+      // all j of the chunk classified 8 at a time (about 13% of an adjacent
+      // cell pair's pairs lie in range: batches keep the table reads and
+      // atomics of independent pairs in flight together); the rare undecided
+      // pairs are walked through the reference arithmetic per run of 32
+      for (int c0 = 0; c0 < jn; c0 += 32) {
+        const int cn = min(32, jn - c0);
+        unsigned int mask = 0u;
+        constexpr int B = 8;
+        for (int b0 = 0; b0 < cn; b0 += B) {
+          int kk[B];
+          bool dec[B], gone[B];
+          if constexpr (TAB) {
+            float lo[B], hi[B];
+            float2 tt[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+              cl.measure(rdf_r2q(qi, sj[min(c0 + b0 + b, jn - 1)]), kk[b], lo[b], hi[b]);
+              tt[b] = T2[kk[b]];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) cl.decide(kk[b], lo[b], hi[b], tt[b], dec[b], gone[b]);
+          } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+              cl.rbins(rdf_r2q(qi, sj[min(c0 + b0 + b, jn - 1)]), kk[b], dec[b], gone[b]);
+          }
+#pragma unroll
+          for (int b = 0; b < B; ++b) {
+            const int t = c0 + b0 + b;
+            const bool ok = (t < jn) & (jb + t > jlo);
+            if (ok & dec[b]) atomicAdd(&hw[kk[b]], 1u);
+            mask |= (ok & !dec[b] & !gone[b]) ? (1u << (b0 + b)) : 0u;
+          }
+        }
+        while (mask) {
+          const int t = c0 + __builtin_ctz(mask);
+          mask &= mask - 1u;
+          const int bin = rdf_exact_bin<RULE, true>(pos[qi.w], pos[sj[t].w], L, (float)L, thr, T,
+                                                    nbins, Tmax, Tlast, est_scale);
+          if constexpr (GPU) {
+            if (bin != nbins - 1) {
+              if (bin >= 0) atomicAdd(&hw[bin], 1u);
+              else ++nexcl;
+            }
+          } else {
+            if (bin >= 0) atomicAdd(&hw[bin], 1u);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  unsigned* rep = reps + (size_t)((blockIdx.x + gridDim.x * blockIdx.y) % kRdfReps) * nbins;
+  unsigned long long tot = nexcl;  // GPU rule: this thread's share of the pass's counted pairs
+  for (int b = threadIdx.x; b < nbins; b += kCellBlock) {
+    unsigned int v = 0u;
+    for (int w = 0; w < kWaves; ++w) v += hs[w * nbins + b];
+    if (v) atomicAdd(&rep[b], v);
+    tot += v;
+  }
+  if constexpr (GPU) {
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) tot += __shfl_xor(tot, s, 64);
+    if (lane == 0 && tot) atomicAdd(&s_other, tot);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_other)
+      atomicAdd(&other[(blockIdx.x + gridDim.x * blockIdx.y) % kRdfReps], s_other);
+  }
+}
+
+// one workgroup of kRdfReps threads per bin: hist[b] += 2 sum of the bin's
+// replicas (cleared); workgroup 0 also adds the GPU rule's clamp bin,
+// 2 (P - sum of other), clears other and the flag and sets result when the
+// pass was void.  A workgroup that reads the flag after workgroup 0 cleared
+// it adds only zeros (a void pass leaves every replica at zero).
+template <int RULE>
+__global__ __launch_bounds__(kCellBlock) void k_rdf_cells(
+    const uint4* __restrict__ sorted, const int* __restrict__ start, const float4* __restrict__ pos,
+    int nc, double L, int nbins, const float* __restrict__ tab, float est_scale, float t0, float t1,
+    float t2, float t3, double qscale, const unsigned* __restrict__ flag,
+    unsigned* __restrict__ reps, unsigned long long* __restrict__ other)
+{
+  rdf_cells_pairs<RULE>(sorted, start, pos, nc, L, nbins, tab, est_scale, t0, t1, t2, t3, qscale,
+                        flag, reps, other);
+}
+
+// Sums the replicas into hist and clears them (one wave per bin); the GPU
+// rule's clamp bin gets every pair the pass did not count elsewhere, 2 (P -
+// other) as ordered pairs; clears the flag after noting it in result for the
+// host.  (The same done by the pair kernel's last workgroup, behind a release
+// count per workgroup, measured 67 us against 51 + 4.6: on this chip an
+// agent-scope release writes back the XCD's L2.)
+__global__ __launch_bounds__(kRdfReps) void k_rdf_cells_reduce(
+    unsigned* __restrict__ reps, unsigned long long* __restrict__ other, int nbins, int gpu,
+    unsigned long long P, unsigned* __restrict__ flag, unsigned* __restrict__ result,
+    unsigned long long* __restrict__ hist)
+{
+  static_assert(kRdfReps == 64, "one wave per bin");
+  const int b = blockIdx.x, r = threadIdx.x;
+  const bool bad = *flag != 0u;
+  unsigned long long s = reps[(size_t)r * nbins + b];
+  reps[(size_t)r * nbins + b] = 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (r == 0 && !bad && s) atomicAdd(&hist[b], 2ull * s);
+  if (b == 0) {
+    unsigned long long so = other[r];
+    other[r] = 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) so += __shfl_xor(so, o, 64);
+    if (r == 0) {
+      if (!bad && gpu) atomicAdd(&hist[nbins - 1], 2ull * (P - so));
+      if (bad) *result = 1u;
+      *flag = 0u;
+    }
+  }
+}
+
+}  // namespace
+
+// the cell count per axis for a range ending at `end` (real r^2 units), or 0
+// when the cell list does not apply (fewer than 3 cells per axis, or more
+// than kRdfMaxCells per axis)
+int rdf_cells_per_axis(double L, double end)
+{
+  if (!(L > 0.) || !(end > 0.) || !std::isfinite(end)) return 0;
+  // the device's far bound at the guaranteed span A = 3 L (RdfClass::far_bound)
+  const double qs = 4294967296. / L, S2 = qs * qs, u = 0x1p-24, A = 3. * L;
+  const double Krd = 1.01 * 3.4641016151377546 * (2. * A + 3. * L) * u * qs;
+  const double K2d = 1.01 * 16. * u;
+  const double K0d = 1.01 * (3. * u * u * (2. * A + 5. * L) * (2. * A + 5. * L) * S2 + Krd * Krd);
+  const double Tq = (double)(float)end * S2;
+  const double dl2 = Krd * std::sqrt(2. * Tq) + K2d * 2. * Tq + K0d;
+  // the float rounding of X up, and 1e-6 beside it: the cell side (in
+  // fixed-point units) must square to at least X
+  const double X = (Tq + 2. * dl2) * (1. + 0x1p-20) * (1. + 1e-6);
+  const double side = std::sqrt(X);  // fixed-point units
+  const int nc = (int)std::floor(4294967296. / side);
+  if (nc < 3 || nc > kRdfMaxCells) return 0;
+  return nc;
+}
+
+int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins)
+{
+  if (N <= w.cap_n && ncell <= w.cap_cells && nbins <= w.cap_bins && w.flag) return LJ_OK;
+  (void)hipDeviceSynchronize();  // the arrays may be in use by earlier passes
+  rdf_cells_free(w);
+  const int cn = N, cc = ncell, cb = nbins;
+  LJ_TRY(dev_alloc(&w.count, cc));
+  LJ_TRY(dev_alloc(&w.start, (size_t)cc + 1));
+  LJ_TRY(dev_alloc(&w.cellof, cn));
+  LJ_TRY(dev_alloc(&w.rank, cn));
+  LJ_TRY(dev_alloc(&w.sorted, cn));
+  LJ_TRY(dev_alloc(&w.reps, (size_t)kRdfReps * cb));
+  LJ_TRY(dev_alloc(&w.other, kRdfReps));
+  LJ_TRY(dev_alloc(&w.flag, 2));
+  w.result = w.flag + 1;
+  LJ_TRY(dev_alloc(&w.bcnt, kCellScanMax));
+  w.cap_m = kCellScanMax;
+  // the counts, replicas and flags start at zero; the passes clear them after use
+  if (hipMemset(w.count, 0, sizeof(int) * cc) != hipSuccess ||
+      hipMemset(w.reps, 0, sizeof(unsigned) * kRdfReps * cb) != hipSuccess ||
+      hipMemset(w.other, 0, sizeof(unsigned long long) * kRdfReps) != hipSuccess ||
+      hipMemset(w.flag, 0, sizeof(unsigned) * 2) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess)
+    return fail(LJ_ERR_DEVICE, "rdf_cells_ensure: clearing the workspace failed");
+  w.cap_n = cn;
+  w.cap_cells = cc;
+  w.cap_bins = cb;
+  return LJ_OK;
+}
+
+void rdf_cells_free(RdfCellsWs& w)
+{
+  dev_free(w.count);
+  dev_free(w.start);
+  dev_free(w.cellof);
+  dev_free(w.rank);
+  dev_free(w.sorted);
+  dev_free(w.reps);
+  dev_free(w.other);
+  dev_free(w.flag);
+  dev_free(w.bcnt);
+  w.result = nullptr;
+  w.cap_m = 0;
+  w.cap_n = w.cap_cells = w.cap_bins = 0;
+}
+
+// the cell-list pass serves periodic boxes with at most 1024 bins and 3 or
+// more cells per axis, given a workspace large enough (LJ_RDF_CELLS=0 turns
+// it off)
+bool rdf_uses_cells(const RdfArgs& a)
+{
+  static const char* e = std::getenv("LJ_RDF_CELLS");
+  if (e && e[0] == '0') return false;
+  if (!a.cells || !a.periodic || a.nbins > 1024) return false;
+  const int nc = rdf_cells_per_axis(a.L, a.end);
+  return nc > 0 && a.cells->flag && a.N <= a.cells->cap_n && nc * nc * nc <= a.cells->cap_cells &&
+         a.nbins <= a.cells->cap_bins;
+}
+
+hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a, int nc)
+{
+  const RdfCellsWs& w = *a.cells;
+  const int ncell = nc * nc * nc;
+  const double qs = 4294967296. / a.L;
+  const int nb = (a.N + kCellBlock - 1) / kCellBlock;
+  const int na = (a.N + kCellBlock * kAssignPer - 1) / (kCellBlock * kAssignPer);
+  if (ncell <= kCellLds && (long)ncell * na <= (long)w.cap_m) {
+    // per-block counts in LDS; the cells' starts and the blocks' offsets
+    // found and the particles placed by the same blocks in a second launch
+    hipLaunchKernelGGL(k_cells_assign_lds, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, a.L, qs, nc,
+                       ncell, na, w.bcnt, w.cellof, w.rank, w.flag);
+    hipLaunchKernelGGL(k_cells_place, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, qs, ncell, na,
+                       w.bcnt, w.cellof, w.rank, w.start, w.sorted);
+  } else {
+    hipLaunchKernelGGL(k_cells_assign, dim3(nb), dim3(kCellBlock), 0, st, a.pos, a.N, a.L, qs, nc,
+                       w.count, w.cellof, w.rank, w.flag);
+    hipLaunchKernelGGL(k_cells_scan, dim3(1), dim3(1024), 0, st, w.count, w.start, ncell);
+    hipLaunchKernelGGL(k_cells_scatter, dim3(nb), dim3(kCellBlock), 0, st, a.pos, a.N, qs, w.cellof,
+                       w.rank, w.start, w.sorted);
+  }
+  const size_t lds = (a.rule != LJ_RDF_RBINS ? sizeof(float2) * a.nbins : 0) +
+                     sizeof(float) * (a.nbins + 1) + sizeof(unsigned) * kWaves * a.nbins;
+  const dim3 grid(ncell, 14);
+#define LJ_CELLS_ARGS w.sorted, w.start, a.pos, nc, a.L, a.nbins, a.tab, a.est_scale, a.thr[0], \
+                      a.thr[1], a.thr[2], a.thr[3], qs, w.flag, w.reps, w.other
+  if (a.rule == LJ_RDF_REF_CPU)
+    hipLaunchKernelGGL(k_rdf_cells<LJ_RDF_REF_CPU>, grid, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);
+  else if (a.rule == LJ_RDF_REF_GPU)
+    hipLaunchKernelGGL(k_rdf_cells<LJ_RDF_REF_GPU>, grid, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);
+  else
+    hipLaunchKernelGGL(k_rdf_cells<LJ_RDF_RBINS>, grid, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);
+#undef LJ_CELLS_ARGS
+  const unsigned long long P = (unsigned long long)a.N * (unsigned long long)(a.N - 1) / 2ull;
+  hipLaunchKernelGGL(k_rdf_cells_reduce, dim3(a.nbins), dim3(kRdfReps), 0, st, w.reps, w.other,
+                     a.nbins, a.rule == LJ_RDF_REF_GPU ? 1 : 0, P, w.flag, w.result, a.hist);
+  return hipGetLastError();
+}
+
+}  // namespace lj

@@ -23,6 +23,11 @@ RULES = {RDF_REF_CPU: ljamd.RDF_REF_CPU, RDF_REF_GPU: ljamd.RDF_REF_GPU,
     (3001, 0.3, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),
     (16384, 0.85, ljamd.BC_PERIODIC, RDF_RBINS, 500),  # the N = 16384 workload: 500 bins
     (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 256),
+    # the cell-list pass (periodic, 3 or more cells of side >= ~5.06 per axis):
+    (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),  # 5 cells per axis
+    (3000, 0.5, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),     # L = 18.2: 3 cells per axis
+    (6000, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 256),    # L = 19.2: 3 cells per axis
+    (9000, 0.3, ljamd.BC_PERIODIC, RDF_REF_GPU, 128),     # L = 31.1: 6 cells, range r^2 < 12.8
 ])
 def test_rdf_bitexact(N, rho, bc, rule, nbins):
     L, pos, _ = perturbed_state(N, rho, 1.0, seed=31)
@@ -85,3 +90,110 @@ def test_rdf_bad_args():
             s.rdf(ljamd.RDF_RBINS, 0)
         with pytest.raises(ljamd.LJError):
             s.rdf(ljamd.RDF_RBINS, 100000)
+
+
+def test_rdf_cells_out_of_span_falls_back():
+    """Coordinates beyond [-L, 2 L) void the cell-list pass's bound: lj_rdf
+    then counts with the all-pairs pass, still bit-exact (one particle moved
+    by +3 L, a periodic image of itself)."""
+    N, rho = 6000, 0.85
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=8)
+    p = pos.reshape(-1, 4)
+    p[17, 0] += np.float32(3.0 * L)
+    pos = p.reshape(-1).copy()
+    dr2 = ORC.orc_rdf_dr2(N)
+    for rule in (RDF_REF_GPU, RDF_REF_CPU):
+        with ljamd.LJSystem(N, rho, 1.0) as s:
+            s.set_state(pos)
+            got = s.rdf(RULES[rule], 256, dr2=dr2)
+            again = s.rdf(RULES[rule], 256, dr2=dr2)
+        exp = rdf(pos, L, True, rule, 256, dr2=dr2)
+        assert np.array_equal(got.astype(np.int64), exp)
+        assert np.array_equal(again, got)
+
+
+def _cells_on_and_off(code, tmp_path, timeout=300):
+    """Run `code` (argv: package path, output .npz) in two child processes,
+    LJ_RDF_CELLS=1 (the cell-list pass where it applies) and LJ_RDF_CELLS=0
+    (the all-pairs pass), and return both lists of saved arrays."""
+    import os
+    import subprocess
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    py = os.path.join(os.path.dirname(here), "lennard-jones-cuda_amd", "python")
+    res = {}
+    for cells in ("1", "0"):
+        outf = str(tmp_path / f"cells{cells}.npz")
+        env = dict(os.environ, LJ_RDF_CELLS=cells, PYTHONPATH=here + os.pathsep + py)
+        r = subprocess.run([sys.executable, "-c", code, py, outf], env=env, cwd=here,
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                           timeout=timeout)
+        assert r.returncode == 0, r.stderr[-2000:]
+        with np.load(outf) as z:
+            res[cells] = [z[k] for k in sorted(z.files, key=lambda t: int(t.split("_")[1]))]
+    return res["1"], res["0"]
+
+
+def test_step_rdf_cells_equal_all_pairs_pass(tmp_path):
+    """The step RDF of a periodic box at N = 6000 (3 cells per axis): summed
+    over 10 captured steps with the cell-list pass, and in a child process
+    with LJ_RDF_CELLS=0 (the all-pairs pass) from the same state: the same
+    counts, bit for bit, for both reference rules; the trajectory too."""
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import ljamd
+from oracle_lib import lattice_state
+N, rho, T0 = 6000, 0.85, 1.0
+L, pos, vel = lattice_state(N, rho, T0, 21)
+out = []
+for rule in (ljamd.RDF_REF_GPU, ljamd.RDF_REF_CPU):
+    with ljamd.LJSystem(N, rho, T0, canonical=ljamd.TVN) as s:
+        s.set_state(pos, vel)
+        s.CalculateForces()
+        s.set_step_sampling(rdf=ljamd.SAMPLE_RDF_SUM, rdf_rule=rule, rdf_nbins=256)
+        s.Integrate(0.005, 10)
+        got, n = s.step_rdf_counts()
+        out.append(got.astype(np.int64))
+        out.append(s.pos.astype(np.float64))
+np.savez(sys.argv[2], *out)
+"""
+    on, off = _cells_on_and_off(code, tmp_path)
+    for a, b in zip(on, off):
+        assert np.array_equal(a, b)
+    assert on[0].sum() > 0 and on[2].sum() > 0
+
+
+@pytest.mark.parametrize("N", [70000, 131072])
+def test_rdf_cells_large_boxes_equal_all_pairs_pass(N, tmp_path):
+    """Above 65536 particles the cell sort runs over many workgroups: per-block
+    counts in LDS (N = 70000: 8^3 cells) or global counters (N = 131072: 10^3
+    cells).  Both reference rules, perturbed lattice and after 20 TVN steps:
+    the same counts as the all-pairs pass, bit for bit, and every pair counted
+    (the GPU rule's total is N (N - 1): each ordered pair once)."""
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import ljamd
+from helpers import perturbed_state
+from oracle_lib import ORC
+N = int(sys.argv[3])
+L, pos, vel = perturbed_state(N, 0.85, 1.0, seed=5)
+dr2 = ORC.orc_rdf_dr2(N)
+out = []
+with ljamd.LJSystem(N, 0.85, 1.0, canonical=ljamd.TVN) as s:
+    s.set_state(pos, vel)
+    for rule in (ljamd.RDF_REF_GPU, ljamd.RDF_REF_CPU):
+        out.append(s.rdf(rule, 256, dr2=dr2).astype(np.int64))
+    s.CalculateForces()
+    s.Integrate(0.005, 20)
+    for rule in (ljamd.RDF_REF_GPU, ljamd.RDF_REF_CPU):
+        out.append(s.rdf(rule, 256, dr2=dr2).astype(np.int64))
+np.savez(sys.argv[2], *out)
+""".replace("int(sys.argv[3])", str(N))
+    on, off = _cells_on_and_off(code, tmp_path)
+    for a, b in zip(on, off):
+        assert np.array_equal(a, b)
+    assert on[0].sum() == N * (N - 1) and on[2].sum() == N * (N - 1)
+    assert 0 < on[1].sum() < N * (N - 1)
