@@ -95,10 +95,10 @@ def gpu_leg(steps, warmup, boundary=None):
     t = time.perf_counter() - t0
     obs = s.observables()
     reps = KERNEL_REPS
-    us = s.time_force_kernel(reps)
+    us, span = s.time_force_kernel_stamped(reps)
     pair_once, nslots = s.force_plan()
     s.close()
-    return {"t": t, "hist": hist, "obs": obs, "force_us": us, "reps": reps,
+    return {"t": t, "hist": hist, "obs": obs, "force_us": us, "force_span_us": span, "reps": reps,
             "pair_once": pair_once, "nslots": nslots, "settle_ms": settle_ms}
 
 
@@ -108,7 +108,7 @@ def sliced_kernel_us(reps=KERNEL_REPS):
 
     with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED, jslices=32) as s:
         settle(s)
-        return s.time_force_kernel(reps)
+        return s.time_force_kernel_stamped(reps)[0]
 
 
 def batch_leg(steps, plain_steps=40, nboxes=512):
@@ -336,10 +336,10 @@ def profile_force(kernel, reps=20, warm=600, copy_mib=256, boundary="periodic", 
     bc = ljamd.BC_WALL if boundary == "wall" else ljamd.BC_PERIODIC
     with ljamd.LJSystem(n, RHO, T0, canonical=ljamd.TVN, boundary=bc, seed=SEED, jslices=jsl) as s:
         s.time_force_kernel(warm)
-        us = s.time_force_kernel(reps)
+        us, span = s.time_force_kernel_stamped(reps)
         pair_once, nslots = s.force_plan()
     print(json.dumps({"profile": "force", "N": n, "boundary": boundary, "warm": warm, "reps": reps,
-                      "kernel_us": us,
+                      "kernel_us": us, "kernel_us_back_to_back": span,
                       "pair_once": pair_once, "nslots": nslots, "copy_bytes": nbytes}))
 
 
@@ -498,9 +498,11 @@ def main():
                      "kernel": ("k_force_n3 (each unordered pair once, periodic fixed point)"
                                 if g["pair_once"] else "k_force (sliced, periodic fixed point)"),
                      "kernel_us": g["force_us"],
-                     "kernel_us_source": f"HIP events over {g['reps']} back-to-back launches "
-                                         "on the kernel's stream, this run, after "
+                     "kernel_us_source": f"mean over {g['reps']} back-to-back launches on the "
+                                         "kernel's stream of each launch's own start / stop HIP "
+                                         "events (hipExtLaunchKernel), this run, after "
                                          f"{g['settle_ms']:.0f} ms of settling force passes",
+                     "kernel_us_back_to_back": g["force_span_us"],
                      "traffic": profiled_traffic(kname),
                      "traffic_source": f"profiles/{kname}_n16384_periodic.md (rocprofv3 --pmc "
                                        "FETCH_SIZE and WRITE_SIZE, scaled by a calibration copy); "

@@ -258,6 +258,11 @@ int lj_device_arrays(lj_ctx* ctx, float** pos, float** vel, float** force);
 /* time of the force kernel alone, averaged over `reps` launches on the
  * context stream, measured with HIP events (for roofline reporting). */
 int lj_time_force_kernel(lj_ctx* ctx, int reps, double* us_per_launch);
+/* the same launches (1 <= reps <= 4096, FP32 mode), each stamped with its
+ * own start and stop events (hipExtLaunchKernel): us_kernel = the mean of the
+ * launches' own durations (what a kernel trace reports), us_span = events
+ * around all of them / reps (the back-to-back rate, dispatch gaps included) */
+int lj_time_force_kernel_stamped(lj_ctx* ctx, int reps, double* us_kernel, double* us_span);
 /* which force kernel the context runs: pair_once = 1 for the kernel that
  * evaluates each unordered pair once (FP32 mode, N >= 3072, j slices
  * automatic), 0 for the sliced kernel; nslots = partial sums per particle */

@@ -31,6 +31,7 @@ int fail(int code, const char* fmt, ...)
   return code;
 }
 const std::string& last_error() { return g_err; }
+thread_local LaunchStamp tl_force_stamp;
 }  // namespace lj
 
 namespace lj {
@@ -854,6 +855,48 @@ int lj_device_arrays(lj_ctx* c, float** pos, float** vel, float** force)
   if (pos) *pos = reinterpret_cast<float*>(c->pos);
   if (vel) *vel = reinterpret_cast<float*>(c->vel);
   if (force) *force = reinterpret_cast<float*>(c->force);
+  return LJ_OK;
+}
+
+int lj_time_force_kernel_stamped(lj_ctx* c, int reps, double* us_kernel, double* us_span)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (reps <= 0 || reps > 4096 || !us_kernel || !us_span)
+    return fail(LJ_ERR_ARG, "lj_time_force_kernel_stamped: bad arguments");
+  if (is_fp64(c)) return fail(LJ_ERR_STATE, "lj_time_force_kernel_stamped: FP32 mode only");
+  std::vector<hipEvent_t> ev((size_t)2 * reps, nullptr);
+  struct Cleanup {  // the stamp is cleared and the events destroyed on every path
+    std::vector<hipEvent_t>& ev;
+    ~Cleanup()
+    {
+      lj::tl_force_stamp = lj::LaunchStamp{};
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } cleanup{ev};
+  for (hipEvent_t& e : ev) LJ_HIP(hipEventCreate(&e));
+  hipStream_t st = c->stream;
+  if (c->ps.periodic) LJ_HIP(launch_prep_q(st, c->pos, c->q, c->cfg.N, c->ps.qscale));
+  const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
+  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false,
+                      c->step_ew));  // warm
+  for (int k = 0; k < reps; ++k) {
+    lj::tl_force_stamp = lj::LaunchStamp{ev[2 * k], ev[2 * k + 1]};
+    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false,
+                        c->step_ew));
+  }
+  lj::tl_force_stamp = lj::LaunchStamp{};
+  LJ_HIP(hipEventSynchronize(ev.back()));
+  double sum = 0.0;
+  for (int k = 0; k < reps; ++k) {
+    float ms = 0.f;
+    LJ_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+    sum += ms;
+  }
+  float span = 0.f;
+  LJ_HIP(hipEventElapsedTime(&span, ev[0], ev.back()));
+  *us_kernel = 1000.0 * sum / reps;
+  *us_span = 1000.0 * span / reps;
   return LJ_OK;
 }
 

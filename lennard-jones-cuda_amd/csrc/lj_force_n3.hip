@@ -815,16 +815,17 @@ hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& p
   const int nitems = ENERGY ? pl.nitems : pl.nitems_step;
   const int2* items = ENERGY ? pl.items : pl.items_step;
   dim3 grid(nitems * (SPLIT ? 2 : 1)), block(64 * W * W);
+  const LaunchStamp ts = tl_force_stamp;
   if (ps.periodic) {
     if (ties)
-      hipLaunchKernelGGL((k_force_n3<true, true, W, ENERGY, SPLIT>), grid, block, 0, st, q, posf,
-                         items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+      hipExtLaunchKernelGGL((k_force_n3<true, true, W, ENERGY, SPLIT>), grid, block, 0, st, ts.start,
+                            ts.stop, 0, q, posf, items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
     else
-      hipLaunchKernelGGL((k_force_n3<true, false, W, ENERGY, SPLIT>), grid, block, 0, st, q, posf,
-                         items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+      hipExtLaunchKernelGGL((k_force_n3<true, false, W, ENERGY, SPLIT>), grid, block, 0, st, ts.start,
+                            ts.stop, 0, q, posf, items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
   } else {
-    hipLaunchKernelGGL((k_force_n3<false, false, W, ENERGY, SPLIT>), grid, block, 0, st, q, posf,
-                       items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+    hipExtLaunchKernelGGL((k_force_n3<false, false, W, ENERGY, SPLIT>), grid, block, 0, st, ts.start,
+                          ts.stop, 0, q, posf, items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
   }
   return hipGetLastError();
 }

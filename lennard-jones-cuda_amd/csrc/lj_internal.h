@@ -2,6 +2,7 @@
 // host helpers of libljhip.so (not part of the public C-ABI).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -148,6 +149,14 @@ hipError_t launch_reset_averaging(hipStream_t st, lj_obs* obs);
 hipError_t launch_store_vw(hipStream_t st, const double* ev, lj_obs* obs);
 // obs->t = 0, obs->step = 0
 hipError_t launch_reset_clock(hipStream_t st, lj_obs* obs);
+
+// Start and stop events for the force-pass launches of this thread (both null
+// except inside lj_time_force_kernel_stamped): the launchers hand them to
+// hipExtLaunchKernel, which stamps them with the kernel's own start and end
+struct LaunchStamp {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchStamp tl_force_stamp;
 
 // RDF
 // Device workspace of the cell-list RDF pass (lj_rdf_cells.hip), owned by a

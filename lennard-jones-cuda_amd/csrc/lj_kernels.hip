@@ -781,14 +781,16 @@ hipError_t launch_force_t(hipStream_t st, const ForcePlan& pl, const PairScalars
       // t mod 8 by every slice's workgroup, as the 64-particle row t of the
       // integrator kernels that read its partials is scheduled (round robin)
       dim3 grid((unsigned)((((pl.N + 63) / 64) + 7) / 8 * 8), pl.S);
-      hipLaunchKernelGGL((k_force_split<PERIODIC, GUARDALL, COMP, TIES>), grid, dim3(kBlock), 0, st,
-                         q, posf, fpart, apart, pl.N, pl.Npad, pl.jchunk, ps.C1, ps.thr);
+      hipExtLaunchKernelGGL((k_force_split<PERIODIC, GUARDALL, COMP, TIES>), grid, dim3(kBlock), 0,
+                            st, tl_force_stamp.start, tl_force_stamp.stop, 0, q, posf, fpart, apart,
+                            pl.N, pl.Npad, pl.jchunk, ps.C1, ps.thr);
       return hipGetLastError();
     }
   }
   dim3 grid(pl.gx, pl.S);
-  hipLaunchKernelGGL((k_force<PERIODIC, GUARDALL, IPL, COMP, TIES>), grid, dim3(kBlock), 0, st, q,
-                     posf, fpart, apart, pl.N, pl.Npad, pl.jchunk, ps.C1, ps.thr);
+  hipExtLaunchKernelGGL((k_force<PERIODIC, GUARDALL, IPL, COMP, TIES>), grid, dim3(kBlock), 0, st,
+                        tl_force_stamp.start, tl_force_stamp.stop, 0, q, posf, fpart, apart, pl.N,
+                        pl.Npad, pl.jchunk, ps.C1, ps.thr);
   return hipGetLastError();
 }
 

@@ -117,6 +117,7 @@ SIGNATURES = [
     ("lj_synchronize", _i, [_vp]),
     ("lj_device_arrays", _i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     ("lj_time_force_kernel", _i, [_vp, _i, C.POINTER(_d)]),
+    ("lj_time_force_kernel_stamped", _i, [_vp, _i, C.POINTER(_d), C.POINTER(_d)]),
     ("lj_time_rdf", _i, [_vp, _i, _i, _f, _d, _i, C.POINTER(_d)]),
     ("lj_forces_device", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("lj_set_array_device", _i, [_vp, _i, _vp, _vp]),

@@ -169,6 +169,13 @@ class LJSystem:
         check(lib().lj_time_force_kernel(self.h, int(reps), C.byref(us)))
         return us.value
 
+    def time_force_kernel_stamped(self, reps=200):
+        """(mean of the launches' own durations, back-to-back time per launch),
+        microseconds, over `reps` force-pass launches (FP32 mode)."""
+        k, s = C.c_double(), C.c_double()
+        check(lib().lj_time_force_kernel_stamped(self.h, int(reps), C.byref(k), C.byref(s)))
+        return k.value, s.value
+
     def velocity_histogram(self, step, nbins):
         """Speed histogram with the binning of MDSystem::initvelo (uint64[nbins])."""
         out = np.zeros(int(nbins), np.uint64)

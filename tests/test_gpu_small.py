@@ -272,3 +272,23 @@ def test_maxwell_initial_state():
     # the Gaussian draw, for contrast, is Maxwell-distributed too (same law)
     assert stats.kstest(np.linalg.norm(v_g.astype(np.float64), axis=1),
                         stats.chi(df=3, scale=np.sqrt(T0)).cdf).pvalue > 1e-3
+
+
+@pytest.mark.parametrize("jslices", [0, 32])
+def test_stamped_force_timing(jslices):
+    """lj_time_force_kernel_stamped: each launch's own duration (hipExtLaunchKernel
+    start / stop events) is positive and no longer than the back-to-back time
+    per launch, for the pair-once (jslices 0 at N = 16384) and the sliced
+    kernel; the stamped launches leave the forces as the plain ones do;
+    out-of-range repetition counts are refused."""
+    N = 16384
+    with ljamd.LJSystem(N, 0.85, 1.0, seed=3, jslices=jslices) as s:
+        s.CalculateForces()
+        f0 = s.force.copy()
+        k, span = s.time_force_kernel_stamped(50)
+        assert 0.0 < k <= span * 1.02
+        s.CalculateForces()
+        assert np.array_equal(s.force, f0)
+        for bad in (0, 5000):
+            with pytest.raises(ljamd.LJError):
+                s.time_force_kernel_stamped(bad)
