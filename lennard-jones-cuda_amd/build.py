@@ -100,6 +100,14 @@ def build_tasks():
     return out
 
 
+def build_valu_rates():
+    """The issue-rate measurement program (csrc/valu_rates.hip)."""
+    exe = os.path.join(LIB, "valu_rates")
+    _run([HIPCC, *HIPFLAGS, "-x", "hip", os.path.join(CSRC, "valu_rates.hip"), "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_checkers():
     _run(["make", "-s", "-C", ORACLE, "oracle"])
     if os.path.isdir(os.path.join(REFERENCE, "src", "library")):
@@ -110,6 +118,7 @@ def build_all(product_only=False, verbose=False):
     t = build_product(verbose=verbose)
     build_driver()
     build_tasks()
+    build_valu_rates()
     if not product_only:
         build_checkers()
     return t
