@@ -136,6 +136,7 @@ int lj_set_step_sampling(lj_ctx* c, const lj_step_sampling* s)
       cpu_minimage_thresholds(c->L, a.thr);
     a.end = rule == LJ_RDF_REF_GPU ? e[nbins - 1] : e[nbins];
     LJ_TRY(attach_rdf_cells(c, a));
+    a.sampling = 1;
     a.hist = c->rdf_step;
     LJ_HIP(launch_fill_u64(st, c->rdf_step, nbins));
     LJ_HIP(hipStreamSynchronize(st));
@@ -159,10 +160,11 @@ int lj_step_rdf_counts(lj_ctx* c, uint64_t* counts, int64_t* nsamples, int clear
                           hipMemcpyDeviceToHost, st));
   unsigned bad = 0u;
   if (rdf_uses_cells(c->smp_args))
-    LJ_HIP(hipMemcpyAsync(&bad, c->smp_args.cells->result, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    LJ_HIP(hipMemcpyAsync(&bad, c->smp_args.cells->result_step, sizeof(unsigned),
+                          hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
   if (bad) {
-    LJ_HIP(hipMemsetAsync(c->smp_args.cells->result, 0, sizeof(unsigned), st));
+    LJ_HIP(hipMemsetAsync(c->smp_args.cells->result_step, 0, sizeof(unsigned), st));
     LJ_HIP(hipStreamSynchronize(st));
     return fail(LJ_ERR_STATE, "lj_step_rdf_counts: particle coordinates left [-L, 2L) during the "
                               "sampled steps; the step RDF is incomplete");

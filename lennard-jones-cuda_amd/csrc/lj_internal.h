@@ -173,7 +173,8 @@ struct RdfCellsWs {
   unsigned* reps = nullptr;        // [kRdfReps][cap_bins]
   unsigned long long* other = nullptr;  // [kRdfReps]
   unsigned* flag = nullptr;        // coordinates beyond [-L, 2 L) in the current pass
-  unsigned* result = nullptr;      // the last pass's flag, for the host
+  unsigned* result = nullptr;      // a flagged lj_rdf pass, for the host
+  unsigned* result_step = nullptr; // a flagged pass of the per-step sampling, for the host
   int* bcnt = nullptr;             // [cap_m]: per-block cell counts, block-major
   int cap_n = 0, cap_cells = 0, cap_bins = 0, cap_m = 0;
 };
@@ -195,6 +196,7 @@ struct RdfArgs {
   double end;        // the counted range's end, tab[nbins - 1] (REF_GPU) or tab[nbins]
   unsigned long long* hist;
   const RdfCellsWs* cells;  // may be null: no cell-list pass
+  int sampling;             // the per-step sampling's pass (its own result word)
 };
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
 hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a, int nc);

@@ -468,15 +468,16 @@ int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins)
   LJ_TRY(dev_alloc(&w.sorted, cn));
   LJ_TRY(dev_alloc(&w.reps, (size_t)kRdfReps * cb));
   LJ_TRY(dev_alloc(&w.other, kRdfReps));
-  LJ_TRY(dev_alloc(&w.flag, 2));
+  LJ_TRY(dev_alloc(&w.flag, 3));
   w.result = w.flag + 1;
+  w.result_step = w.flag + 2;
   LJ_TRY(dev_alloc(&w.bcnt, kCellScanMax));
   w.cap_m = kCellScanMax;
   // the counts, replicas and flags start at zero; the passes clear them after use
   if (hipMemset(w.count, 0, sizeof(int) * cc) != hipSuccess ||
       hipMemset(w.reps, 0, sizeof(unsigned) * kRdfReps * cb) != hipSuccess ||
       hipMemset(w.other, 0, sizeof(unsigned long long) * kRdfReps) != hipSuccess ||
-      hipMemset(w.flag, 0, sizeof(unsigned) * 2) != hipSuccess ||
+      hipMemset(w.flag, 0, sizeof(unsigned) * 3) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess)
     return fail(LJ_ERR_DEVICE, "rdf_cells_ensure: clearing the workspace failed");
   w.cap_n = cn;
@@ -496,7 +497,7 @@ void rdf_cells_free(RdfCellsWs& w)
   dev_free(w.other);
   dev_free(w.flag);
   dev_free(w.bcnt);
-  w.result = nullptr;
+  w.result = w.result_step = nullptr;
   w.cap_m = 0;
   w.cap_n = w.cap_cells = w.cap_bins = 0;
 }
@@ -549,7 +550,8 @@ hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a, int nc)
 #undef LJ_CELLS_ARGS
   const unsigned long long P = (unsigned long long)a.N * (unsigned long long)(a.N - 1) / 2ull;
   hipLaunchKernelGGL(k_rdf_cells_reduce, dim3(a.nbins), dim3(kRdfReps), 0, st, w.reps, w.other,
-                     a.nbins, a.rule == LJ_RDF_REF_GPU ? 1 : 0, P, w.flag, w.result, a.hist);
+                     a.nbins, a.rule == LJ_RDF_REF_GPU ? 1 : 0, P, w.flag,
+                     a.sampling ? w.result_step : w.result, a.hist);
   return hipGetLastError();
 }
 

@@ -197,3 +197,30 @@ np.savez(sys.argv[2], *out)
         assert np.array_equal(a, b)
     assert on[0].sum() == N * (N - 1) and on[2].sum() == N * (N - 1)
     assert 0 < on[1].sum() < N * (N - 1)
+
+
+def test_step_rdf_out_of_span_reported_after_lj_rdf():
+    """A coordinate outside [-L, 2 L) seen by a sampled step's RDF pass makes
+    lj_step_rdf_counts fail (the cell-list pass counted nothing for that
+    step), also when an lj_rdf call (whose own flag is separate) comes in
+    between; the sampling then resumes from a state inside the box."""
+    N, rho = 6000, 0.85
+    L, pos, vel = perturbed_state(N, rho, 1.0, seed=8)
+    p = pos.reshape(-1, 4).copy()
+    p[17, 0] += np.float32(3.0 * L)
+    with ljamd.LJSystem(N, rho, 1.0, canonical=ljamd.TVN) as s:
+        s.set_state(p.reshape(-1), vel)
+        s.CalculateForces()
+        s.set_step_sampling(rdf=ljamd.SAMPLE_RDF_SUM, rdf_rule=ljamd.RDF_REF_GPU, rdf_nbins=256)
+        s.Integrate(0.005, 1)  # the force pass sees x + 3 L; the step wraps it
+        got = s.rdf(ljamd.RDF_REF_GPU, 256, dr2=ORC.orc_rdf_dr2(N))
+        assert got.sum() == N * (N - 1)
+        with pytest.raises(ljamd.LJError):
+            s.step_rdf_counts()
+        # the periodic wrap moves a coordinate by one L per step: start again
+        # from the state inside the box
+        s.set_state(pos, vel)
+        s.step_rdf_counts(clear=True)  # the flag was reported once
+        s.Integrate(0.005, 2)
+        counts, n = s.step_rdf_counts()
+        assert n == 2 and counts.sum() == 2 * N * (N - 1)
