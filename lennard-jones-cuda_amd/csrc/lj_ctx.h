@@ -96,7 +96,8 @@ inline int attach_rdf_cells(lj_ctx* c, RdfArgs& a)
 {
   a.cells = nullptr;
   if (!a.periodic || a.nbins > 1024) return LJ_OK;
-  const int nc = rdf_cells_per_axis(a.L, a.end);
+  int reach = 1;
+  const int nc = rdf_cells_layout(a.L, a.end, &reach);
   if (nc <= 0) return LJ_OK;
   const int rc = rdf_cells_ensure(c->rdf_cells, a.N, nc * nc * nc, a.nbins);
   if (rc != LJ_OK) return rc;

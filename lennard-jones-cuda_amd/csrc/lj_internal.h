@@ -165,7 +165,8 @@ constexpr int kRdfReps = 64;       // replicas of the workgroup histograms
 constexpr int kRdfMaxCells = 64;   // cells per axis at most
 constexpr int kCellScanMax = 1 << 16;  // (cells x blocks) of the LDS-counted sort
 struct RdfCellsWs {
-  int* count = nullptr;            // [cap_cells]: particles per cell (cleared by the scan)
+  int* count = nullptr;            // [cap_cells]: particles per cell (cleared by the scan or
+                                   // the reduction)
   int* start = nullptr;            // [cap_cells + 1]
   int* cellof = nullptr;           // [cap_n]
   int* rank = nullptr;             // [cap_n]
@@ -175,11 +176,13 @@ struct RdfCellsWs {
   unsigned* flag = nullptr;        // coordinates beyond [-L, 2 L) in the current pass
   unsigned* result = nullptr;      // a flagged lj_rdf pass, for the host
   unsigned* result_step = nullptr; // a flagged pass of the per-step sampling, for the host
-  int* bcnt = nullptr;             // [cap_m]: per-block cell counts, block-major
+  int* bcnt = nullptr;             // [cap_m]: each block's offset within each cell, block-major
   int cap_n = 0, cap_cells = 0, cap_bins = 0, cap_m = 0;
 };
 // cells per axis for a periodic range ending at r^2 = end, or 0 (fewer than 3)
 int rdf_cells_per_axis(double L, double end);
+// cells per axis of the layout the pass takes (0: none), and its reach (1 or 2)
+int rdf_cells_layout(double L, double end, int* reach);
 int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins);  // LJ_OK or a status
 void rdf_cells_free(RdfCellsWs& w);
 
@@ -199,7 +202,7 @@ struct RdfArgs {
   int sampling;             // the per-step sampling's pass (its own result word)
 };
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
-hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a, int nc);
+hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a);
 // whether launch_rdf takes the cell-list pass for these arguments
 bool rdf_uses_cells(const RdfArgs& a);
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);

@@ -75,15 +75,17 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_assign(const float4* __res
 }
 
 // The same with the counts in LDS (ncell <= kCellLds): each block ranks its
-// particles in a private histogram and writes its count of every cell to
-// bcnt[block * ncell + c], from which k_cells_place finds the cells' starts
-// and each block's offset within a cell -- no global atomics, which on a few
-// hundred counters serialised (28 us at N = 16384 over 125 cells, against 3 us).
+// particles in a private histogram, then adds each cell's count to the
+// cell's total with one global atomic, whose old value is the block's offset
+// within the cell (bcnt[block * ncell + c]).  One atomic per (block, cell)
+// instead of one per particle, which on a few hundred counters serialised
+// (28 us at N = 16384 over 125 cells, against 3 us).
 constexpr int kCellLds = 4096;
 constexpr int kAssignPer = 4;  // particles per thread of the LDS-counted assignment
 __global__ __launch_bounds__(kCellBlock) void k_cells_assign_lds(const float4* __restrict__ pos, int N,
                                                                  double L, double qs, int nc,
-                                                                 int ncell, int nblk,
+                                                                 int ncell,
+                                                                 int* __restrict__ count,
                                                                  int* __restrict__ bcnt,
                                                                  int* __restrict__ cellof,
                                                                  int* __restrict__ rank,
@@ -106,17 +108,20 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_assign_lds(const float4* _
       atomicOr(flag, 1u);
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < ncell; c += kCellBlock) bcnt[(size_t)blockIdx.x * ncell + c] = h[c];
+  // the block's place within each cell it holds particles of (the blocks'
+  // order within a cell varies from pass to pass; the counts do not)
+  for (int c = threadIdx.x; c < ncell; c += kCellBlock)
+    if (h[c]) bcnt[(size_t)blockIdx.x * ncell + c] = atomicAdd(&count[c], h[c]);
 }
 
-// After k_cells_assign_lds, one launch over the same blocks: each block sums
-// the per-block counts into the cells' totals, scans them into the cells'
-// starts (block 0 also writes them to start), adds the counts of the blocks
-// before it, and scatters its own particles.  The scan is repeated by every
-// block (ncell x nblk reads each, from L2) instead of a separate one-block
-// launch.
+// After k_cells_assign_lds, one launch over the same blocks: each block scans
+// the cells' totals into their starts (block 0 also writes them to start) and
+// scatters its own particles to start + the block's offset + rank.  The scan
+// is repeated by every block (ncell reads each, from L2) instead of a
+// separate one-block launch; the totals are cleared by k_rdf_cells_reduce.
 __global__ __launch_bounds__(kCellBlock) void k_cells_place(const float4* __restrict__ pos, int N,
-                                                            double qs, int ncell, int nblk,
+                                                            double qs, int ncell,
+                                                            const int* __restrict__ count,
                                                             const int* __restrict__ bcnt,
                                                             const int* __restrict__ cellof,
                                                             const int* __restrict__ rank,
@@ -129,15 +134,7 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_place(const float4* __rest
   int carry = 0;
   for (int base = 0; base < ncell; base += kCellBlock) {
     const int k = base + (int)threadIdx.x;
-    int tot = 0, mine = 0;
-    if (k < ncell) {
-#pragma unroll 8
-      for (int t = 0; t < nblk; ++t) {
-        const int v = bcnt[(size_t)t * ncell + k];
-        tot += v;
-        mine += t < (int)blockIdx.x ? v : 0;
-      }
-    }
+    const int tot = k < ncell ? count[k] : 0;
     int x = tot;  // inclusive scan inside the wave
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -154,7 +151,7 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_place(const float4* __rest
     }
     if (k < ncell) {
       const int ex = before + x - tot;
-      cs[k] = ex + mine;
+      cs[k] = ex;
       if (blockIdx.x == 0) start[k] = ex;
     }
     carry += total;
@@ -165,7 +162,8 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_place(const float4* __rest
     const int i = blockIdx.x * (kCellBlock * kAssignPer) + u * kCellBlock + (int)threadIdx.x;
     if (i >= N) break;
     const float4 p = pos[i];
-    sorted[cs[cellof[i]] + rank[i]] =
+    const int c = cellof[i];
+    sorted[cs[c] + bcnt[(size_t)blockIdx.x * ncell + c] + rank[i]] =
         make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), (unsigned)i);
   }
 }
@@ -214,8 +212,8 @@ __global__ __launch_bounds__(1024) void k_cells_scan(int* __restrict__ count, in
   if (threadIdx.x == 0) start[n] = carry;
 }
 
-// nblk > 0: start holds the blocks' offsets per cell (k_cells_assign_lds);
-// else the cells' starts with global ranks
+// the particles to their cells' starts + their ranks (k_cells_assign's
+// global counters)
 __global__ __launch_bounds__(kCellBlock) void k_cells_scatter(const float4* __restrict__ pos, int N,
                                                               double qs,
                                                               const int* __restrict__ cellof,
@@ -399,6 +397,182 @@ __global__ __launch_bounds__(kCellBlock) void k_rdf_cells(
                         flag, reps, other);
 }
 
+// Reach 2: cells of side at least sqrt(X) / 2, so every pair below the far
+// bound lies within two cells on each axis.  The candidate volume per
+// particle is 125 cells of side s / 2 (a 5^3 block, 125 / 8 = 15.6 cells of
+// side s) against 27 cells of side s for reach 1, 42% less.  One workgroup
+// per cell: the cell's own particles and those of its 62 forward neighbours
+// in [-2, 2]^3 are staged in LDS (self first, kJStage2 at a time), and the
+// 256 lanes are spread over the cell's n particles and 256 / n phases of the
+// staged j, so cells of any occupancy keep almost every lane busy.  The
+// classification and counting are those of k_rdf_cells.
+constexpr int kNb2 = 63;         // the cell itself, then its 62 forward neighbours
+constexpr int kJStage2 = 1536;   // staged j (24 KiB)
+
+// offset o = 0..62 of the reach-2 half shell: (dx, dy, dz) in [-2, 2]^3 at
+// lexicographic (dz, dy, dx) index 62 + o (62 = the cell itself)
+__device__ __forceinline__ void shell2_offset(int o, int& dx, int& dy, int& dz)
+{
+  const int k = o + 62;
+  dx = k % 5 - 2;
+  dy = (k / 5) % 5 - 2;
+  dz = k / 25 - 2;
+}
+
+template <int RULE>
+__global__ __launch_bounds__(kCellBlock) void k_rdf_cells2(
+    const uint4* __restrict__ sorted, const int* __restrict__ start, const float4* __restrict__ pos,
+    int nc, double L, int nbins, const float* __restrict__ tab, float est_scale, float t0, float t1,
+    float t2, float t3, double qscale, const unsigned* __restrict__ flag,
+    unsigned* __restrict__ reps, unsigned long long* __restrict__ other)
+{
+  constexpr bool TAB = RULE != LJ_RDF_RBINS;
+  constexpr bool GPU = RULE == LJ_RDF_REF_GPU;
+  if (*flag) return;  // coordinates beyond the bound's span: no count
+  const int c = blockIdx.x;
+  const int i0 = start[c], ni = start[c + 1] - i0;
+  if (ni == 0) return;  // uniform, before any barrier
+  const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
+
+  // dynamic LDS as k_rdf_cells
+  extern __shared__ float2 sh_cells[];
+  float2* T2 = sh_cells;
+  float* T = reinterpret_cast<float*>(T2 + (TAB ? nbins : 0));
+  unsigned int* hs = reinterpret_cast<unsigned int*>(T + nbins + 1);
+  __shared__ unsigned long long s_other;
+  __shared__ int s_cell[kNb2];
+  __shared__ int s_off[kNb2 + 1];  // s_off[k]: staged index of neighbour k's first particle
+  __shared__ uint4 sj[kJStage2];
+  for (int b = threadIdx.x; b < kWaves * nbins; b += kCellBlock) hs[b] = 0u;
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  unsigned int* hw = hs + wave * nbins;
+  const double S2 = qscale * qscale;
+  for (int b = threadIdx.x; b <= nbins; b += kCellBlock) {
+    const float t = tab[b];
+    T[b] = t;
+    if constexpr (TAB) {
+      const float tq = (float)((double)t * S2);
+      if (b < nbins) T2[b].x = tq;
+      if (b > 0) T2[b - 1].y = tq;
+    }
+  }
+  if (wave == 0) {  // the neighbours, their counts scanned across the wave
+    int cnt = 0;
+    if (lane < kNb2) {
+      int dx, dy, dz;
+      shell2_offset(lane, dx, dy, dz);
+      const int c2 = (cx + dx + nc) % nc + nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc));
+      s_cell[lane] = c2;
+      cnt = start[c2 + 1] - start[c2];
+    }
+    int x = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane < kNb2) s_off[lane + 1] = x;
+    if (lane == 0) s_off[0] = 0;
+  }
+  if (threadIdx.x == 0) s_other = 0ull;
+  __syncthreads();
+  const int total = s_off[kNb2];
+  const int nself = s_off[1];  // the cell's own particles, staged first
+
+  const float thr[4] = {t0, t1, t2, t3};
+  const float Tmax = T[nbins], Tlast = T[nbins - 1];
+  RdfClass<RULE> cl;
+  cl.init(3. * L, L, qscale, est_scale, nbins, Tlast, Tmax);
+  unsigned int nexcl = 0u;  // GPU rule: excluded pairs r2 <= 1e-6 (this lane's)
+
+  const int tid = (int)threadIdx.x;
+  for (int ib = 0; ib < ni; ib += kCellBlock) {
+    const int n = min(ni - ib, kCellBlock);
+    const int P = kCellBlock / n;  // phases over the staged j
+    const int li = tid % n, ph = tid / n;
+    const bool act = ph < P;
+    const int iloc = ib + li;  // the lane's particle, by its place in the cell
+    const uint4 qi = sorted[i0 + iloc];
+    for (int jb = 0; jb < total; jb += kJStage2) {
+      const int jn = min(kJStage2, total - jb);
+      __syncthreads();  // the previous stage is no longer read
+      for (int e = tid; e < jn; e += kCellBlock) {
+        const int g = jb + e;
+        int lo = 0, hi = kNb2;  // s_off[lo] <= g < s_off[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_off[mid] <= g) lo = mid;
+          else hi = mid;
+        }
+        sj[e] = sorted[start[s_cell[lo]] + (g - s_off[lo])];
+      }
+      __syncthreads();
+      if (act) {
+        // j = t0 + b P, 8 at a time; in the cell itself only j after the
+        // lane's particle (each unordered pair once)
+        for (int tb = ph; tb < jn; tb += 8 * P) {
+          constexpr int B = 8;
+          int kk[B];
+          bool dec[B], gone[B];
+          if constexpr (TAB) {
+            float lo[B], hi[B];
+            float2 tt[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+              cl.measure(rdf_r2q(qi, sj[min(tb + b * P, jn - 1)]), kk[b], lo[b], hi[b]);
+              tt[b] = T2[kk[b]];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) cl.decide(kk[b], lo[b], hi[b], tt[b], dec[b], gone[b]);
+          } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+              cl.rbins(rdf_r2q(qi, sj[min(tb + b * P, jn - 1)]), kk[b], dec[b], gone[b]);
+          }
+          unsigned int mask = 0u;
+#pragma unroll
+          for (int b = 0; b < B; ++b) {
+            const int t = tb + b * P, g = jb + t;
+            const bool ok = (t < jn) & ((g >= nself) | (g > iloc));
+            if (ok & dec[b]) atomicAdd(&hw[kk[b]], 1u);
+            mask |= (ok & !dec[b] & !gone[b]) ? (1u << b) : 0u;
+          }
+          while (mask) {
+            const int t = tb + __builtin_ctz(mask) * P;
+            mask &= mask - 1u;
+            const int bin = rdf_exact_bin<RULE, true>(pos[qi.w], pos[sj[t].w], L, (float)L, thr, T,
+                                                      nbins, Tmax, Tlast, est_scale);
+            if constexpr (GPU) {
+              if (bin != nbins - 1) {
+                if (bin >= 0) atomicAdd(&hw[bin], 1u);
+                else ++nexcl;
+              }
+            } else {
+              if (bin >= 0) atomicAdd(&hw[bin], 1u);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  unsigned* rep = reps + (size_t)(blockIdx.x % kRdfReps) * nbins;
+  unsigned long long tot = nexcl;  // GPU rule: this thread's share of the pass's counted pairs
+  for (int b = threadIdx.x; b < nbins; b += kCellBlock) {
+    unsigned int v = 0u;
+    for (int w = 0; w < kWaves; ++w) v += hs[w * nbins + b];
+    if (v) atomicAdd(&rep[b], v);
+    tot += v;
+  }
+  if constexpr (GPU) {
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) tot += __shfl_xor(tot, s, 64);
+    if (lane == 0 && tot) atomicAdd(&s_other, tot);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_other) atomicAdd(&other[blockIdx.x % kRdfReps], s_other);
+  }
+}
+
 // Sums the replicas into hist and clears them (one wave per bin); the GPU
 // rule's clamp bin gets every pair the pass did not count elsewhere, 2 (P -
 // other) as ordered pairs; clears the flag after noting it in result for the
@@ -408,10 +582,12 @@ __global__ __launch_bounds__(kCellBlock) void k_rdf_cells(
 __global__ __launch_bounds__(kRdfReps) void k_rdf_cells_reduce(
     unsigned* __restrict__ reps, unsigned long long* __restrict__ other, int nbins, int gpu,
     unsigned long long P, unsigned* __restrict__ flag, unsigned* __restrict__ result,
-    unsigned long long* __restrict__ hist)
+    unsigned long long* __restrict__ hist, int* __restrict__ count, int ncell)
 {
   static_assert(kRdfReps == 64, "one wave per bin");
   const int b = blockIdx.x, r = threadIdx.x;
+  // the cells' totals of the LDS-counted sort, zero for the next pass
+  for (int c = b * kRdfReps + r; c < ncell; c += gridDim.x * kRdfReps) count[c] = 0;
   const bool bad = *flag != 0u;
   unsigned long long s = reps[(size_t)r * nbins + b];
   reps[(size_t)r * nbins + b] = 0u;
@@ -433,12 +609,13 @@ __global__ __launch_bounds__(kRdfReps) void k_rdf_cells_reduce(
 
 }  // namespace
 
-// the cell count per axis for a range ending at `end` (real r^2 units), or 0
-// when the cell list does not apply (fewer than 3 cells per axis, or more
-// than kRdfMaxCells per axis)
-int rdf_cells_per_axis(double L, double end)
+namespace {
+
+// sqrt(X) in fixed-point units: the side a reach-1 cell needs (0 for an
+// invalid range)
+double rdf_far_side(double L, double end)
 {
-  if (!(L > 0.) || !(end > 0.) || !std::isfinite(end)) return 0;
+  if (!(L > 0.) || !(end > 0.) || !std::isfinite(end)) return 0.;
   // the device's far bound at the guaranteed span A = 3 L (RdfClass::far_bound)
   const double qs = 4294967296. / L, S2 = qs * qs, u = 0x1p-24, A = 3. * L;
   const double Krd = 1.01 * 3.4641016151377546 * (2. * A + 3. * L) * u * qs;
@@ -449,10 +626,45 @@ int rdf_cells_per_axis(double L, double end)
   // the float rounding of X up, and 1e-6 beside it: the cell side (in
   // fixed-point units) must square to at least X
   const double X = (Tq + 2. * dl2) * (1. + 0x1p-20) * (1. + 1e-6);
-  const double side = std::sqrt(X);  // fixed-point units
-  const int nc = (int)std::floor(4294967296. / side);
-  if (nc < 3 || nc > kRdfMaxCells) return 0;
-  return nc;
+  return std::sqrt(X);
+}
+
+}  // namespace
+
+// reach-1 cells per axis for a range ending at `end` (real r^2 units): at
+// most kRdfMaxCells, or 0 when fewer than 3 fit
+int rdf_cells_per_axis(double L, double end)
+{
+  const double side = rdf_far_side(L, end);
+  if (!(side > 0.)) return 0;
+  // larger cells than needed are as exact: at most kRdfMaxCells per axis
+  const int nc = (int)std::min((double)kRdfMaxCells, std::floor(4294967296. / side));
+  return nc >= 3 ? nc : 0;
+}
+
+// cells per axis and reach (1: side >= sqrt(X), the cell and 13 forward
+// neighbours; 2: side >= sqrt(X) / 2, the cell and 62 forward neighbours) for
+// a periodic range ending at r^2 = end, or 0 where reach 1 gives fewer than 3
+// cells per axis.  The layout with the smaller candidate volume, 27 / nc1^3
+// against 125 / nc2^3 of the box, is taken; reach 2 needs 5 or more cells per
+// axis (distinct neighbours).  LJ_RDF_CELLS=1 / 2 holds the reach at 1 / 2
+// (2 where it applies).
+int rdf_cells_layout(double L, double end, int* reach)
+{
+  *reach = 1;
+  const int nc1 = rdf_cells_per_axis(L, end);
+  if (nc1 <= 0) return 0;
+  const double side = rdf_far_side(L, end);
+  const int nc2 = (int)std::min((double)kRdfMaxCells, std::floor(2. * 4294967296. / side));
+  static const char* e = std::getenv("LJ_RDF_CELLS");
+  if (nc2 >= 5) {
+    const bool auto2 = 125. / ((double)nc2 * nc2 * nc2) < 27. / ((double)nc1 * nc1 * nc1);
+    if ((e && e[0] == '2') || (!(e && e[0] == '1') && auto2)) {
+      *reach = 2;
+      return nc2;
+    }
+  }
+  return nc1;
 }
 
 int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins)
@@ -510,14 +722,17 @@ bool rdf_uses_cells(const RdfArgs& a)
   static const char* e = std::getenv("LJ_RDF_CELLS");
   if (e && e[0] == '0') return false;
   if (!a.cells || !a.periodic || a.nbins > 1024) return false;
-  const int nc = rdf_cells_per_axis(a.L, a.end);
+  int reach = 1;
+  const int nc = rdf_cells_layout(a.L, a.end, &reach);
   return nc > 0 && a.cells->flag && a.N <= a.cells->cap_n && nc * nc * nc <= a.cells->cap_cells &&
          a.nbins <= a.cells->cap_bins;
 }
 
-hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a, int nc)
+hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a)
 {
   const RdfCellsWs& w = *a.cells;
+  int reach = 1;
+  const int nc = rdf_cells_layout(a.L, a.end, &reach);
   const int ncell = nc * nc * nc;
   const double qs = 4294967296. / a.L;
   const int nb = (a.N + kCellBlock - 1) / kCellBlock;
@@ -526,9 +741,9 @@ hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a, int nc)
     // per-block counts in LDS; the cells' starts and the blocks' offsets
     // found and the particles placed by the same blocks in a second launch
     hipLaunchKernelGGL(k_cells_assign_lds, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, a.L, qs, nc,
-                       ncell, na, w.bcnt, w.cellof, w.rank, w.flag);
-    hipLaunchKernelGGL(k_cells_place, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, qs, ncell, na,
-                       w.bcnt, w.cellof, w.rank, w.start, w.sorted);
+                       ncell, w.count, w.bcnt, w.cellof, w.rank, w.flag);
+    hipLaunchKernelGGL(k_cells_place, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, qs, ncell,
+                       w.count, w.bcnt, w.cellof, w.rank, w.start, w.sorted);
   } else {
     hipLaunchKernelGGL(k_cells_assign, dim3(nb), dim3(kCellBlock), 0, st, a.pos, a.N, a.L, qs, nc,
                        w.count, w.cellof, w.rank, w.flag);
@@ -538,20 +753,25 @@ hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a, int nc)
   }
   const size_t lds = (a.rule != LJ_RDF_RBINS ? sizeof(float2) * a.nbins : 0) +
                      sizeof(float) * (a.nbins + 1) + sizeof(unsigned) * kWaves * a.nbins;
-  const dim3 grid(ncell, 14);
 #define LJ_CELLS_ARGS w.sorted, w.start, a.pos, nc, a.L, a.nbins, a.tab, a.est_scale, a.thr[0], \
                       a.thr[1], a.thr[2], a.thr[3], qs, w.flag, w.reps, w.other
-  if (a.rule == LJ_RDF_REF_CPU)
-    hipLaunchKernelGGL(k_rdf_cells<LJ_RDF_REF_CPU>, grid, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);
-  else if (a.rule == LJ_RDF_REF_GPU)
-    hipLaunchKernelGGL(k_rdf_cells<LJ_RDF_REF_GPU>, grid, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);
-  else
-    hipLaunchKernelGGL(k_rdf_cells<LJ_RDF_RBINS>, grid, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);
+#define LJ_CELLS_LAUNCH(K, G)                                                                    \
+  do {                                                                                           \
+    if (a.rule == LJ_RDF_REF_CPU)                                                                \
+      hipLaunchKernelGGL(K<LJ_RDF_REF_CPU>, G, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);        \
+    else if (a.rule == LJ_RDF_REF_GPU)                                                           \
+      hipLaunchKernelGGL(K<LJ_RDF_REF_GPU>, G, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);        \
+    else                                                                                         \
+      hipLaunchKernelGGL(K<LJ_RDF_RBINS>, G, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);          \
+  } while (0)
+  if (reach == 2) LJ_CELLS_LAUNCH(k_rdf_cells2, dim3(ncell));
+  else LJ_CELLS_LAUNCH(k_rdf_cells, dim3(ncell, 14));
+#undef LJ_CELLS_LAUNCH
 #undef LJ_CELLS_ARGS
   const unsigned long long P = (unsigned long long)a.N * (unsigned long long)(a.N - 1) / 2ull;
   hipLaunchKernelGGL(k_rdf_cells_reduce, dim3(a.nbins), dim3(kRdfReps), 0, st, w.reps, w.other,
                      a.nbins, a.rule == LJ_RDF_REF_GPU ? 1 : 0, P, w.flag,
-                     a.sampling ? w.result_step : w.result, a.hist);
+                     a.sampling ? w.result_step : w.result, a.hist, w.count, ncell);
   return hipGetLastError();
 }
 

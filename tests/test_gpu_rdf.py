@@ -23,11 +23,12 @@ RULES = {RDF_REF_CPU: ljamd.RDF_REF_CPU, RDF_REF_GPU: ljamd.RDF_REF_GPU,
     (3001, 0.3, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),
     (16384, 0.85, ljamd.BC_PERIODIC, RDF_RBINS, 500),  # the N = 16384 workload: 500 bins
     (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 256),
-    # the cell-list pass (periodic, 3 or more cells of side >= ~5.06 per axis):
-    (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),  # 5 cells per axis
-    (3000, 0.5, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),     # L = 18.2: 3 cells per axis
-    (6000, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 256),    # L = 19.2: 3 cells per axis
-    (9000, 0.3, ljamd.BC_PERIODIC, RDF_REF_GPU, 128),     # L = 31.1: 6 cells, range r^2 < 12.8
+    # the cell-list pass (periodic; cells per axis at reach 1 / reach 2, the
+    # layout taken by default being reach 2 in all four):
+    (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),  # 5 / 10
+    (3000, 0.5, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),     # L = 18.2: 3 / 7
+    (6000, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 256),    # L = 19.2: 3 / 7
+    (9000, 0.3, ljamd.BC_PERIODIC, RDF_REF_GPU, 128),     # L = 31.1, range r^2 < 12.8: 8 / 17
 ])
 def test_rdf_bitexact(N, rho, bc, rule, nbins):
     L, pos, _ = perturbed_state(N, rho, 1.0, seed=31)
@@ -113,9 +114,10 @@ def test_rdf_cells_out_of_span_falls_back():
 
 
 def _cells_on_and_off(code, tmp_path, timeout=300):
-    """Run `code` (argv: package path, output .npz) in two child processes,
-    LJ_RDF_CELLS=1 (the cell-list pass where it applies) and LJ_RDF_CELLS=0
-    (the all-pairs pass), and return both lists of saved arrays."""
+    """Run `code` (argv: package path, output .npz) in three child processes:
+    LJ_RDF_CELLS=1 (the cell-list pass with reach 1: cells of side >= the far
+    bound's root), =2 (reach 2: half that side) and =0 (the all-pairs pass).
+    Return the three lists of saved arrays."""
     import os
     import subprocess
     import sys
@@ -123,7 +125,7 @@ def _cells_on_and_off(code, tmp_path, timeout=300):
     here = os.path.dirname(os.path.abspath(__file__))
     py = os.path.join(os.path.dirname(here), "lennard-jones-cuda_amd", "python")
     res = {}
-    for cells in ("1", "0"):
+    for cells in ("1", "2", "0"):
         outf = str(tmp_path / f"cells{cells}.npz")
         env = dict(os.environ, LJ_RDF_CELLS=cells, PYTHONPATH=here + os.pathsep + py)
         r = subprocess.run([sys.executable, "-c", code, py, outf], env=env, cwd=here,
@@ -132,14 +134,15 @@ def _cells_on_and_off(code, tmp_path, timeout=300):
         assert r.returncode == 0, r.stderr[-2000:]
         with np.load(outf) as z:
             res[cells] = [z[k] for k in sorted(z.files, key=lambda t: int(t.split("_")[1]))]
-    return res["1"], res["0"]
+    return res["1"], res["2"], res["0"]
 
 
 def test_step_rdf_cells_equal_all_pairs_pass(tmp_path):
-    """The step RDF of a periodic box at N = 6000 (3 cells per axis): summed
-    over 10 captured steps with the cell-list pass, and in a child process
-    with LJ_RDF_CELLS=0 (the all-pairs pass) from the same state: the same
-    counts, bit for bit, for both reference rules; the trajectory too."""
+    """The step RDF of a periodic box at N = 6000 (3 cells per axis at reach
+    1, 7 at reach 2): summed over 10 captured steps with each cell-list
+    layout, and in a child process with LJ_RDF_CELLS=0 (the all-pairs pass)
+    from the same state: the same counts, bit for bit, for both reference
+    rules; the trajectory too."""
     code = r"""
 import sys, numpy as np
 sys.path.insert(0, sys.argv[1])
@@ -159,17 +162,19 @@ for rule in (ljamd.RDF_REF_GPU, ljamd.RDF_REF_CPU):
         out.append(s.pos.astype(np.float64))
 np.savez(sys.argv[2], *out)
 """
-    on, off = _cells_on_and_off(code, tmp_path)
-    for a, b in zip(on, off):
-        assert np.array_equal(a, b)
-    assert on[0].sum() > 0 and on[2].sum() > 0
+    on1, on2, off = _cells_on_and_off(code, tmp_path)
+    for a, b, c in zip(on1, on2, off):
+        assert np.array_equal(a, c)
+        assert np.array_equal(b, c)
+    assert on1[0].sum() > 0 and on1[2].sum() > 0
 
 
 @pytest.mark.parametrize("N", [70000, 131072])
 def test_rdf_cells_large_boxes_equal_all_pairs_pass(N, tmp_path):
     """Above 65536 particles the cell sort runs over many workgroups: per-block
-    counts in LDS (N = 70000: 8^3 cells) or global counters (N = 131072: 10^3
-    cells).  Both reference rules, perturbed lattice and after 20 TVN steps:
+    counts in LDS (N = 70000: 8^3 cells at reach 1) or global counters (N =
+    131072: 10^3 cells at reach 1; reach 2 has 17^3 and 21^3 cells, global
+    counters).  Both reference rules, perturbed lattice and after 20 TVN steps:
     the same counts as the all-pairs pass, bit for bit, and every pair counted
     (the GPU rule's total is N (N - 1): each ordered pair once)."""
     code = r"""
@@ -192,11 +197,12 @@ with ljamd.LJSystem(N, 0.85, 1.0, canonical=ljamd.TVN) as s:
         out.append(s.rdf(rule, 256, dr2=dr2).astype(np.int64))
 np.savez(sys.argv[2], *out)
 """.replace("int(sys.argv[3])", str(N))
-    on, off = _cells_on_and_off(code, tmp_path)
-    for a, b in zip(on, off):
-        assert np.array_equal(a, b)
-    assert on[0].sum() == N * (N - 1) and on[2].sum() == N * (N - 1)
-    assert 0 < on[1].sum() < N * (N - 1)
+    on1, on2, off = _cells_on_and_off(code, tmp_path)
+    for a, b, c in zip(on1, on2, off):
+        assert np.array_equal(a, c)
+        assert np.array_equal(b, c)
+    assert on1[0].sum() == N * (N - 1) and on1[2].sum() == N * (N - 1)
+    assert 0 < on1[1].sum() < N * (N - 1)
 
 
 def test_step_rdf_out_of_span_reported_after_lj_rdf():
