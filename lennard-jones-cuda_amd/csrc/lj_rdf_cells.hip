@@ -2,15 +2,18 @@
 // counted range is short against the box (the reference rules' r^2 < 25.6 at
 // liquid densities: N = 16384, rho* = 0.85 has 5 cells per axis).
 //
-// The particles are sorted into nc^3 cells of side s >= sqrt(X), X the
-// classification's far bound (lj_rdf_device.h): every pair whose fixed-point
-// r2 is below X lies in the same or adjacent cells, and every other pair is,
-// by the error bound, beyond the counted range in the reference's arithmetic
-// too -- dropped (CPU rule, r-bins) or in the clamped last bin (GPU rule).  A
-// work item is (cell c, forward offset o): the cell's own pairs once (o = 0)
-// or its pairs with one of the 13 forward neighbours (half a shell), so every
-// unordered pair of adjacent cells is met once (nc >= 3 makes the 27
-// neighbours distinct).  Its pairs are classified as in k_rdf_fx, with the
+// The particles are sorted into nc^3 cells of side s >= sqrt(X) (reach 1)
+// or s >= sqrt(X) / 2 (reach 2), X the classification's far bound
+// (lj_rdf_device.h): every pair whose fixed-point r2 is below X lies in the
+// same or adjacent cells (within two cells on each axis at reach 2), and
+// every other pair is, by the error bound, beyond the counted range in the
+// reference's arithmetic too -- dropped (CPU rule, r-bins) or in the clamped
+// last bin (GPU rule).  Reach 1 (k_rdf_cells): a work item is (cell c,
+// forward offset o), the cell's own pairs once (o = 0) or its pairs with one
+// of the 13 forward neighbours (half a shell), so every unordered pair of
+// adjacent cells is met once (nc >= 3 makes the 27 neighbours distinct).
+// Reach 2 (k_rdf_cells2): one workgroup per cell against itself and its 62
+// forward neighbours (nc >= 5).  Its pairs are classified as in k_rdf_fx, with the
 // reference arithmetic for the pairs near a bin edge.  The GPU rule's clamp bin then holds the N (N - 1) / 2 pairs
 // less those counted elsewhere (or excluded at r^2 <= 1e-6).
 //
