@@ -17,6 +17,64 @@ namespace {
 
 using namespace dev;
 
+// 1 / x in double: the hardware estimate (v_rcp_f64) refined by two Newton
+// steps, within an ulp or two of the quotient (the IEEE division's scaling
+// and fix-up sequence cost several times as much in the pair loop)
+__device__ __forceinline__ double rcp64(double x)
+{
+  double y = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, y, 1.);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.);
+  return __builtin_fma(y, e, y);
+}
+
+// one pair (i, j) into the lane's sums; GUARD: j may be i (the self pair,
+// excluded)
+template <bool PERIODIC, bool GUARD>
+__device__ __forceinline__ void pair64(const double4& xi, const double4& xj, int i, int j, double L,
+                                       double invL, double s2, double& fx, double& fy, double& fz,
+                                       double& a, double& b)
+{
+  double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+  if constexpr (PERIODIC) {
+    dx -= L * __builtin_rint(dx * invL);
+    dy -= L * __builtin_rint(dy * invL);
+    dz -= L * __builtin_rint(dz * invL);
+  }
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double ir2 = s2 * rcp64(GUARD && j == i ? 1. : r2);
+  if constexpr (GUARD) ir2 = j == i ? 0. : ir2;
+  const double ir6 = ir2 * ir2 * ir2;
+  const double w = ir6 * (2. * ir6 - 1.);
+  const double sc = w * ir2;
+  fx += sc * dx;
+  fy += sc * dy;
+  fz += sc * dz;
+  a += ir6;
+  b += w;
+}
+
+// pairs (i, j), j in [jb, je), j streamed through SGPRs (uniform scalar
+// loads), four j loaded ahead of their pairs: a loop of one j waited on each
+// scalar load in turn
+template <bool PERIODIC, bool GUARD>
+__device__ __forceinline__ void force64_range(const double4* __restrict__ x, const double4& xi,
+                                              int i, int jb, int je, double L, double invL,
+                                              double s2, double& fx, double& fy, double& fz,
+                                              double& a, double& b)
+{
+  int j = __builtin_amdgcn_readfirstlane(jb);
+  for (; j + 4 <= je; j += 4) {
+    const double4 x0 = x[j], x1 = x[j + 1], x2 = x[j + 2], x3 = x[j + 3];
+    pair64<PERIODIC, GUARD>(xi, x0, i, j, L, invL, s2, fx, fy, fz, a, b);
+    pair64<PERIODIC, GUARD>(xi, x1, i, j + 1, L, invL, s2, fx, fy, fz, a, b);
+    pair64<PERIODIC, GUARD>(xi, x2, i, j + 2, L, invL, s2, fx, fy, fz, a, b);
+    pair64<PERIODIC, GUARD>(xi, x3, i, j + 3, L, invL, s2, fx, fy, fz, a, b);
+  }
+  for (; j < je; ++j) pair64<PERIODIC, GUARD>(xi, x[j], i, j, L, invL, s2, fx, fy, fz, a, b);
+}
+
 template <bool PERIODIC>
 __global__ __launch_bounds__(kBlock) void k_force64(const double4* __restrict__ x, int N, int Npad,
                                                     int jchunk, double L, double invL, double s2,
@@ -33,32 +91,14 @@ __global__ __launch_bounds__(kBlock) void k_force64(const double4* __restrict__ 
   const int i = i0 + lane;
   const double4 xi = x[min(i, N - 1)];
   double fx = 0., fy = 0., fz = 0., a = 0., b = 0.;
-#pragma unroll 2
-  for (int j = j0; j < j1; ++j) {
-    const int ju = __builtin_amdgcn_readfirstlane(j);
-    const double4 xj = x[ju];
-    double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    if constexpr (PERIODIC) {
-      dx -= L * __builtin_rint(dx * invL);
-      dy -= L * __builtin_rint(dy * invL);
-      dz -= L * __builtin_rint(dz * invL);
-    }
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const double ir2 = s2 / (ju == i ? 1. : r2);
-    const double ir6 = ir2 * ir2 * ir2;
-    double w = ir6 * (2. * ir6 - 1.);
-    double sc = w * ir2;
-    double e6 = ir6;
-    if (ju == i) {
-      w = 0.;
-      sc = 0.;
-      e6 = 0.;
-    }
-    fx += sc * dx;
-    fy += sc * dy;
-    fz += sc * dz;
-    a += e6;
-    b += w;
+  // the self pair can only lie in the wave's own 64 j: the guard runs there
+  const int d0 = max(j0, i0), d1 = min(j1, i0 + 64);
+  if (d0 >= d1) {
+    force64_range<PERIODIC, false>(x, xi, i, j0, j1, L, invL, s2, fx, fy, fz, a, b);
+  } else {
+    force64_range<PERIODIC, false>(x, xi, i, j0, d0, L, invL, s2, fx, fy, fz, a, b);
+    force64_range<PERIODIC, true>(x, xi, i, d0, d1, L, invL, s2, fx, fy, fz, a, b);
+    force64_range<PERIODIC, false>(x, xi, i, d1, j1, L, invL, s2, fx, fy, fz, a, b);
   }
   if (i < N) {
     const size_t o = (size_t)s * Npad + i;
