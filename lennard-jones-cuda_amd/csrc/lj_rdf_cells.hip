@@ -1,21 +1,24 @@
 // lj_rdf_cells.hip -- the RDF pass over a cell list, for periodic boxes whose
 // counted range is short against the box (the reference rules' r^2 < 25.6 at
-// liquid densities: N = 16384, rho* = 0.85 has 5 cells per axis).
+// liquid densities: N = 16384, rho* = 0.85 has 10 cells per axis at reach 2).
 //
 // The particles are sorted into nc^3 cells of side s >= sqrt(X) (reach 1)
 // or s >= sqrt(X) / 2 (reach 2), X the classification's far bound
-// (lj_rdf_device.h): every pair whose fixed-point r2 is below X lies in the
-// same or adjacent cells (within two cells on each axis at reach 2), and
-// every other pair is, by the error bound, beyond the counted range in the
-// reference's arithmetic too -- dropped (CPU rule, r-bins) or in the clamped
-// last bin (GPU rule).  Reach 1 (k_rdf_cells): a work item is (cell c,
-// forward offset o), the cell's own pairs once (o = 0) or its pairs with one
-// of the 13 forward neighbours (half a shell), so every unordered pair of
-// adjacent cells is met once (nc >= 3 makes the 27 neighbours distinct).
-// Reach 2 (k_rdf_cells2): one workgroup per cell against itself and its 62
-// forward neighbours (nc >= 5).  Its pairs are classified as in k_rdf_fx, with the
-// reference arithmetic for the pairs near a bin edge.  The GPU rule's clamp bin then holds the N (N - 1) / 2 pairs
-// less those counted elsewhere (or excluded at r^2 <= 1e-6).
+// (lj_rdf_device.h).  Every pair whose fixed-point r2 is below X lies in the
+// same or adjacent cells (within two cells on each axis at reach 2).  Every
+// other pair is, by the error bound, beyond the counted range in the
+// reference's arithmetic too: dropped (CPU rule, r-bins) or in the clamped
+// last bin (GPU rule).
+//   Reach 1 (k_rdf_cells): a work item is (cell c, forward offset o), the
+//   cell's own pairs once (o = 0) or its pairs with one of the 13 forward
+//   neighbours (half a shell), so every unordered pair of adjacent cells is
+//   met once (nc >= 3 makes the 27 neighbours distinct).
+//   Reach 2 (k_rdf_cells2): one workgroup per cell against itself and its 62
+//   forward neighbours (nc >= 5).
+// The pairs are classified as in k_rdf_fx, with the reference arithmetic for
+// the pairs near a bin edge.  The GPU rule's clamp bin then holds the
+// N (N - 1) / 2 pairs less those counted elsewhere (or excluded at
+// r^2 <= 1e-6).
 //
 // The bound holds for coordinate spans up to A = 3 L (all coordinates in
 // [-L, 2 L), as the periodic wrap keeps them); the assignment kernel flags
@@ -23,14 +26,14 @@
 // for the host (lj_rdf falls back to the all-pairs pass).
 //
 // Kernels: the sort -- up to 4096 cells, k_cells_assign_lds (cell and rank
-// in block-private LDS counts) and k_cells_place (the cells' starts, the
-// sorted fixed-point positions with their index); else assign (global
-// counters), scan (one block) and scatter -- then the pairs and the
-// reduction.  Counts are integers added by atomics (order-independent), so the
-// histogram is exactly the reference's whatever the order within a cell.
-// Workgroup histograms go to one of kRdfReps replicas, which the reduction
-// sums, doubles into ordered-pair counts, adds to hist and clears for the
-// next pass (no memset node in a captured step).
+// in block-private LDS counts, the block's offset in each cell) and
+// k_cells_place (the cells' starts, the sorted fixed-point positions with
+// their index); else assign (global counters), scan (one block) and scatter
+// -- then the pairs and the reduction.  Counts are integers added by atomics
+// (order-independent), so the histogram is exactly the reference's whatever
+// the order within a cell.  Workgroup histograms go to one of kRdfReps
+// replicas, which the reduction sums, doubles into ordered-pair counts, adds
+// to hist and clears for the next pass (no memset node in a captured step).
 #include <cmath>
 
 #include <cstdlib>
