@@ -29,6 +29,10 @@ RULES = {RDF_REF_CPU: ljamd.RDF_REF_CPU, RDF_REF_GPU: ljamd.RDF_REF_GPU,
     (3000, 0.5, ljamd.BC_PERIODIC, RDF_REF_GPU, 256),     # L = 18.2: 3 / 7
     (6000, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 256),    # L = 19.2: 3 / 7
     (9000, 0.3, ljamd.BC_PERIODIC, RDF_REF_GPU, 128),     # L = 31.1, range r^2 < 12.8: 8 / 17
+    # 512 bins of dr2 = 0.1 (range r^2 < 51.2): 3 / 7 cells per axis, 48
+    # particles per reach-2 cell, about 3000 staged j per cell (two stages)
+    (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 512),
+    (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_GPU, 512),
 ])
 def test_rdf_bitexact(N, rho, bc, rule, nbins):
     L, pos, _ = perturbed_state(N, rho, 1.0, seed=31)
