@@ -234,3 +234,30 @@ def test_step_rdf_out_of_span_reported_after_lj_rdf():
         s.Integrate(0.005, 2)
         counts, n = s.step_rdf_counts()
         assert n == 2 and counts.sum() == 2 * N * (N - 1)
+
+
+def test_rdf_cells_crowded_cells_equal_all_pairs_pass(tmp_path):
+    """Cells holding more particles than a workgroup has lanes: N = 70000,
+    1024 bins of dr2 = 0.18 (range r^2 < 184), 3 cells per axis at reach 1
+    (2600 particles each) and 6 at reach 2 (324 each: two i rounds, 14 j
+    stages).  Both rules, both layouts: the all-pairs pass's counts."""
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import ljamd
+from helpers import perturbed_state
+N = 70000
+L, pos, vel = perturbed_state(N, 0.85, 1.0, seed=6)
+out = []
+with ljamd.LJSystem(N, 0.85, 1.0) as s:
+    s.set_state(pos, vel)
+    for rule in (ljamd.RDF_REF_GPU, ljamd.RDF_REF_CPU):
+        out.append(s.rdf(rule, 1024, dr2=0.18).astype(np.int64))
+np.savez(sys.argv[2], *out)
+"""
+    on1, on2, off = _cells_on_and_off(code, tmp_path)
+    for a, b, c in zip(on1, on2, off):
+        assert np.array_equal(a, c)
+        assert np.array_equal(b, c)
+    N = 70000
+    assert on2[0].sum() == N * (N - 1) and 0 < on2[1].sum() < N * (N - 1)
