@@ -228,10 +228,19 @@ def rdf_leg(reps=20):
         us_r = s.time_rdf(ljamd.RDF_RBINS, 500, reps=reps)
         us_g = s.time_rdf(ljamd.RDF_REF_GPU, 256, reps=reps)
         us_c = s.time_rdf(ljamd.RDF_REF_CPU, 256, reps=reps)
+    # the same with hard walls (the workload's second run)
+    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, boundary=ljamd.BC_WALL, seed=SEED) as s:
+        s.Integrate(DT, 20)
+        s.synchronize()
+        wr = s.time_rdf(ljamd.RDF_RBINS, 500, reps=reps)
+        wg = s.time_rdf(ljamd.RDF_REF_GPU, 256, reps=reps)
+        wc = s.time_rdf(ljamd.RDF_REF_CPU, 256, reps=reps)
     pairs = float(N) * N
     return {"rbins_500_to_half_box_us": us_r, "rbins_pairs_per_s": pairs / (us_r * 1e-6),
             "ref_gpu_256_r2_bins_us": us_g, "ref_gpu_pairs_per_s": pairs / (us_g * 1e-6),
-            "ref_cpu_256_r2_bins_us": us_c, "ref_cpu_pairs_per_s": pairs / (us_c * 1e-6)}
+            "ref_cpu_256_r2_bins_us": us_c, "ref_cpu_pairs_per_s": pairs / (us_c * 1e-6),
+            "hard_wall": {"rbins_500_to_half_box_us": wr, "ref_gpu_256_r2_bins_us": wg,
+                          "ref_cpu_256_r2_bins_us": wc}}
 
 
 def precision_leg(steps=100):

@@ -91,13 +91,14 @@ double4* state64(lj_ctx* c, int type);
 // plan of the force pass that fills fpart/apart
 inline const ForcePlan& fplan(const lj_ctx* c) { return c->plan3.n3 ? c->plan3 : c->plan; }
 // gives an RDF pass of the context the cell-list workspace where that pass
-// applies (periodic, at most 1024 bins, 3 or more cells per axis), grown first
+// applies (at most 1024 bins; periodic boxes with 3 or more cells per axis,
+// hard-wall boxes with 6 or more), grown first
 inline int attach_rdf_cells(lj_ctx* c, RdfArgs& a)
 {
   a.cells = nullptr;
-  if (!a.periodic || a.nbins > 1024) return LJ_OK;
+  if (a.nbins > 1024) return LJ_OK;
   int reach = 1;
-  const int nc = rdf_cells_layout(a.L, a.end, &reach);
+  const int nc = rdf_cells_grid(a, &reach);
   if (nc <= 0) return LJ_OK;
   const int rc = rdf_cells_ensure(c->rdf_cells, a.N, nc * nc * nc, a.nbins);
   if (rc != LJ_OK) return rc;

@@ -181,7 +181,8 @@ struct RdfCellsWs {
 };
 // cells per axis for a periodic range ending at r^2 = end, or 0 (fewer than 3)
 int rdf_cells_per_axis(double L, double end);
-// cells per axis of the layout the pass takes (0: none), and its reach (1 or 2)
+// cells per axis of the periodic layout the pass takes (0: none), and its
+// reach (1 or 2)
 int rdf_cells_layout(double L, double end, int* reach);
 int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins);  // LJ_OK or a status
 void rdf_cells_free(RdfCellsWs& w);
@@ -203,6 +204,9 @@ struct RdfArgs {
 };
 hipError_t launch_rdf(hipStream_t st, const RdfArgs& a);
 hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a);
+// cells per axis (0: none) and reach of the cell-list pass for these
+// arguments, periodic or hard-wall
+int rdf_cells_grid(const RdfArgs& a, int* reach);
 // whether launch_rdf takes the cell-list pass for these arguments
 bool rdf_uses_cells(const RdfArgs& a);
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);

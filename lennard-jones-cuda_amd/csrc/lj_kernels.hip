@@ -1028,7 +1028,7 @@ hipError_t launch_rdf(hipStream_t st, const RdfArgs& a)
   const bool small = a.nbins <= kRdfWaveCopy;
   const dim3 grid(ng, ng), blk(kBlock);
   const size_t tab = sizeof(float) * (a.nbins + 1);
-  if (small && a.periodic && rdf_uses_cells(a)) return launch_rdf_cells(st, a);
+  if (small && rdf_uses_cells(a)) return launch_rdf_cells(st, a);
   if (small && a.periodic) {
     const size_t lds = (a.rule != LJ_RDF_RBINS ? sizeof(float2) * a.nbins : 0) + tab +
                        sizeof(unsigned) * kWaves * a.nbins;

@@ -60,8 +60,37 @@ __device__ __forceinline__ int cell_of(const uint4& q, int nc)
   return cx + nc * (cy + nc * cz);
 }
 
+// A particle's cell and its sorted record.  Periodic boxes: the cell of the
+// fixed-point coordinates, the record (q.x, q.y, q.z, index), and `bad` when
+// a coordinate lies outside [-L, 2 L) (the bound's span).  Hard walls
+// (WALL): floor(x nc / L) per axis clamped to the grid -- particles a little
+// outside [0, L] go to the edge cells, and the map stays monotone, so a pair
+// closer than a cell's side is at most one cell apart per axis -- and the
+// record holds the float coordinates' bits.
+template <bool WALL>
+__device__ __forceinline__ int cell_record(const float4& p, int i, double L, double qs, float inv_side,
+                                           int nc, uint4& rec, bool& bad)
+{
+  if constexpr (WALL) {
+    rec = make_uint4(__float_as_uint(p.x), __float_as_uint(p.y), __float_as_uint(p.z), (unsigned)i);
+    bad = false;
+    const int cx = min(max((int)floorf(p.x * inv_side), 0), nc - 1);
+    const int cy = min(max((int)floorf(p.y * inv_side), 0), nc - 1);
+    const int cz = min(max((int)floorf(p.z * inv_side), 0), nc - 1);
+    return cx + nc * (cy + nc * cz);
+  } else {
+    rec = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), (unsigned)i);
+    const double lo = -L, hi = 2. * L;
+    bad = !((double)p.x >= lo && (double)p.x < hi && (double)p.y >= lo && (double)p.y < hi &&
+            (double)p.z >= lo && (double)p.z < hi);
+    return cell_of(rec, nc);
+  }
+}
+
+template <bool WALL>
 __global__ __launch_bounds__(kCellBlock) void k_cells_assign(const float4* __restrict__ pos, int N,
-                                                             double L, double qs, int nc,
+                                                             double L, double qs, float inv_side,
+                                                             int nc,
                                                              int* __restrict__ count,
                                                              int* __restrict__ cellof,
                                                              int* __restrict__ rank,
@@ -69,15 +98,12 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_assign(const float4* __res
 {
   const int i = blockIdx.x * kCellBlock + threadIdx.x;
   if (i >= N) return;
-  const float4 p = pos[i];
-  const uint4 q = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), 0u);
-  const int c = cell_of(q, nc);
+  uint4 rec;
+  bool bad;
+  const int c = cell_record<WALL>(pos[i], i, L, qs, inv_side, nc, rec, bad);
   cellof[i] = c;
   rank[i] = atomicAdd(&count[c], 1);
-  const double lo = -L, hi = 2. * L;
-  if (!((double)p.x >= lo && (double)p.x < hi && (double)p.y >= lo && (double)p.y < hi &&
-        (double)p.z >= lo && (double)p.z < hi))
-    atomicOr(flag, 1u);
+  if (bad) atomicOr(flag, 1u);
 }
 
 // The same with the counts in LDS (ncell <= kCellLds): each block ranks its
@@ -88,8 +114,10 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_assign(const float4* __res
 // (28 us at N = 16384 over 125 cells, against 3 us).
 constexpr int kCellLds = 4096;
 constexpr int kAssignPer = 4;  // particles per thread of the LDS-counted assignment
+template <bool WALL>
 __global__ __launch_bounds__(kCellBlock) void k_cells_assign_lds(const float4* __restrict__ pos, int N,
-                                                                 double L, double qs, int nc,
+                                                                 double L, double qs,
+                                                                 float inv_side, int nc,
                                                                  int ncell,
                                                                  int* __restrict__ count,
                                                                  int* __restrict__ bcnt,
@@ -103,15 +131,12 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_assign_lds(const float4* _
   for (int k = 0; k < kAssignPer; ++k) {
     const int i = blockIdx.x * (kCellBlock * kAssignPer) + k * kCellBlock + (int)threadIdx.x;
     if (i >= N) break;
-    const float4 p = pos[i];
-    const uint4 q = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), 0u);
-    const int c = cell_of(q, nc);
+    uint4 rec;
+    bool bad;
+    const int c = cell_record<WALL>(pos[i], i, L, qs, inv_side, nc, rec, bad);
     cellof[i] = c;
     rank[i] = atomicAdd(&h[c], 1);
-    const double lo = -L, hi = 2. * L;
-    if (!((double)p.x >= lo && (double)p.x < hi && (double)p.y >= lo && (double)p.y < hi &&
-          (double)p.z >= lo && (double)p.z < hi))
-      atomicOr(flag, 1u);
+    if (bad) atomicOr(flag, 1u);
   }
   __syncthreads();
   // the block's place within each cell it holds particles of (the blocks'
@@ -125,6 +150,7 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_assign_lds(const float4* _
 // scatters its own particles to start + the block's offset + rank.  The scan
 // is repeated by every block (ncell reads each, from L2) instead of a
 // separate one-block launch; the totals are cleared by k_rdf_cells_reduce.
+template <bool WALL>
 __global__ __launch_bounds__(kCellBlock) void k_cells_place(const float4* __restrict__ pos, int N,
                                                             double qs, int ncell,
                                                             const int* __restrict__ count,
@@ -170,7 +196,8 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_place(const float4* __rest
     const float4 p = pos[i];
     const int c = cellof[i];
     sorted[cs[c] + bcnt[(size_t)blockIdx.x * ncell + c] + rank[i]] =
-        make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), (unsigned)i);
+        WALL ? make_uint4(__float_as_uint(p.x), __float_as_uint(p.y), __float_as_uint(p.z), (unsigned)i)
+             : make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), (unsigned)i);
   }
 }
 
@@ -220,6 +247,7 @@ __global__ __launch_bounds__(1024) void k_cells_scan(int* __restrict__ count, in
 
 // the particles to their cells' starts + their ranks (k_cells_assign's
 // global counters)
+template <bool WALL>
 __global__ __launch_bounds__(kCellBlock) void k_cells_scatter(const float4* __restrict__ pos, int N,
                                                               double qs,
                                                               const int* __restrict__ cellof,
@@ -230,7 +258,9 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_scatter(const float4* __re
   const int i = blockIdx.x * kCellBlock + threadIdx.x;
   if (i >= N) return;
   const float4 p = pos[i];
-  sorted[start[cellof[i]] + rank[i]] = make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), (unsigned)i);
+  sorted[start[cellof[i]] + rank[i]] =
+      WALL ? make_uint4(__float_as_uint(p.x), __float_as_uint(p.y), __float_as_uint(p.z), (unsigned)i)
+           : make_uint4(fixq(p.x, qs), fixq(p.y, qs), fixq(p.z, qs), (unsigned)i);
 }
 
 // the 13 forward neighbours of a cell (dz > 0, or dz = 0 and dy > 0, or
@@ -425,7 +455,23 @@ __device__ __forceinline__ void shell2_offset(int o, int& dx, int& dy, int& dz)
   dz = k / 25 - 2;
 }
 
+// the reference's r^2 of a hard-wall pair (no image), as rdf_exact_bin
+// computes it: float differences, then fmaf (GPU rule) or a plain float sum
 template <int RULE>
+__device__ __forceinline__ float wall_r2(const float4& pi, const uint4& rj)
+{
+#pragma clang fp contract(off)
+  const float rx = pi.x - __uint_as_float(rj.x), ry = pi.y - __uint_as_float(rj.y),
+              rz = pi.z - __uint_as_float(rj.z);
+  if constexpr (RULE == LJ_RDF_REF_GPU) return __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+  else return rx * rx + ry * ry + rz * rz;
+}
+
+// WALL: a hard-wall box.  The records hold float coordinates, neighbours
+// beyond the grid are left out (no wrap), and every candidate pair takes the
+// reference arithmetic directly (r^2 and the bin from the threshold table):
+// there is no image to approximate.
+template <int RULE, bool WALL>
 __global__ __launch_bounds__(kCellBlock) void k_rdf_cells2(
     const uint4* __restrict__ sorted, const int* __restrict__ start, const float4* __restrict__ pos,
     int nc, double L, int nbins, const float* __restrict__ tab, float est_scale, float t0, float t1,
@@ -467,9 +513,17 @@ __global__ __launch_bounds__(kCellBlock) void k_rdf_cells2(
     if (lane < kNb2) {
       int dx, dy, dz;
       shell2_offset(lane, dx, dy, dz);
-      const int c2 = (cx + dx + nc) % nc + nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc));
+      int c2;
+      bool in = true;
+      if constexpr (WALL) {
+        const int x = cx + dx, y = cy + dy, z = cz + dz;
+        in = x >= 0 && x < nc && y >= 0 && y < nc && z >= 0 && z < nc;
+        c2 = in ? x + nc * (y + nc * z) : c;
+      } else {
+        c2 = (cx + dx + nc) % nc + nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc));
+      }
       s_cell[lane] = c2;
-      cnt = start[c2 + 1] - start[c2];
+      cnt = in ? start[c2 + 1] - start[c2] : 0;  // a neighbour beyond a wall holds nothing
     }
     int x = cnt;
 #pragma unroll
@@ -499,6 +553,8 @@ __global__ __launch_bounds__(kCellBlock) void k_rdf_cells2(
     const bool act = ph < P;
     const int iloc = ib + li;  // the lane's particle, by its place in the cell
     const uint4 qi = sorted[i0 + iloc];
+    const float4 pi = make_float4(__uint_as_float(qi.x), __uint_as_float(qi.y),
+                                  __uint_as_float(qi.z), 0.f);  // WALL
     for (int jb = 0; jb < total; jb += kJStage2) {
       const int jn = min(kJStage2, total - jb);
       __syncthreads();  // the previous stage is no longer read
@@ -513,7 +569,36 @@ __global__ __launch_bounds__(kCellBlock) void k_rdf_cells2(
         sj[e] = sorted[start[s_cell[lo]] + (g - s_off[lo])];
       }
       __syncthreads();
-      if (act) {
+      if (WALL && act) {
+        // the reference arithmetic for every candidate, 8 at a time
+        for (int tb = ph; tb < jn; tb += 8 * P) {
+          constexpr int B = 8;
+          float r2[B];
+          int kk[B];
+          bool ok[B];
+#pragma unroll
+          for (int b = 0; b < B; ++b) {
+            const int t = tb + b * P, g = jb + t;
+            r2[b] = wall_r2<RULE>(pi, sj[min(t, jn - 1)]);
+            ok[b] = (t < jn) & ((g >= nself) | (g > iloc));
+            const float e = RULE == LJ_RDF_RBINS ? __builtin_amdgcn_sqrtf(r2[b]) * est_scale
+                                                 : r2[b] * est_scale;
+            kk[b] = r2[b] >= Tlast ? nbins - 1 : rdf_bin(r2[b], T, nbins, (int)e);
+          }
+#pragma unroll
+          for (int b = 0; b < B; ++b) {
+            bool count;
+            if constexpr (GPU) {  // counted below the clamp bin; r2 <= 1e-6 excluded
+              const bool in = r2[b] > 1e-6f;
+              nexcl += (ok[b] & !in) ? 1u : 0u;
+              count = ok[b] & in & (r2[b] < Tlast);
+            } else {  // dropped beyond the last bin
+              count = ok[b] & (r2[b] < Tmax);
+            }
+            if (count) atomicAdd(&hw[kk[b]], 1u);
+          }
+        }
+      } else if (act) {
         // j = t0 + b P, 8 at a time; in the cell itself only j after the
         // lane's particle (each unordered pair once)
         for (int tb = ph; tb < jn; tb += 8 * P) {
@@ -723,54 +808,86 @@ void rdf_cells_free(RdfCellsWs& w)
 // the cell-list pass serves periodic boxes with at most 1024 bins and 3 or
 // more cells per axis, given a workspace large enough (LJ_RDF_CELLS=0 turns
 // it off)
+// hard walls: reach-2 cells of side at least half the counted range's end
+// (with a relative and an absolute margin over the float rounding of r^2
+// and of the cell coordinates), 6 to 64 per axis, or 0 (fewer than 6: the
+// neighbourhoods would cover most of the box)
+int rdf_wall_cells_per_axis(double L, double end)
+{
+  if (!(L > 0.) || !(end > 0.) || !std::isfinite(end)) return 0;
+  const double reach = std::sqrt((double)(float)end) * (1. + 1e-4) + 1e-4;
+  const int nc = (int)std::min((double)kRdfMaxCells, std::floor(2. * L / reach));
+  return nc >= 6 ? nc : 0;
+}
+
+int rdf_cells_grid(const RdfArgs& a, int* reach)
+{
+  *reach = 2;
+  return a.periodic ? rdf_cells_layout(a.L, a.end, reach) : rdf_wall_cells_per_axis(a.L, a.end);
+}
+
 bool rdf_uses_cells(const RdfArgs& a)
 {
   static const char* e = std::getenv("LJ_RDF_CELLS");
   if (e && e[0] == '0') return false;
-  if (!a.cells || !a.periodic || a.nbins > 1024) return false;
+  if (!a.cells || a.nbins > 1024) return false;
   int reach = 1;
-  const int nc = rdf_cells_layout(a.L, a.end, &reach);
+  const int nc = rdf_cells_grid(a, &reach);
   return nc > 0 && a.cells->flag && a.N <= a.cells->cap_n && nc * nc * nc <= a.cells->cap_cells &&
          a.nbins <= a.cells->cap_bins;
+}
+
+template <bool WALL>
+void launch_cells_sort(hipStream_t st, const RdfArgs& a, const RdfCellsWs& w, int nc, double qs,
+                       float inv_side)
+{
+  const int ncell = nc * nc * nc;
+  const int nb = (a.N + kCellBlock - 1) / kCellBlock;
+  const int na = (a.N + kCellBlock * kAssignPer - 1) / (kCellBlock * kAssignPer);
+  if (ncell <= kCellLds && (long)ncell * na <= (long)w.cap_m) {
+    // per-block counts in LDS; the cells' starts and the blocks' offsets
+    // found and the particles placed by the same blocks in a second launch
+    hipLaunchKernelGGL(k_cells_assign_lds<WALL>, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, a.L,
+                       qs, inv_side, nc, ncell, w.count, w.bcnt, w.cellof, w.rank, w.flag);
+    hipLaunchKernelGGL(k_cells_place<WALL>, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, qs, ncell,
+                       w.count, w.bcnt, w.cellof, w.rank, w.start, w.sorted);
+  } else {
+    hipLaunchKernelGGL(k_cells_assign<WALL>, dim3(nb), dim3(kCellBlock), 0, st, a.pos, a.N, a.L, qs,
+                       inv_side, nc, w.count, w.cellof, w.rank, w.flag);
+    hipLaunchKernelGGL(k_cells_scan, dim3(1), dim3(1024), 0, st, w.count, w.start, ncell);
+    hipLaunchKernelGGL(k_cells_scatter<WALL>, dim3(nb), dim3(kCellBlock), 0, st, a.pos, a.N, qs,
+                       w.cellof, w.rank, w.start, w.sorted);
+  }
 }
 
 hipError_t launch_rdf_cells(hipStream_t st, const RdfArgs& a)
 {
   const RdfCellsWs& w = *a.cells;
   int reach = 1;
-  const int nc = rdf_cells_layout(a.L, a.end, &reach);
+  const int nc = rdf_cells_grid(a, &reach);
   const int ncell = nc * nc * nc;
   const double qs = 4294967296. / a.L;
-  const int nb = (a.N + kCellBlock - 1) / kCellBlock;
-  const int na = (a.N + kCellBlock * kAssignPer - 1) / (kCellBlock * kAssignPer);
-  if (ncell <= kCellLds && (long)ncell * na <= (long)w.cap_m) {
-    // per-block counts in LDS; the cells' starts and the blocks' offsets
-    // found and the particles placed by the same blocks in a second launch
-    hipLaunchKernelGGL(k_cells_assign_lds, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, a.L, qs, nc,
-                       ncell, w.count, w.bcnt, w.cellof, w.rank, w.flag);
-    hipLaunchKernelGGL(k_cells_place, dim3(na), dim3(kCellBlock), 0, st, a.pos, a.N, qs, ncell,
-                       w.count, w.bcnt, w.cellof, w.rank, w.start, w.sorted);
-  } else {
-    hipLaunchKernelGGL(k_cells_assign, dim3(nb), dim3(kCellBlock), 0, st, a.pos, a.N, a.L, qs, nc,
-                       w.count, w.cellof, w.rank, w.flag);
-    hipLaunchKernelGGL(k_cells_scan, dim3(1), dim3(1024), 0, st, w.count, w.start, ncell);
-    hipLaunchKernelGGL(k_cells_scatter, dim3(nb), dim3(kCellBlock), 0, st, a.pos, a.N, qs, w.cellof,
-                       w.rank, w.start, w.sorted);
-  }
+  const bool wall = !a.periodic;
+  if (wall) launch_cells_sort<true>(st, a, w, nc, qs, (float)(nc / a.L));
+  else launch_cells_sort<false>(st, a, w, nc, qs, 0.f);
   const size_t lds = (a.rule != LJ_RDF_RBINS ? sizeof(float2) * a.nbins : 0) +
                      sizeof(float) * (a.nbins + 1) + sizeof(unsigned) * kWaves * a.nbins;
 #define LJ_CELLS_ARGS w.sorted, w.start, a.pos, nc, a.L, a.nbins, a.tab, a.est_scale, a.thr[0], \
                       a.thr[1], a.thr[2], a.thr[3], qs, w.flag, w.reps, w.other
-#define LJ_CELLS_LAUNCH(K, G)                                                                    \
+#define LJ_CELLS_LAUNCH(K, G, ...)                                                               \
   do {                                                                                           \
     if (a.rule == LJ_RDF_REF_CPU)                                                                \
-      hipLaunchKernelGGL(K<LJ_RDF_REF_CPU>, G, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);        \
+      hipLaunchKernelGGL((K<LJ_RDF_REF_CPU __VA_ARGS__>), G, dim3(kCellBlock), lds, st,          \
+                         LJ_CELLS_ARGS);                                                         \
     else if (a.rule == LJ_RDF_REF_GPU)                                                           \
-      hipLaunchKernelGGL(K<LJ_RDF_REF_GPU>, G, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);        \
+      hipLaunchKernelGGL((K<LJ_RDF_REF_GPU __VA_ARGS__>), G, dim3(kCellBlock), lds, st,          \
+                         LJ_CELLS_ARGS);                                                         \
     else                                                                                         \
-      hipLaunchKernelGGL(K<LJ_RDF_RBINS>, G, dim3(kCellBlock), lds, st, LJ_CELLS_ARGS);          \
+      hipLaunchKernelGGL((K<LJ_RDF_RBINS __VA_ARGS__>), G, dim3(kCellBlock), lds, st,            \
+                         LJ_CELLS_ARGS);                                                         \
   } while (0)
-  if (reach == 2) LJ_CELLS_LAUNCH(k_rdf_cells2, dim3(ncell));
+  if (wall) LJ_CELLS_LAUNCH(k_rdf_cells2, dim3(ncell), , true);
+  else if (reach == 2) LJ_CELLS_LAUNCH(k_rdf_cells2, dim3(ncell), , false);
   else LJ_CELLS_LAUNCH(k_rdf_cells, dim3(ncell, 14));
 #undef LJ_CELLS_LAUNCH
 #undef LJ_CELLS_ARGS

@@ -33,6 +33,11 @@ RULES = {RDF_REF_CPU: ljamd.RDF_REF_CPU, RDF_REF_GPU: ljamd.RDF_REF_GPU,
     # particles per reach-2 cell, about 3000 staged j per cell (two stages)
     (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_CPU, 512),
     (16384, 0.85, ljamd.BC_PERIODIC, RDF_REF_GPU, 512),
+    # hard walls: the cell list without wrap (10 cells per axis at N = 16384;
+    # the perturbed lattice puts some particles just outside [0, L])
+    (16384, 0.85, ljamd.BC_WALL, RDF_REF_CPU, 256),
+    (16384, 0.85, ljamd.BC_WALL, RDF_REF_GPU, 256),
+    (9000, 0.3, ljamd.BC_WALL, RDF_REF_GPU, 128),
 ])
 def test_rdf_bitexact(N, rho, bc, rule, nbins):
     L, pos, _ = perturbed_state(N, rho, 1.0, seed=31)
@@ -261,3 +266,46 @@ np.savez(sys.argv[2], *out)
         assert np.array_equal(b, c)
     N = 70000
     assert on2[0].sum() == N * (N - 1) and 0 < on2[1].sum() < N * (N - 1)
+
+
+def test_rdf_wall_cells_equal_all_pairs_pass(tmp_path):
+    """Hard walls at N = 16384: the captured per-step sums of 30 TVN steps
+    and the trajectory, then both reference rules on that state with the
+    particles near each wall moved 0.4 outside [0, L] (the clamped edge
+    cells), counted over the wall cell list, equal the all-pairs pass
+    (LJ_RDF_CELLS=0), bit for bit."""
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import ljamd
+from helpers import perturbed_state
+N = 16384
+L, pos, vel = perturbed_state(N, 0.85, 1.0, seed=12)
+out = []
+with ljamd.LJSystem(N, 0.85, 1.0, canonical=ljamd.TVN, boundary=ljamd.BC_WALL) as s:
+    s.set_state(pos, vel)
+    s.CalculateForces()
+    s.set_step_sampling(rdf=ljamd.SAMPLE_RDF_SUM, rdf_rule=ljamd.RDF_REF_GPU, rdf_nbins=256)
+    s.Integrate(0.005, 30)
+    got, n = s.step_rdf_counts()
+    out.append(got.astype(np.int64))
+    out.append(s.pos.astype(np.float64))
+    p = s.pos.reshape(-1, 4).copy()
+    xyz = p[:, :3]
+    xyz[xyz < 1.0] -= np.float32(0.4)
+    xyz[xyz > L - 1.0] += np.float32(0.4)
+    p[:, :3] = xyz
+    out.append(np.array([np.sum((xyz < 0) | (xyz > L))]))
+    s.set_state(p.reshape(-1), s.vel)
+    for rule in (ljamd.RDF_REF_GPU, ljamd.RDF_REF_CPU):
+        out.append(s.rdf(rule, 256, dr2=0.1).astype(np.int64))
+np.savez(sys.argv[2], *out)
+"""
+    on1, on2, off = _cells_on_and_off(code, tmp_path)
+    for a, b, c in zip(on1, on2, off):
+        assert np.array_equal(a, c)
+        assert np.array_equal(b, c)
+    N = 16384
+    assert on1[0].sum() == 30 * N * (N - 1)
+    assert on1[2][0] > 1000  # coordinates outside [0, L]: the clamped edge cells
+    assert on1[3].sum() == N * (N - 1) and 0 < on1[4].sum() < N * (N - 1)
