@@ -551,13 +551,41 @@ __global__ __launch_bounds__(kBlock) void k_rdf(const float4* __restrict__ pos, 
   const float thr[4] = {t0, t1, t2, t3};
   const float Tmax = T[nbins], Tlast = T[nbins - 1];
   unsigned int last = 0u;
+  if constexpr (!PERIODIC && SMALL) {
+    // hard walls: the reference arithmetic of rdf_exact_bin without its
+    // early returns, 8 pairs at a time (their LDS reads in flight together)
+    const int jlo = valid ? (diag ? (int)threadIdx.x : -1) : nj;
+    for (int j0b = 0; j0b < nj; j0b += 8) {
+      constexpr int B = 8;
+      float r2[B];
+      int kk[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        r2[b] = rdf_wall_r2<RULE>(pi, PJ[min(j0b + b, nj - 1)]);
+        const float e = RULE == LJ_RDF_RBINS ? __builtin_amdgcn_sqrtf(r2[b]) * est_scale
+                                             : r2[b] * est_scale;
+        kk[b] = rdf_bin(r2[b], T, nbins, (int)e);
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int jj = j0b + b;
+        // kept: r2 > 1e-6 (GPU rule) or below the last bin's end (the others)
+        const bool keep = RULE == LJ_RDF_REF_GPU ? r2[b] > 1e-6f : r2[b] < Tmax;
+        const bool count = (jj < nj) & (jj > jlo) & keep;
+        const bool inlast = r2[b] >= Tlast;
+        last += (count & inlast) ? 1u : 0u;
+        if (count & !inlast) atomicAdd(&hw[kk[b]], 1u);
+      }
+    }
+  } else {
 #pragma unroll 4
-  for (int jj = 0; jj < nj; ++jj) {
-    if (!valid || (diag && jj <= (int)threadIdx.x)) continue;
-    const int bin =
-        rdf_exact_bin<RULE, PERIODIC>(pi, PJ[jj], L, (float)L, thr, T, nbins, Tmax, Tlast, est_scale);
-    if (bin == nbins - 1) ++last;
-    else if (bin >= 0) atomicAdd(&hw[bin], 1u);
+    for (int jj = 0; jj < nj; ++jj) {
+      if (!valid || (diag && jj <= (int)threadIdx.x)) continue;
+      const int bin = rdf_exact_bin<RULE, PERIODIC>(pi, PJ[jj], L, (float)L, thr, T, nbins, Tmax,
+                                                    Tlast, est_scale);
+      if (bin == nbins - 1) ++last;
+      else if (bin >= 0) atomicAdd(&hw[bin], 1u);
+    }
   }
   rdf_flush(hs, hw, SMALL ? kWaves : 1, nbins, last, hist);  // !SMALL: the waves share one
 }

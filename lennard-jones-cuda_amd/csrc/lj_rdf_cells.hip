@@ -455,18 +455,6 @@ __device__ __forceinline__ void shell2_offset(int o, int& dx, int& dy, int& dz)
   dz = k / 25 - 2;
 }
 
-// the reference's r^2 of a hard-wall pair (no image), as rdf_exact_bin
-// computes it: float differences, then fmaf (GPU rule) or a plain float sum
-template <int RULE>
-__device__ __forceinline__ float wall_r2(const float4& pi, const uint4& rj)
-{
-#pragma clang fp contract(off)
-  const float rx = pi.x - __uint_as_float(rj.x), ry = pi.y - __uint_as_float(rj.y),
-              rz = pi.z - __uint_as_float(rj.z);
-  if constexpr (RULE == LJ_RDF_REF_GPU) return __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-  else return rx * rx + ry * ry + rz * rz;
-}
-
 // WALL: a hard-wall box.  The records hold float coordinates, neighbours
 // beyond the grid are left out (no wrap), and every candidate pair takes the
 // reference arithmetic directly (r^2 and the bin from the threshold table):
@@ -579,7 +567,9 @@ __global__ __launch_bounds__(kCellBlock) void k_rdf_cells2(
 #pragma unroll
           for (int b = 0; b < B; ++b) {
             const int t = tb + b * P, g = jb + t;
-            r2[b] = wall_r2<RULE>(pi, sj[min(t, jn - 1)]);
+            const uint4 rj = sj[min(t, jn - 1)];
+            r2[b] = rdf_wall_r2<RULE>(pi, make_float4(__uint_as_float(rj.x), __uint_as_float(rj.y),
+                                                      __uint_as_float(rj.z), 0.f));
             ok[b] = (t < jn) & ((g >= nself) | (g > iloc));
             const float e = RULE == LJ_RDF_RBINS ? __builtin_amdgcn_sqrtf(r2[b]) * est_scale
                                                  : r2[b] * est_scale;

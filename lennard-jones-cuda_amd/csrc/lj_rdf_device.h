@@ -65,6 +65,17 @@ __device__ __forceinline__ float mi_gpu1(float d, float Lf, float t0)
 // The reference arithmetic for one pair: image, r^2, bin (MDSystem.cpp:269-285
 // for REF_CPU and RBINS, MDSystem.cu:28-42 for REF_GPU).  Returns the bin, or
 // -1 for a pair that is not counted.
+// the reference's r^2 of a pair without image (hard walls), as rdf_exact_bin
+// computes it: float differences, then fmaf (GPU rule) or a plain float sum
+template <int RULE>
+__device__ __forceinline__ float rdf_wall_r2(const float4& pi, const float4& pj)
+{
+#pragma clang fp contract(off)
+  const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+  if constexpr (RULE == LJ_RDF_REF_GPU) return __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+  else return rx * rx + ry * ry + rz * rz;
+}
+
 template <int RULE, bool PERIODIC>
 __device__ __forceinline__ int rdf_exact_bin(const float4& pi, const float4& pj, double L, float Lf,
                                              const float (&thr)[4], const float* T, int nbins,
