@@ -431,8 +431,12 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
     n3_items(c->plan3.ng, it.data());
     n3_step_items(c->plan3, it.data() + c->plan3.nitems);
     A(dev_alloc(&c->n3items, it.size()));
+    // copies on the context's stream: a null-stream copy can break another
+    // thread's stream capture
     if (rc == LJ_OK &&
-        hipMemcpy(c->n3items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice) != hipSuccess)
+        (hipMemcpyAsync(c->n3items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice,
+                        c->stream) != hipSuccess ||
+         hipStreamSynchronize(c->stream) != hipSuccess))
       A(fail(LJ_ERR_DEVICE, "item table upload failed"));
     c->plan3.items = c->n3items;
     c->plan3.items_step = c->n3items + c->plan3.nitems;
@@ -455,23 +459,24 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
   if (rc == LJ_OK && hipEventCreate(&c->e1) != hipSuccess) A(fail(LJ_ERR_DEVICE, "hipEventCreate"));
   if (rc == LJ_OK) A(ensure_hist(c, 1024));
   if (rc == LJ_OK) {
-    // zero state: host buffers copied so that no memset node ever enters a graph
+    // zero state: host buffers copied so that no memset node ever enters a
+    // graph, on the context's stream (see the item table above)
     std::vector<float> z((size_t)4 * N, 0.f);
     lj_obs o{};
     const size_t bytes = sizeof(float) * 4 * (size_t)N;
-    if (hipMemcpy(c->pos, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->vel, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->force, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->tF, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->obs, &o, sizeof(o), hipMemcpyHostToDevice) != hipSuccess)
+    hipStream_t st = c->stream;
+    const auto up = [st](void* d, const void* h, size_t n) {
+      return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    if (!up(c->pos, z.data(), bytes) || !up(c->vel, z.data(), bytes) ||
+        !up(c->force, z.data(), bytes) || !up(c->tF, z.data(), bytes) ||
+        !up(c->obs, &o, sizeof(o)) || hipStreamSynchronize(st) != hipSuccess)
       A(fail(LJ_ERR_DEVICE, "initial upload failed"));
     if (rc == LJ_OK && is_fp64(c)) {
       std::vector<double> zd((size_t)4 * N, 0.);
       const size_t db = sizeof(double) * 4 * (size_t)N;
-      if (hipMemcpy(c->xd, zd.data(), db, hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(c->vd, zd.data(), db, hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(c->fd, zd.data(), db, hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(c->tfd, zd.data(), db, hipMemcpyHostToDevice) != hipSuccess)
+      if (!up(c->xd, zd.data(), db) || !up(c->vd, zd.data(), db) || !up(c->fd, zd.data(), db) ||
+          !up(c->tfd, zd.data(), db) || hipStreamSynchronize(st) != hipSuccess)
         A(fail(LJ_ERR_DEVICE, "initial upload failed"));
     }
   }

@@ -214,9 +214,14 @@ int build_tables(lj_batch* b)
   if (!b->box) LJ_TRY(dev_alloc(&b->box, b->nboxes));
   if (!b->pblock) LJ_TRY(dev_alloc(&b->pblock, pblock.size()));
   if (!b->blk) LJ_TRY(dev_alloc(&b->blk, (size_t)blk0 * kBlkFields));
-  LJ_HIP(hipMemcpy(b->box, bp.data(), sizeof(BoxParam) * bp.size(), hipMemcpyHostToDevice));
-  LJ_HIP(hipMemcpy(b->pblock, pblock.data(), sizeof(int2) * pblock.size(), hipMemcpyHostToDevice));
-  LJ_HIP(hipMemcpy(b->items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice));
+  // on the batch's stream: a null-stream copy can break another thread's capture
+  LJ_HIP(hipMemcpyAsync(b->box, bp.data(), sizeof(BoxParam) * bp.size(), hipMemcpyHostToDevice,
+                        b->stream));
+  LJ_HIP(hipMemcpyAsync(b->pblock, pblock.data(), sizeof(int2) * pblock.size(), hipMemcpyHostToDevice,
+                        b->stream));
+  LJ_HIP(hipMemcpyAsync(b->items, items.data(), sizeof(BatchItem) * items.size(),
+                        hipMemcpyHostToDevice, b->stream));
+  LJ_HIP(hipStreamSynchronize(b->stream));
   b->hbox = bp;
   b->nblocks = (int)pblock.size();
   b->nitems = (int)items.size();
@@ -353,12 +358,14 @@ int lj_batch_create(int nboxes, const int* N, const double* rho, const double* T
     std::memset(o.data(), 0, sizeof(lj_obs) * nboxes);
     std::vector<double> zs((size_t)nboxes * kNStat, 0.);
     const size_t bytes = sizeof(float4) * (size_t)off;
-    if (hipMemcpy(b->pos, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->vel, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->force, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->tF, z.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->obs, o.data(), sizeof(lj_obs) * nboxes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->stats, zs.data(), sizeof(double) * zs.size(), hipMemcpyHostToDevice) != hipSuccess)
+    hipStream_t st = b->stream;
+    const auto up = [st](void* d, const void* h, size_t n) {
+      return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    if (!up(b->pos, z.data(), bytes) || !up(b->vel, z.data(), bytes) ||
+        !up(b->force, z.data(), bytes) || !up(b->tF, z.data(), bytes) ||
+        !up(b->obs, o.data(), sizeof(lj_obs) * nboxes) ||
+        !up(b->stats, zs.data(), sizeof(double) * zs.size()) || hipStreamSynchronize(st) != hipSuccess)
       A(fail(LJ_ERR_DEVICE, "initial upload failed"));
   }
   if (rc != LJ_OK) {

@@ -100,7 +100,7 @@ inline int attach_rdf_cells(lj_ctx* c, RdfArgs& a)
   int reach = 1;
   const int nc = rdf_cells_grid(a, &reach);
   if (nc <= 0) return LJ_OK;
-  const int rc = rdf_cells_ensure(c->rdf_cells, a.N, nc * nc * nc, a.nbins);
+  const int rc = rdf_cells_ensure(c->rdf_cells, a.N, nc * nc * nc, a.nbins, c->stream);
   if (rc != LJ_OK) return rc;
   a.cells = &c->rdf_cells;
   return LJ_OK;

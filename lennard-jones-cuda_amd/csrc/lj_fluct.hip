@@ -284,8 +284,9 @@ int fl_configure(FlSampler& s, hipStream_t st, int nbox, const double* L, const 
   LJ_TRY(dev_alloc(&s.counts, ne));
   LJ_TRY(dev_alloc(&s.last, ne));
   LJ_TRY(dev_alloc(&s.stats, ne * kFlStat));
-  LJ_HIP(hipMemcpy(s.lad, hl.data(), sizeof(FlLadderDev) * hl.size(), hipMemcpyHostToDevice));
-  LJ_HIP(hipMemcpy(s.tab, ht.data(), sizeof(double) * ne, hipMemcpyHostToDevice));
+  LJ_HIP(hipMemcpyAsync(s.lad, hl.data(), sizeof(FlLadderDev) * hl.size(), hipMemcpyHostToDevice, st));
+  LJ_HIP(hipMemcpyAsync(s.tab, ht.data(), sizeof(double) * ne, hipMemcpyHostToDevice, st));
+  LJ_HIP(hipStreamSynchronize(st));
   s.nbox = nbox;
   s.nl = nl;
   s.E = E;

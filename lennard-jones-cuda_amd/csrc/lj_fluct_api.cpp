@@ -121,7 +121,9 @@ int lj_set_step_sampling(lj_ctx* c, const lj_step_sampling* s)
     }
     std::vector<float> e(nbins + 1);
     rdf_bin_thresholds(rule, nbins, dr2, rmax, e.data());
-    LJ_HIP(hipMemcpy(c->step_tab, e.data(), sizeof(float) * (nbins + 1), hipMemcpyHostToDevice));
+    LJ_HIP(hipMemcpyAsync(c->step_tab, e.data(), sizeof(float) * (nbins + 1), hipMemcpyHostToDevice,
+                          st));
+    LJ_HIP(hipStreamSynchronize(st));
     a.pos = c->pos;
     a.N = c->cfg.N;
     a.periodic = c->cfg.boundary == LJ_BC_PERIODIC;

@@ -184,7 +184,9 @@ int rdf_cells_per_axis(double L, double end);
 // cells per axis of the periodic layout the pass takes (0: none), and its
 // reach (1 or 2)
 int rdf_cells_layout(double L, double end, int* reach);
-int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins);  // LJ_OK or a status
+// LJ_OK or a status; st: the owner's stream (its work is waited for, and the
+// new arrays are cleared on it)
+int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins, hipStream_t st);
 void rdf_cells_free(RdfCellsWs& w);
 
 struct RdfArgs {

@@ -748,10 +748,13 @@ int rdf_cells_layout(double L, double end, int* reach)
   return nc1;
 }
 
-int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins)
+int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins, hipStream_t st)
 {
   if (N <= w.cap_n && ncell <= w.cap_cells && nbins <= w.cap_bins && w.flag) return LJ_OK;
-  (void)hipDeviceSynchronize();  // the arrays may be in use by earlier passes
+  // the arrays may be in use by earlier passes on the owner's stream (only
+  // that stream: a device-wide wait can break another thread's capture)
+  if (hipStreamSynchronize(st) != hipSuccess)
+    return fail(LJ_ERR_DEVICE, "rdf_cells_ensure: synchronizing the stream failed");
   rdf_cells_free(w);
   const int cn = N, cc = ncell, cb = nbins;
   LJ_TRY(dev_alloc(&w.count, cc));
@@ -767,11 +770,11 @@ int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins)
   LJ_TRY(dev_alloc(&w.bcnt, kCellScanMax));
   w.cap_m = kCellScanMax;
   // the counts, replicas and flags start at zero; the passes clear them after use
-  if (hipMemset(w.count, 0, sizeof(int) * cc) != hipSuccess ||
-      hipMemset(w.reps, 0, sizeof(unsigned) * kRdfReps * cb) != hipSuccess ||
-      hipMemset(w.other, 0, sizeof(unsigned long long) * kRdfReps) != hipSuccess ||
-      hipMemset(w.flag, 0, sizeof(unsigned) * 3) != hipSuccess ||
-      hipDeviceSynchronize() != hipSuccess)
+  if (hipMemsetAsync(w.count, 0, sizeof(int) * cc, st) != hipSuccess ||
+      hipMemsetAsync(w.reps, 0, sizeof(unsigned) * kRdfReps * cb, st) != hipSuccess ||
+      hipMemsetAsync(w.other, 0, sizeof(unsigned long long) * kRdfReps, st) != hipSuccess ||
+      hipMemsetAsync(w.flag, 0, sizeof(unsigned) * 3, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
     return fail(LJ_ERR_DEVICE, "rdf_cells_ensure: clearing the workspace failed");
   w.cap_n = cn;
   w.cap_cells = cc;
