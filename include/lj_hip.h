@@ -17,7 +17,10 @@
  *      one HIP stream and the captured step graph.  Every call returns
  *      LJ_OK (0) or a negative status and never exits the process.
  *      Calls are synchronous unless documented otherwise; distinct contexts
- *      may be used from distinct host threads.
+ *      may be used from distinct host threads (their work, copies and waits
+ *      stay on their own streams, so one thread's step capture is not
+ *      disturbed by another's calls; the compatibility set (1) keeps the
+ *      reference's synchronous semantics instead).
  *
  *  (3) Device-pointer API for callers that keep their own device buffers
  *      (the PyTorch binding): forces and the adjoint pair kernel on caller
