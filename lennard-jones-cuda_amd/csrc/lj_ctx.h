@@ -100,8 +100,13 @@ inline int attach_rdf_cells(lj_ctx* c, RdfArgs& a)
   int reach = 1;
   const int nc = rdf_cells_grid(a, &reach);
   if (nc <= 0) return LJ_OK;
-  const int rc = rdf_cells_ensure(c->rdf_cells, a.N, nc * nc * nc, a.nbins, c->stream);
+  bool grown = false;
+  const int rc = rdf_cells_ensure(c->rdf_cells, a.N, nc * nc * nc, a.nbins, c->stream, &grown);
   if (rc != LJ_OK) return rc;
+  // reallocated: captured steps that sample the RDF hold the old arrays'
+  // addresses (lj_rdf and the per-step sampling share the workspace); they
+  // are captured again on the next run
+  if (grown) drop_graph(c);
   a.cells = &c->rdf_cells;
   return LJ_OK;
 }

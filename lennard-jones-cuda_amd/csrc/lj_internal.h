@@ -185,8 +185,8 @@ int rdf_cells_per_axis(double L, double end);
 // reach (1 or 2)
 int rdf_cells_layout(double L, double end, int* reach);
 // LJ_OK or a status; st: the owner's stream (its work is waited for, and the
-// new arrays are cleared on it)
-int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins, hipStream_t st);
+// new arrays are cleared on it); *grown: whether the arrays were reallocated
+int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins, hipStream_t st, bool* grown);
 void rdf_cells_free(RdfCellsWs& w);
 
 struct RdfArgs {

@@ -34,8 +34,8 @@
 // the order within a cell.  Workgroup histograms go to one of kRdfReps
 // replicas, which the reduction sums, doubles into ordered-pair counts, adds
 // to hist and clears for the next pass (no memset node in a captured step).
+#include <algorithm>
 #include <cmath>
-
 #include <cstdlib>
 
 #include "lj_internal.h"
@@ -748,15 +748,18 @@ int rdf_cells_layout(double L, double end, int* reach)
   return nc1;
 }
 
-int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins, hipStream_t st)
+int rdf_cells_ensure(RdfCellsWs& w, int N, int ncell, int nbins, hipStream_t st, bool* grown)
 {
+  *grown = false;
   if (N <= w.cap_n && ncell <= w.cap_cells && nbins <= w.cap_bins && w.flag) return LJ_OK;
+  *grown = w.flag != nullptr;  // arrays an earlier capture may hold
   // the arrays may be in use by earlier passes on the owner's stream (only
   // that stream: a device-wide wait can break another thread's capture)
   if (hipStreamSynchronize(st) != hipSuccess)
     return fail(LJ_ERR_DEVICE, "rdf_cells_ensure: synchronizing the stream failed");
+  // grown to at least the old capacities, so that alternating layouts settle
+  const int cn = std::max(N, w.cap_n), cc = std::max(ncell, w.cap_cells), cb = std::max(nbins, w.cap_bins);
   rdf_cells_free(w);
-  const int cn = N, cc = ncell, cb = nbins;
   LJ_TRY(dev_alloc(&w.count, cc));
   LJ_TRY(dev_alloc(&w.start, (size_t)cc + 1));
   LJ_TRY(dev_alloc(&w.cellof, cn));

@@ -309,3 +309,27 @@ np.savez(sys.argv[2], *out)
     assert on1[0].sum() == 30 * N * (N - 1)
     assert on1[2][0] > 1000  # coordinates outside [0, L]: the clamped edge cells
     assert on1[3].sum() == N * (N - 1) and 0 < on1[4].sum() < N * (N - 1)
+
+
+def test_step_rdf_after_lj_rdf_grows_the_cell_workspace():
+    """lj_rdf with more bins than the per-step sampling reallocates the
+    cell-list workspace the captured steps use: the steps are captured again,
+    and the sampled sums and the trajectory equal a run without that call."""
+    N, rho, T0 = 16384, 0.85, 1.0
+    L, pos, vel = perturbed_state(N, rho, T0, seed=14)
+    runs = []
+    for grow in (False, True):
+        with ljamd.LJSystem(N, rho, T0, canonical=ljamd.TVN) as s:
+            s.set_state(pos, vel)
+            s.CalculateForces()
+            s.set_step_sampling(rdf=ljamd.SAMPLE_RDF_SUM, rdf_rule=ljamd.RDF_REF_GPU, rdf_nbins=256)
+            s.Integrate(0.005, 5)
+            if grow:  # 1024 bins over the same range: a larger workspace
+                got = s.rdf(ljamd.RDF_REF_CPU, 1024, dr2=0.025)
+                assert 0 < got.sum() < N * (N - 1)
+            s.Integrate(0.005, 5)
+            counts, n = s.step_rdf_counts()
+            runs.append((counts.astype(np.int64), n, s.pos.copy()))
+    assert runs[0][1] == runs[1][1] == 10
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][2], runs[1][2])
