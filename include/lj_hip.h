@@ -141,7 +141,8 @@ enum {
 };
 enum {
   LJ_PREC_FP32 = 0,        /* FP32 pair math and per-slice sums, FP64 cross-slice sums */
-  LJ_PREC_COMPENSATED = 1, /* + TwoSum-compensated per-particle sums inside each slice */
+  LJ_PREC_COMPENSATED = 1, /* + per-particle sums in FP32 blocks of 32 pairs added by
+                              TwoSum, inside each partial entry (slice or group pair) */
   LJ_PREC_FP64 = 2         /* double state, double pair math, the integrator formulas of
                               MDSystem.cpp:442-510 in double: the reference of the
                               precision study.  The float arrays are rounded mirrors

@@ -405,10 +405,13 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
   c->L = box_length(cfg->N, cfg->rho);
   refresh_scalars(c);
   c->plan = make_force_plan(cfg->N, cfg->jslices, 1);
-  // the pair-once kernel serves the FP32 mode at sizes that fill the GPU; an
-  // explicit j-slice count keeps the sliced kernel
-  if (cfg->precision == LJ_PREC_FP32 && cfg->jslices == 0 && n3_eligible(cfg->N))
-    c->plan3 = make_n3_plan(cfg->N);
+  // the pair-once kernel serves the FP32 and compensated modes at sizes that
+  // fill the GPU; an explicit j-slice count keeps the sliced kernel
+  if (cfg->jslices == 0 && n3_eligible(cfg->N)) {
+    if (cfg->precision == LJ_PREC_FP32) c->plan3 = make_n3_plan(cfg->N, false);
+    else if (cfg->precision == LJ_PREC_COMPENSATED && n3_comp_enabled())
+      c->plan3 = make_n3_plan(cfg->N, true);
+  }
   const size_t npart = (size_t)std::max(c->plan.S, c->plan3.S) * c->plan.Npad;
   const int N = cfg->N;
   c->nblk = (N + kBlock - 1) / kBlock;

@@ -1,5 +1,6 @@
 // lj_force_n3.hip -- the force pass with every unordered pair evaluated once
-// (Newton's third law), for the FP32 mode of single boxes.
+// (Newton's third law), for the FP32 and compensated modes of single boxes
+// (k_force_n3, k_force_n3c) and for batches (k_batch_force_n3).
 //
 // Physics as the reference's pair loop (MDSystem.cu:21-138, MDSystem.cpp:
 // 260-309); only the order of evaluation differs.
@@ -196,6 +197,142 @@ __device__ __forceinline__ void self_tile(const uint4* P0, int lane, int i1, int
   }
 }
 
+// rotate by k lanes: lane l receives lane (l + k) mod 64 (full execution mask)
+__device__ __forceinline__ float rotk(float v, int k)
+{
+  const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + k) & 63) << 2, __float_as_int(v)));
+}
+
+__device__ __forceinline__ void acc_two_sum(Acc& hi, Acc& lo, const Acc& x)
+{
+  two_sum(hi.fx, lo.fx, x.fx);
+  two_sum(hi.fy, lo.fy, x.fy);
+  two_sum(hi.fz, lo.fz, x.fz);
+  two_sum(hi.a, lo.a, x.a);
+  two_sum(hi.b, lo.b, x.b);
+}
+
+// Compensated mode (LJ_PREC_COMPENSATED): the sums of pair_tile in blocks of
+// kJB = 32 pairs in FP32, each block added into a (hi, lo) pair by TwoSum, as
+// the sliced kernel's j_range does.
+//   i side: the lane's i meets one j per step, so a block is 32 steps; the
+//   block sums go into (a1, l1) / (a2, l2) at t = 32 and t = 64.
+//   j side: the travelling sums meet two i per step, so a block is 16 steps.
+//   The (hi, lo) pairs of the j slots stay in their lanes between folds: at
+//   the fold after step t = 16 k - 1 lane l's block holds slot (l + 16 k) mod
+//   64 and its (hi, lo) slot (l + 16 (k - 1)), so the pairs are first rotated
+//   by 16 lanes.  After the fold at t = 64 lane l holds slot l, as in
+//   pair_tile.  The i (hi, lo) are added into a1 / a2 by the caller.
+template <bool PERIODIC, bool TIES, bool PARTIAL, bool ENERGY>
+__device__ __forceinline__ void pair_tile_c(const uint4* P0, int lane, const PosT<PERIODIC>& pi1,
+                                            const PosT<PERIODIC>& pi2, bool v1, bool v2, float C1,
+                                            int nsub, Acc& a1, Acc& a2, Acc& l1, Acc& l2,
+                                            float (&J)[2][4])
+{
+  constexpr int NJ = ENERGY ? 4 : 3;
+#pragma unroll
+  for (int st = 0; st < 2; ++st) {
+    if (st >= nsub) break;  // wave-uniform
+    const uint4* P = P0 + st * 128;
+    float jh[4] = {0.f, 0.f, 0.f, 0.f}, jl[4] = {0.f, 0.f, 0.f, 0.f};
+    Acc b1, b2;
+    acc_zero(b1);
+    acc_zero(b2);
+#pragma unroll 1
+    for (int qt = 0; qt < 4; ++qt) {  // quarters of 16 steps
+      float jb[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+      for (int u = 0; u < 16; ++u) {
+        const uint4 pj = P[lane + 16 * qt + u];
+        bool ok1 = true, ok2 = true;
+        if constexpr (PARTIAL) {
+          const bool jv = pj.w != 0u;
+          ok1 = v1 && jv;
+          ok2 = v2 && jv;
+        }
+        float dx1, dy1, dz1, s1, e1, w1, dx2, dy2, dz2, s2, e2, w2;
+        terms<PERIODIC, TIES, PARTIAL>(pi1, pj, C1, ok1, dx1, dy1, dz1, s1, e1, w1);
+        terms<PERIODIC, TIES, PARTIAL>(pi2, pj, C1, ok2, dx2, dy2, dz2, s2, e2, w2);
+        b1.fx = __builtin_fmaf(dx1, s1, b1.fx);
+        b1.fy = __builtin_fmaf(dy1, s1, b1.fy);
+        b1.fz = __builtin_fmaf(dz1, s1, b1.fz);
+        b1.a += e1;
+        b1.b += w1;
+        b2.fx = __builtin_fmaf(dx2, s2, b2.fx);
+        b2.fy = __builtin_fmaf(dy2, s2, b2.fy);
+        b2.fz = __builtin_fmaf(dz2, s2, b2.fz);
+        b2.a += e2;
+        b2.b += w2;
+        jb[0] = rot1(__builtin_fmaf(-dx2, s2, __builtin_fmaf(-dx1, s1, jb[0])));
+        jb[1] = rot1(__builtin_fmaf(-dy2, s2, __builtin_fmaf(-dy1, s1, jb[1])));
+        jb[2] = rot1(__builtin_fmaf(-dz2, s2, __builtin_fmaf(-dz1, s1, jb[2])));
+        if constexpr (ENERGY) jb[3] = rot1(jb[3] + ((e1 - w1) + (e2 - w2)));
+      }
+#pragma unroll
+      for (int c = 0; c < NJ; ++c) {
+        // (the first fold rotates zeros)
+        jh[c] = rotk(jh[c], 16);
+        jl[c] = rotk(jl[c], 16);
+        two_sum(jh[c], jl[c], jb[c]);
+      }
+      if (qt & 1) {  // the end of a 32-step block of the i sums
+        acc_two_sum(a1, l1, b1);
+        acc_two_sum(a2, l2, b2);
+        acc_zero(b1);
+        acc_zero(b2);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) J[st][c] = c < NJ ? jh[c] + jl[c] : 0.f;
+  }
+}
+
+// Compensated form of self_tile: the i sums in blocks of 32 steps, each
+// added into (a, l) by TwoSum
+template <bool PERIODIC, bool TIES>
+__device__ __forceinline__ void self_tile_c(const uint4* P0, int lane, int i1, int i2, int jbase,
+                                            const PosT<PERIODIC>& pi1, const PosT<PERIODIC>& pi2,
+                                            bool v1, bool v2, float C1, int nsub, Acc& a1, Acc& a2,
+                                            Acc& l1, Acc& l2)
+{
+#pragma unroll
+  for (int st = 0; st < 2; ++st) {
+    if (st >= nsub) break;  // wave-uniform
+    const uint4* P = P0 + st * 128;
+#pragma unroll 1
+    for (int half = 0; half < 2; ++half) {
+      Acc b1, b2;
+      acc_zero(b1);
+      acc_zero(b2);
+#pragma unroll 8
+      for (int t = 32 * half; t < 32 * half + 32; ++t) {
+        const int jx = lane + t;
+        const uint4 pj = P[jx];
+        const int j = jbase + st * 64 + (jx & 63);
+        const bool jv = pj.w != 0u;
+        const bool ok1 = v1 && jv && j != i1;
+        const bool ok2 = v2 && jv && j != i2;
+        float dx, dy, dz, s, e, w;
+        terms<PERIODIC, TIES, true>(pi1, pj, C1, ok1, dx, dy, dz, s, e, w);
+        b1.fx = __builtin_fmaf(dx, s, b1.fx);
+        b1.fy = __builtin_fmaf(dy, s, b1.fy);
+        b1.fz = __builtin_fmaf(dz, s, b1.fz);
+        b1.a += e;
+        b1.b += w;
+        terms<PERIODIC, TIES, true>(pi2, pj, C1, ok2, dx, dy, dz, s, e, w);
+        b2.fx = __builtin_fmaf(dx, s, b2.fx);
+        b2.fy = __builtin_fmaf(dy, s, b2.fy);
+        b2.fz = __builtin_fmaf(dz, s, b2.fz);
+        b2.a += e;
+        b2.b += w;
+      }
+      acc_two_sum(a1, l1, b1);
+      acc_two_sum(a2, l2, b2);
+    }
+  }
+}
+
 __device__ __forceinline__ void encode(const Acc& a, int cls, float (&v)[5])
 {
   v[0] = a.fx;
@@ -208,13 +345,6 @@ __device__ __forceinline__ void encode(const Acc& a, int cls, float (&v)[5])
     v[3] = a.b;
     v[4] = a.a;
   }
-}
-
-// rotate by k lanes: lane l receives lane (l + k) mod 64 (full execution mask)
-__device__ __forceinline__ float rotk(float v, int k)
-{
-  const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + k) & 63) << 2, __float_as_int(v)));
 }
 
 // One step of two once-evaluated pair streams of a wave: the lane's i particle
@@ -470,7 +600,9 @@ __device__ __forceinline__ void n3_diag_pair(N3Lds<2>& lds, const uint4* __restr
 // waves.  Each half writes its own slots, 2 h + half for the i side
 // and 2 g + half for the j side (zeros for the j it did not meet): every
 // particle still receives exactly one entry per slot, S = 2 groups.
-template <bool PERIODIC, bool TIES, int W, bool ENERGY, bool SPLIT = false>
+// COMP: the compensated mode's sums (pair_tile_c, self_tile_c; W = 2, no
+// split items).
+template <bool PERIODIC, bool TIES, int W, bool ENERGY, bool SPLIT = false, bool COMP = false>
 __device__ __forceinline__ void n3_item(N3Lds<W>& lds, const uint4* __restrict__ q,
                                         const float4* __restrict__ pf,
                                         float4* __restrict__ fpart, float* __restrict__ apart,
@@ -521,7 +653,25 @@ __device__ __forceinline__ void n3_item(N3Lds<W>& lds, const uint4* __restrict__
   acc_zero(a2);
   const uint4* P0 = s_pos + wj * 256;
   float J[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-  if (cls == 1) {
+  if constexpr (COMP) {
+    static_assert(W == 2 && !SPLIT, "compensated items: groups of 256, not split");
+    Acc l1, l2;
+    acc_zero(l1);
+    acc_zero(l2);
+    if (cls == 1) {
+      if (partial)
+        pair_tile_c<PERIODIC, TIES, true, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, nsub, a1, a2, l1,
+                                                  l2, J);
+      else
+        pair_tile_c<PERIODIC, TIES, false, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, 2, a1, a2, l1,
+                                                   l2, J);
+    } else if (cls == 2) {
+      self_tile_c<PERIODIC, TIES>(P0, lane, i1, i2, kG * h + kTile * wj, pi1, pi2, v1, v2, C1, nsub,
+                                  a1, a2, l1, l2);
+    }
+    a1.fx += l1.fx, a1.fy += l1.fy, a1.fz += l1.fz, a1.a += l1.a, a1.b += l1.b;
+    a2.fx += l2.fx, a2.fy += l2.fy, a2.fz += l2.fz, a2.a += l2.a, a2.b += l2.b;
+  } else if (cls == 1) {
     if (partial)
       pair_tile<PERIODIC, TIES, true, ENERGY>(P0, lane, pi1, pi2, v1, v2, C1, nsub, a1, a2, J, st0);
     else
@@ -657,6 +807,27 @@ __global__ __launch_bounds__(64 * W * W) __attribute__((amdgpu_waves_per_eu(8)))
                                             it.x, it.y, SPLIT ? wg & 1 : 0);
 }
 
+// Compensated mode: one workgroup per item of the unpaired list (the diagonal
+// items through self_tile_c), groups of 256.  The (hi, lo) pairs of the
+// compensated sums take about 26 more registers than the FP32 loop, which
+// holds eight waves per SIMD at 64; the kernel runs at the occupancy its
+// registers allow.
+template <bool PERIODIC, bool TIES, bool ENERGY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_force_n3c(const uint4* __restrict__ q,
+                                                   const float4* __restrict__ pf,
+                                                   const int2* __restrict__ items,
+                                                   float4* __restrict__ fpart,
+                                                   float* __restrict__ apart,
+                                                   double* __restrict__ ew, int N, int Npad,
+                                                   float C1)
+{
+  __shared__ N3Lds<2> lds;
+  const int2 it = items[blockIdx.x];
+  n3_item<PERIODIC, TIES, 2, ENERGY, false, true>(lds, q, pf, fpart, apart,
+                                                  ENERGY ? nullptr : ew + 2 * (size_t)blockIdx.x, N,
+                                                  Npad, C1, it.x, it.y);
+}
+
 // batched ensemble: item = (box, g, h) over all boxes; each box's partials at
 // its own rows (pbase, S = groups of 256, Npad); ENERGY: per-particle energies
 // kept (single evaluations), else the time step's variant with the item's
@@ -746,23 +917,32 @@ int n3_split(int W, int ng)
   return n3_pass_time(2 * full, 0.5, cus) < 0.9 * n3_pass_time(paired, 1.0, cus) ? 2 : 1;
 }
 
-ForcePlan make_n3_plan(int N)
+ForcePlan make_n3_plan(int N, bool comp)
 {
   ForcePlan pl;
-  const int W = n3_waves(N);
+  const int W = comp ? 2 : n3_waves(N);
   const int G = kTile * W;
   pl.N = N;
   pl.n3 = W;
+  pl.comp = comp ? 1 : 0;
   pl.ng = (N + G - 1) / G;
-  pl.split = n3_split(W, pl.ng);
+  pl.split = comp ? 1 : n3_split(W, pl.ng);
   pl.S = pl.ng * pl.split;
   pl.jchunk = G;
   pl.Npad = (N + 63) & ~63;
   pl.nitems = pl.ng * (pl.ng + 1) / 2;
-  // the time step pairs its diagonal items (n3_step_items), unless split
-  pl.nitems_step =
-      W == 2 && pl.split == 1 ? pl.ng * (pl.ng - 1) / 2 + (pl.ng + 1) / 2 : pl.nitems;
+  // the time step pairs its diagonal items (n3_step_items), unless split or
+  // compensated
+  pl.nitems_step = W == 2 && pl.split == 1 && !comp ? pl.ng * (pl.ng - 1) / 2 + (pl.ng + 1) / 2
+                                                    : pl.nitems;
   return pl;
+}
+
+bool n3_comp_enabled()
+{
+  // LJ_PAIR_ONCE_COMP=0 keeps the compensated mode on the sliced kernel
+  const char* e = std::getenv("LJ_PAIR_ONCE_COMP");
+  return !(e && e[0] == '0');
 }
 
 bool n3_eligible(int N)
@@ -830,6 +1010,29 @@ hipError_t launch_n3_w(hipStream_t st, const ForcePlan& pl, const PairScalars& p
   return hipGetLastError();
 }
 
+template <bool ENERGY>
+hipError_t launch_n3_c(hipStream_t st, const ForcePlan& pl, const PairScalars& ps, const uint4* q,
+                       const float4* posf, float4* fpart, float* apart, double* ew, bool ties)
+{
+  // both variants run the unpaired item list (items_step equals it)
+  const int nitems = ENERGY ? pl.nitems : pl.nitems_step;
+  const int2* items = ENERGY ? pl.items : pl.items_step;
+  dim3 grid(nitems), block(256);
+  const LaunchStamp ts = tl_force_stamp;
+  if (ps.periodic) {
+    if (ties)
+      hipExtLaunchKernelGGL((k_force_n3c<true, true, ENERGY>), grid, block, 0, st, ts.start, ts.stop,
+                            0, q, posf, items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+    else
+      hipExtLaunchKernelGGL((k_force_n3c<true, false, ENERGY>), grid, block, 0, st, ts.start, ts.stop,
+                            0, q, posf, items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+  } else {
+    hipExtLaunchKernelGGL((k_force_n3c<false, false, ENERGY>), grid, block, 0, st, ts.start, ts.stop,
+                          0, q, posf, items, fpart, apart, ew, pl.N, pl.Npad, ps.C1);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
 
 template <bool PERIODIC, bool TIES, bool ENERGY>
@@ -868,6 +1071,11 @@ hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalar
 {
   // ew != null: the time step's variant (force partials + per-workgroup totals)
   if (pl.split == 2 && pl.n3 != 2) return hipErrorInvalidValue;
+  if (pl.comp) {
+    if (pl.n3 != 2 || pl.split != 1 || pl.nitems_step != pl.nitems) return hipErrorInvalidValue;
+    return ew ? launch_n3_c<false>(st, pl, ps, q, posf, fpart, apart, ew, false)
+              : launch_n3_c<true>(st, pl, ps, q, posf, fpart, apart, nullptr, ties);
+  }
   if (ew) {
     if (pl.split == 2) return launch_n3_w<2, false, true>(st, pl, ps, q, posf, fpart, apart, ew, false);
     if (pl.n3 == 2) return launch_n3_w<2, false, false>(st, pl, ps, q, posf, fpart, apart, ew, false);

@@ -47,11 +47,16 @@ struct ForcePlan {
   // workgroups (sizes whose items fill under half the resident workgroups);
   // S = split x ng slots
   int split = 1;
+  // 1: the compensated mode's pair-once pass (k_force_n3c: W = 2, not split,
+  // diagonal items unpaired)
+  int comp = 0;
   // workgroups of the time step's pass, one (E, W) pair of totals each
   int step_wgs() const { return nitems_step * split; }
 };
 ForcePlan make_force_plan(int N, int jslices_override, int ipl);
-ForcePlan make_n3_plan(int N);
+ForcePlan make_n3_plan(int N, bool comp = false);
+// false when LJ_PAIR_ONCE_COMP=0 keeps the compensated mode on the sliced kernel
+bool n3_comp_enabled();
 // sizes at which the pair-once kernel serves the time step (N >= 3072)
 bool n3_eligible(int N);
 // the ng (ng + 1) / 2 group pairs of the pair-once kernel, in launch order

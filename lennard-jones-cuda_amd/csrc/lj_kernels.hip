@@ -900,7 +900,8 @@ hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& 
                         bool compensated, bool ties, double* step_ew)
 {
   const bool ga = ps.thr > 0.f;
-  if (pl.n3 && !compensated && !ga)
+  // the pair-once pass of the matching mode (plans are made per context)
+  if (pl.n3 && compensated == (pl.comp != 0) && !ga)
     return launch_force_n3(st, pl, ps, q, posf, fpart, apart, ties, step_ew);
   if (ps.periodic) {
     if (ties)

@@ -15,6 +15,8 @@ Tolerances (FP32 pair arithmetic, FP64 cross-slice sums):
   forces vs reference CPU path: per particle <= 1e-5 + the reference's own
                    deviation from FP64 at that particle (it accumulates in float).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -191,3 +193,42 @@ def test_pair_once_kernel(N, bc):
     assert rel(obs["V"], E) <= 1e-6 and rel(obs["W"], W) <= 1e-5
     assert ew_err <= 1e-5
     assert force_errors(xyz(f4), xyz(f_sl))["global_rel"] <= 2e-6
+
+
+@pytest.mark.parametrize("N,bc", [(16384, ljamd.BC_PERIODIC), (13001, ljamd.BC_WALL),
+                                  (8192, ljamd.BC_NONE)])
+def test_compensated_pair_once(N, bc):
+    """The compensated mode's pair-once pass (N >= 3072; blocks of 32 pairs in
+    FP32 added by TwoSum on both sides of every pair; 13001 ends in a partial
+    group) against FP64, the FP32 pair-once pass and the compensated sliced
+    kernel (LJ_PAIR_ONCE_COMP=0) on one state; two evaluations are
+    bit-identical."""
+    rho = 0.85
+    L, pos, _ = perturbed_state(N, rho, 1.0, seed=17)
+    print(f"\n[case] compensated pair-once N={N} bc={bc}", flush=True)
+    with ljamd.LJSystem(N, rho, 1.0, boundary=bc, precision=ljamd.PREC_COMPENSATED) as s:
+        assert s.force_plan() == (True, (N + 255) // 256)
+        s.set_state(pos, np.zeros_like(pos))
+        obs = s.CalculateForces()
+        fc = s.force.copy()
+        obs2 = s.CalculateForces()
+        assert np.array_equal(s.force, fc) and obs2["V"] == obs["V"] and obs2["W"] == obs["W"]
+    os.environ["LJ_PAIR_ONCE_COMP"] = "0"
+    try:
+        with ljamd.LJSystem(N, rho, 1.0, boundary=bc, precision=ljamd.PREC_COMPENSATED) as s:
+            assert not s.force_plan()[0]
+            s.set_state(pos, np.zeros_like(pos))
+            s.CalculateForces()
+            fsl = s.force.copy()
+    finally:
+        os.environ.pop("LJ_PAIR_ONCE_COMP", None)
+    f32, _ = _gpu_forces(N, rho, bc, pos)
+    F64, ew, E, W = forces_fp64(pos, L, bc == ljamd.BC_PERIODIC)
+    ec, esl, e32 = (force_errors(xyz(f), F64) for f in (fc, fsl, f32))
+    ew_err = np.max(np.abs(fc.reshape(N, 4)[:, 3] - ew[:, 0]) / np.maximum(np.abs(ew[:, 0]), 1.0))
+    print(f"  compensated pair-once {ec}\n  compensated sliced {esl}\n  fp32 pair-once {e32}\n"
+          f"  E {rel(obs['V'], E):.2e} W {rel(obs['W'], W):.2e} Force.w {ew_err:.2e}")
+    assert ec["global_rel"] <= 2e-6 and ec["max_rel_floor"] <= 1e-5
+    assert ec["global_rel"] <= e32["global_rel"]
+    assert rel(obs["V"], E) <= 1e-6 and rel(obs["W"], W) <= 1e-5 and ew_err <= 1e-5
+

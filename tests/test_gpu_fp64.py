@@ -131,9 +131,10 @@ def test_precision_workload_modes_full_size():
     """FP32 and compensated against the FP64 mode on the precision study's
     workload at full size (N = 131072, rho* = 0.3, hard wall, EVN,
     dt = 0.002), over its first 100 steps from one seeded state: the first
-    state's forces, E_pot and P over every step.  Measured on MI355X: forces
-    7.7e-7 / 4.6e-7 (global), E_pot 3.8e-7 / 2.4e-7, P 1.1e-5 / 8.7e-6; the
-    tolerances are about 3x those."""
+    state's forces, E_pot and P over every step.  Measured on MI355X (the
+    compensated mode on the pair-once pass): forces 7.7e-7 / 5.7e-7 (global),
+    E_pot 3.8e-7 / 2.3e-7, P 1.1e-5 / 8.3e-6; the tolerances are about 3x
+    those."""
     import types
 
     from ljamd import precision as pr

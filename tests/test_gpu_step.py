@@ -13,8 +13,8 @@ from oracle_lib import ORC, OracleSystem, lattice_state
 pytestmark = pytest.mark.gpu
 
 
-def _pair(N, rho, T0, canon, bc, pos, vel):
-    g = ljamd.LJSystem(N, rho, T0, canonical=canon, boundary=bc)
+def _pair(N, rho, T0, canon, bc, pos, vel, precision=ljamd.PREC_FP32):
+    g = ljamd.LJSystem(N, rho, T0, canonical=canon, boundary=bc, precision=precision)
     g.set_state(pos, vel)
     g.CalculateForces()
     g.CalculateParameters()
@@ -154,14 +154,16 @@ def test_renormalize_and_modes():
             s.set_mode(3, 0)
 
 
-@pytest.mark.parametrize("canon", [ljamd.EVN, ljamd.TVN])
-def test_step_totals_of_pair_once_pass(canon):
+@pytest.mark.parametrize("canon,prec", [(ljamd.EVN, ljamd.PREC_FP32), (ljamd.TVN, ljamd.PREC_FP32),
+                                        (ljamd.TVN, ljamd.PREC_COMPENSATED)])
+def test_step_totals_of_pair_once_pass(canon, prec):
     """At N >= 3072 the step's force pass sums E and W per workgroup (no
-    per-particle energies).  Its V and W must equal a full force evaluation
-    at the same positions; Force.w reads 0 after the step, as documented."""
+    per-particle energies), in the FP32 and the compensated mode.  Its V and
+    W must equal a full force evaluation at the same positions; Force.w reads
+    0 after the step, as documented."""
     N, rho, T0, dt = 16384, 0.85, 1.0, 0.005
     L, pos, vel = lattice_state(N, rho, T0, seed=5)
-    with ljamd.LJSystem(N, rho, T0, canonical=canon) as s:
+    with ljamd.LJSystem(N, rho, T0, canonical=canon, precision=prec) as s:
         assert s.force_plan()[0]
         s.set_state(pos, vel)
         s.CalculateForces()
@@ -176,3 +178,28 @@ def test_step_totals_of_pair_once_pass(canon):
     assert np.all(f_step[:, 3] == 0.0) and np.any(f_full[:, 3] != 0.0)
     d = np.linalg.norm(f_step[:, :3] - f_full[:, :3]) / np.linalg.norm(f_full[:, :3])
     assert d <= 1e-6
+
+
+@pytest.mark.parametrize("N,canon,bc,nsteps,tolU", [
+    (10000, ljamd.TVN, ljamd.BC_PERIODIC, 10, 2e-6),
+    (13001, ljamd.EVN, ljamd.BC_WALL, 10, 2e-6),
+])
+def test_compensated_trajectory_vs_reference(N, canon, bc, nsteps, tolU):
+    """The compensated mode's time step (pair-once pass with compensated sums,
+    partial last groups) against the reference CPU path, with the
+    tolerances of the FP32 mode's trajectory test at these sizes."""
+    rho, T0 = 0.85, 1.0
+    L, pos, vel = lattice_state(N, rho, T0, seed=77)
+    print(f"\n[case] compensated traj N={N} canon={canon} bc={bc}", flush=True)
+    g, o = _pair(N, rho, T0, canon, bc, pos, vel, precision=ljamd.PREC_COMPENSATED)
+    assert g.force_plan()[0]
+    try:
+        hg = g.Integrate(0.005, nsteps)
+        ho = o.integrate(0.005, nsteps)
+        dU = np.abs(hg[:, 0] - ho[:, 0]) / N
+        dT = np.abs(hg[:, 1] - ho[:, 1])
+        print(f"  max |dU|/N {dU.max():.2e}, max |dT| {dT.max():.2e}")
+        assert dU.max() <= tolU and dT.max() <= 10 * tolU
+    finally:
+        g.close()
+
