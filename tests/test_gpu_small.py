@@ -88,20 +88,21 @@ def pair_once_env():
         os.environ["LJ_PAIR_ONCE"] = old
 
 
+@pytest.mark.parametrize("prec", [ljamd.PREC_FP32, ljamd.PREC_COMPENSATED])
 @pytest.mark.parametrize("N,bc", [(256, ljamd.BC_PERIODIC), (257, ljamd.BC_WALL),
                                   (700, ljamd.BC_PERIODIC)])
-def test_pair_once_forced_small(N, bc, pair_once_env):
-    """LJ_PAIR_ONCE=1 at small N: one group (the diagonal item alone), a last
-    group of a single particle, three groups; forces against FP64, and a
-    20-step TVN run (the step variant, fused 8-step graphs) against the sliced
-    kernel's (LJ_PAIR_ONCE=0) from the same state."""
+def test_pair_once_forced_small(N, bc, prec, pair_once_env):
+    """LJ_PAIR_ONCE=1 at small N, in the FP32 and the compensated mode: one
+    group (the diagonal item alone), a last group of a single particle, three
+    groups; forces against FP64, and a 20-step TVN run (the step variant)
+    against the sliced kernel's (LJ_PAIR_ONCE=0) from the same state."""
     rho = 0.6
     L, pos, vel = perturbed_state(N, rho, 1.0, seed=43)
-    print(f"\n[case] pair-once forced N={N} bc={bc}", flush=True)
+    print(f"\n[case] pair-once forced N={N} bc={bc} prec={prec}", flush=True)
     runs = {}
     for flag in ("1", "0"):
         os.environ["LJ_PAIR_ONCE"] = flag
-        with ljamd.LJSystem(N, rho, 1.0, canonical=ljamd.TVN, boundary=bc) as s:
+        with ljamd.LJSystem(N, rho, 1.0, canonical=ljamd.TVN, boundary=bc, precision=prec) as s:
             assert s.force_plan()[0] == (flag == "1")
             s.set_state(pos, vel)
             obs = s.CalculateForces()
