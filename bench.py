@@ -450,7 +450,7 @@ def main():
         print(json.dumps({"profile": "adjoint", **adjoint_leg()}))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
+        print(f"bench.py: this library runs on one GPU only (asked for {a.gpus}); "
               "the multi-GPU sharding is a design note, not a runnable path", file=sys.stderr)
         sys.exit(2)
 

@@ -1,12 +1,12 @@
-"""Build recipe of libljhip.so (gfx950) and of the checker libraries.
+"""Build recipe of libljhip.so (gfx950) and of the test-oracle libraries.
 
-    python lennard-jones-cuda_amd/build.py            # product + checkers
+    python lennard-jones-cuda_amd/build.py            # product + test oracles
     python lennard-jones-cuda_amd/build.py --product  # libljhip.so only
 
 Outputs (in-tree, git-ignored, shipped with the tree to the GPU host):
     lennard-jones-cuda_amd/lib/libljhip.so
     lennard-jones-cuda_amd/lib/run_isotherm, run_fluctuations (task programs)
-    oracle/liboracle.so, oracle/_ref/*.so  (checkers; _ref only where the
+    oracle/liboracle.so, oracle/_ref/*.so  (test oracles; _ref only where the
                                             reference sources are present)
 """
 import argparse
@@ -108,7 +108,7 @@ def build_valu_rates():
     return exe
 
 
-def build_checkers():
+def build_oracles():
     _run(["make", "-s", "-C", ORACLE, "oracle"])
     if os.path.isdir(os.path.join(REFERENCE, "src", "library")):
         _run(["make", "-s", "-C", ORACLE, "ref", f"REF={REFERENCE}"])
@@ -120,7 +120,7 @@ def build_all(product_only=False, verbose=False):
     build_tasks()
     build_valu_rates()
     if not product_only:
-        build_checkers()
+        build_oracles()
     return t
 
 

@@ -5,7 +5,7 @@
 // Parameter files (src/tasks/run-*/include/run-*-aux.h, ReadParametersFromFile):
 // whitespace-separated `name value` pairs; a token starting with '#' makes the
 // rest of its line a comment; `output_prefix` takes a word, every other name a
-// number.  Unknown names are kept, so files may carry this build's extra
+// number.  Unknown names are kept, so files may carry these programs' extra
 // parameters (seed, timestamp, print_every, batched).
 #pragma once
 
@@ -209,14 +209,14 @@ inline void check(int rc, const char* what)
 }
 
 // the reference checks for a device and falls back to its CPU path
-// (run-*.cpp:20-31); this build has no CPU path, so no device is an error,
+// (run-*.cpp:20-31); this library has no CPU path, so no device is an error,
 // and `useCUDA 0` still runs on the device
 inline void require_device(Parameters& p)
 {
   if (lj_device_count() == 0)
-    die("no HIP device found; this build has no CPU path (the reference's would run on the CPU)");
+    die("no HIP device found; this library has no CPU path (the reference's would run on the CPU)");
   if (p.has("useCUDA") && p.iget("useCUDA") == 0)
-    std::cout << "useCUDA 0: this build runs on the GPU only; running on the GPU\n";
+    std::cout << "useCUDA 0: this library runs on the GPU only; running on the GPU\n";
 }
 
 }  // namespace ljtask

@@ -1,7 +1,7 @@
 /*
  * lj_oracle.c -- CPU restatement of the reference Lennard-Jones time step.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Not linked into the product.
+ * TEST INFRASTRUCTURE ONLY (the test oracle).  Not linked into the product.
  * Reference: vlvovch/lennard-jones-cuda, src/library/MDSystem.cpp (CPU path)
  * and src/library/MDSystem.cu (GPU-variant semantics for the RDF rule).
  * Build: -O2 -ffp-contract=off, no -ffast-math (the reference's x86-64 build

@@ -1,7 +1,7 @@
 /*
  * lj_oracle.h -- CPU restatement of the reference Lennard-Jones time step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY.  This library is the test oracle of the HIP path:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it, and only to produce expected values or a timed CPU baseline.  The
  * product library (lennard-jones-cuda_amd/) never links, loads or calls it.

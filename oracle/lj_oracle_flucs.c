@@ -2,7 +2,7 @@
  * lj_oracle_flucs.c -- CPU restatement of the reference's fluctuation
  * observables and time averages.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Not linked into the product.
+ * TEST INFRASTRUCTURE ONLY (the test oracle).  Not linked into the product.
  * Reference: vlvovch/lennard-jones-cuda
  *   src/tasks/run-fluctuations/include/run-fluctuations-aux.h:128-278
  *     GetNSubsystem, GetNsubVz, GetNSubsystemBatch, GetNsubVzBatch

@@ -4,7 +4,7 @@
 // where they lie (src/library/MDSystem.cpp, splinefunction.cpp of
 // vlvovch/lennard-jones-cuda) by oracle/Makefile into oracle/_ref/:
 //   libref_cpu.so  -- reference built without USE_CUDA_TOOLKIT: its CPU path
-//                     (MDSystem.cpp:252-310) is the checker and CPU baseline;
+//                     (MDSystem.cpp:252-310) is the test reference and CPU baseline;
 //   libref_hip.so  -- reference built with USE_CUDA_TOOLKIT and linked
 //                     against this project's libljhip.so, so that the
 //                     reference's own CPU/GPU switch (MDSystem.cpp:240) drives

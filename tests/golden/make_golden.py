@@ -2,7 +2,7 @@
 reference's own CPU path (MDSystem compiled from its sources by
 oracle/Makefile into oracle/_ref/libref_cpu.so) run on seeded states, its
 outputs saved as data.  Run from the repository root after building the
-checker libraries (`python -c "import __graft_entry__ as g; g.build()"`):
+test-oracle libraries (`python -c "import __graft_entry__ as g; g.build()"`):
 
     python tests/golden/make_golden.py
 

@@ -1,4 +1,4 @@
-"""ctypes bindings to the checker libraries under oracle/ (test infrastructure).
+"""ctypes bindings to the test-oracle libraries under oracle/ (test infrastructure).
 
 liboracle.so is the C restatement of the reference CPU path; oracle/_ref holds
 the reference's own MDSystem class compiled from its sources (when it was
