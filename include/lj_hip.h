@@ -249,6 +249,14 @@ int lj_time_rdf(lj_ctx* ctx, int rule, int nbins, float dr2, double rmax, int re
  * b < nbins (the reference's maxind = (int)(vmax / step) + 1); nbins <= 16384. */
 int lj_velocity_histogram(lj_ctx* ctx, double step, int nbins, uint64_t* counts);
 
+/* Shear stress of the current state as the reference's CPU path computes it
+ * (MDSystem.cpp:299,309,335,353): (sum over ordered pairs of -r_x f_y / 2 +
+ * sum_i fl32(-v_x v_y)) / (N / rho), the pair terms in the force pass's FP32
+ * arithmetic with FP64 sums.  The reference's GPU branch does not compute it,
+ * and neither does the time step (lj_obs.Pshear stays 0); this runs one
+ * all-pairs pass on demand.  FP64 contexts use the float mirrors of the state. */
+int lj_shear_pressure(lj_ctx* ctx, double* pshear);
+
 int lj_renormalize_velocities(lj_ctx* ctx, int recompute_tkin); /* MDSystem.cpp:375-389 */
 int lj_renormalize_to_energy(lj_ctx* ctx, double ust);          /* MDSystem.cpp:391-404 */
 int lj_reset_averaging(lj_ctx* ctx);                            /* MDSystem.cpp:701-705 */

@@ -29,6 +29,8 @@
  *   - at N >= 3072 (FP32) h_Force[4 i + 3] is 0 after Integrate (the step sums
  *     the potential energy per workgroup; V, P are unaffected) and is
  *     refreshed by CalculateForces();
+ *   - Pshear stays 0, as on the reference's GPU branch; UpdateShearPressure()
+ *     sets it to the CPU path's value for the current state;
  *   - every failure throws std::runtime_error with lj_last_error().
  *   Fluctuations (MDSystem.cpp:708-730) counts the sub-volume on the device
  *   (lj_fl_*) and keeps the reference's moment sums and printed line.
@@ -293,6 +295,16 @@ class MDSystem {
     check(lj_reset_averaging(ctx_));
     av_U_tot = av_T_tot = av_p_tot = 0.;
     av_iters = 0;
+  }
+
+  // Pshear as the reference's CPU path leaves it after CalculateForces and
+  // CalculateParameters (MDSystem.cpp:299,309,335,353), for the current state.
+  // The GPU branch, which this class follows, never computes it (Pshear stays
+  // 0 after Integrate and CalculateParameters); this runs one all-pairs pass.
+  double UpdateShearPressure()
+  {
+    check(lj_shear_pressure(ctx_, &Pshear));
+    return Pshear;
   }
 
   // NdNdr2 of the current positions (256 r^2 bins of width rdf_dr2)

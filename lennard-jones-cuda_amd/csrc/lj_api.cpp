@@ -363,6 +363,7 @@ int lj_destroy(lj_ctx* c)
   dev_free(c->n3items);
   dev_free(c->step_ew);
   dev_free(c->vh);
+  dev_free(c->shear);
   dev_free(c->xd);
   dev_free(c->vd);
   dev_free(c->fd);
@@ -774,6 +775,29 @@ int lj_velocity_histogram(lj_ctx* c, double step, int nbins, uint64_t* counts)
   LJ_HIP(launch_fill_u64(st, c->vh, nbins));
   LJ_HIP(launch_velocity_hist(st, c->vel, c->cfg.N, step, nbins, c->vh));
   LJ_HIP(hipMemcpyAsync(counts, c->vh, sizeof(uint64_t) * nbins, hipMemcpyDeviceToHost, st));
+  LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_shear_pressure(lj_ctx* c, double* pshear)
+{
+  LJ_TRY(ctx_setdev(c));
+  if (!pshear) return fail(LJ_ERR_ARG, "lj_shear_pressure: null output");
+  hipStream_t st = c->stream;
+  const ForcePlan& pl = c->plan;  // the sliced grid, whatever kernel the steps run
+  const int n = pl.gx * pl.S + 1;
+  if (n > c->shear_cap) {
+    LJ_HIP(hipStreamSynchronize(st));
+    dev_free(c->shear);
+    c->shear_cap = 0;
+    LJ_TRY(dev_alloc(&c->shear, n));
+    c->shear_cap = n;
+  }
+  // FP64 contexts: the float mirrors of the state
+  if (c->ps.periodic) LJ_HIP(launch_prep_q(st, c->pos, c->q, c->cfg.N, c->ps.qscale));
+  LJ_HIP(launch_shear(st, pl, c->ps, c->q, c->pos, c->vel, c->cfg.N / c->cfg.rho, c->shear,
+                      c->shear + n - 1));
+  LJ_HIP(hipMemcpyAsync(pshear, c->shear + n - 1, sizeof(double), hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
   return LJ_OK;
 }

@@ -40,6 +40,8 @@ struct lj_ctx {
   int edges_cap = 0;
   unsigned long long* vh = nullptr;  // speed histogram counts
   int vh_cap = 0;
+  double* shear = nullptr;  // shear-stress workgroup parts, then the result
+  int shear_cap = 0;
   hipGraph_t graph = nullptr;        // one step
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph_run = nullptr;    // kGraphSteps steps back to back

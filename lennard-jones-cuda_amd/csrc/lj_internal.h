@@ -218,6 +218,11 @@ int rdf_cells_grid(const RdfArgs& a, int* reach);
 bool rdf_uses_cells(const RdfArgs& a);
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, int n);
 // speed histogram (MDSystem.cpp:651-694); hist is added to (clear it first)
+// shear stress of the reference CPU path (MDSystem.cpp:299,309,335,353) over
+// the sliced grid of pl; part: pl.gx * pl.S doubles; out: one double
+hipError_t launch_shear(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                        const uint4* q, const float4* pos, const float4* vel, double volume,
+                        double* part, double* out);
 hipError_t launch_velocity_hist(hipStream_t st, const float4* vel, int N, double step, int nbins,
                                 unsigned long long* hist);
 

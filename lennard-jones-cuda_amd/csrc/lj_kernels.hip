@@ -780,6 +780,75 @@ __global__ __launch_bounds__(kBlock) void k_velocity_hist(const float4* __restri
     if (sh_vh[b]) atomicAdd(&hist[b], (unsigned long long)sh_vh[b]);
 }
 
+// Shear stress of the reference CPU path.  CalculateForces sums -r_x f_y over
+// the ordered pairs (MDSystem.cpp:299) and scales the sum by 4/2 (:309);
+// CalculateParameters adds -v_x v_y per particle (:335) and divides by the
+// volume N / rho (:353).  The pair sum takes the force pass's FP32 pair
+// arithmetic (the fixed-point minimum image in periodic boxes) over the
+// sliced grid (i blocks x j slices), in FP32 blocks of kJB j added in FP64:
+// part[workgroup] = sum of -dx dy s over its pairs (internal units).  j is the
+// same for every lane (scalar loads).
+template <bool PERIODIC>
+__global__ __launch_bounds__(kBlock) void k_shear_pairs(const uint4* __restrict__ q,
+                                                        const float4* __restrict__ pf, int N,
+                                                        int jchunk, float C1,
+                                                        double* __restrict__ part)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int j0 = blockIdx.y * jchunk, j1 = min(N, j0 + jchunk);
+  const PosT<PERIODIC> pi = load_pos<PERIODIC>(q, pf, min(i, N - 1));
+  double v[1] = {0.};
+  for (int b0 = j0; b0 < j1; b0 += kJB) {
+    const int b1 = min(j1, b0 + kJB);
+    float t = 0.f;
+    for (int j = b0; j < b1; ++j) {
+      const PosT<PERIODIC> pj = load_pos<PERIODIC>(q, pf, j);
+      float dx, dy, dz;
+      if constexpr (PERIODIC) {
+        dx = mi_fixed<true>(pi.x, pj.x);
+        dy = mi_fixed<true>(pi.y, pj.y);
+        dz = mi_fixed<true>(pi.z, pj.z);
+      } else {
+        dx = pi.x - pj.x;
+        dy = pi.y - pj.y;
+        dz = pi.z - pj.z;
+      }
+      const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+      float ir2 = __builtin_amdgcn_rcpf(r2) * C1;
+      ir2 = j == i ? 0.f : ir2;  // the reference's j != i
+      const float e6 = ir2 * ir2 * ir2;
+      const float s = e6 * __builtin_fmaf(e6, 2.f, -1.f) * ir2;
+      t = __builtin_fmaf(-dx * dy, s, t);
+    }
+    v[0] += (double)t;
+  }
+  if (i >= N) v[0] = 0.;
+  const int fld[1] = {0};
+  block_sum_store<1>(v, part + (size_t)blockIdx.y * gridDim.x + blockIdx.x, fld);
+}
+
+// out = (scale * sum of the pair parts + sum_i fl32(-v_x v_y)) / volume, both
+// sums in a fixed order; scale = 1/2 (the ordered pairs' -r_x f_y, times 4/2,
+// with the reference's force at a quarter of f) times the force pass's
+// internal-unit factor
+__global__ __launch_bounds__(kBlock) void k_shear_finalize(const double* __restrict__ part,
+                                                           int nparts,
+                                                           const float4* __restrict__ vel, int N,
+                                                           double scale, double volume,
+                                                           double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+  double v[2] = {0., 0.};
+  for (int r = threadIdx.x; r < nparts; r += kBlock) v[0] += part[r];
+  for (int i = threadIdx.x; i < N; i += kBlock) {
+    const float4 u = vel[i];
+    const float ps = -u.x * u.y;  // MDSystem.cpp:335
+    v[1] += (double)ps;
+  }
+  block_allsum<2>(v);
+  if (threadIdx.x == 0) *out = (scale * v[0] + v[1]) / volume;
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int n)
 {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1103,6 +1172,24 @@ hipError_t launch_velocity_hist(hipStream_t st, const float4* vel, int N, double
   const int g = max(1, min(nblocks(N), 512));
   hipLaunchKernelGGL(k_velocity_hist, dim3(g), dim3(kBlock), sizeof(unsigned) * nbins, st, vel, N,
                      step, nbins, hist);
+  return hipGetLastError();
+}
+
+hipError_t launch_shear(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                        const uint4* q, const float4* pos, const float4* vel, double volume,
+                        double* part, double* out)
+{
+  const dim3 grid(pl.gx, pl.S);
+  if (ps.periodic)
+    hipLaunchKernelGGL(k_shear_pairs<true>, grid, dim3(kBlock), 0, st, q, pos, pl.N, pl.jchunk,
+                       ps.C1, part);
+  else
+    hipLaunchKernelGGL(k_shear_pairs<false>, grid, dim3(kBlock), 0, st, q, pos, pl.N, pl.jchunk,
+                       ps.C1, part);
+  // -r_x f_y = -(u dx) (FS s dy): u = L / 2^32 in periodic boxes, 1 otherwise
+  const double u = ps.periodic ? ps.L / 4294967296.0 : 1.0;
+  hipLaunchKernelGGL(k_shear_finalize, dim3(1), dim3(kBlock), 0, st, part, pl.gx * pl.S, vel, pl.N,
+                     0.5 * u * ps.FS, volume, out);
   return hipGetLastError();
 }
 

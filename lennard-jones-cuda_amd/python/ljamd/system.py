@@ -182,6 +182,13 @@ class LJSystem:
         check(lib().lj_velocity_histogram(self.h, float(step), int(nbins), ptr(out)))
         return out
 
+    def shear_pressure(self):
+        """Pshear of the current state as the reference CPU path computes it
+        (MDSystem.cpp:299,309,335,353); one all-pairs pass."""
+        out = C.c_double()
+        check(lib().lj_shear_pressure(self.h, C.byref(out)))
+        return out.value
+
     def force_plan(self):
         """(pair_once, nslots) of the force kernel this context runs."""
         p, n = C.c_int(), C.c_int()

@@ -22,7 +22,7 @@ import pytest
 
 import ljamd
 from helpers import force_errors, perturbed_state, rel, xyz
-from oracle_lib import ORC, forces_fp64, forces_ref, lattice_state
+from oracle_lib import ORC, OracleSystem, forces_fp64, forces_ref, lattice_state
 
 pytestmark = pytest.mark.gpu
 
@@ -231,4 +231,58 @@ def test_compensated_pair_once(N, bc):
     assert ec["global_rel"] <= 2e-6 and ec["max_rel_floor"] <= 1e-5
     assert ec["global_rel"] <= e32["global_rel"]
     assert rel(obs["V"], E) <= 1e-6 and rel(obs["W"], W) <= 1e-5 and ew_err <= 1e-5
+
+
+def _shear_fp64(pos, vel, L, periodic, rows=256):
+    """Pshear of the state in FP64 (eps = sigma = 1): (sum over i < j of
+    -r_x F_y + sum_i -v_x v_y) / L^3, and the same with every term's
+    magnitude (the scale of the tolerance: the stress averages to zero)."""
+    x = xyz(pos)
+    v = xyz(vel)
+    N = x.shape[0]
+    tot, mag = 0.0, 0.0
+    for i0 in range(0, N, rows):
+        d = x[i0:i0 + rows, None, :] - x[None, :, :]
+        if periodic:
+            d -= L * np.round(d / L)
+        r2 = np.sum(d * d, axis=2)
+        keep = np.arange(N)[None, :] > np.arange(i0, min(N, i0 + rows))[:, None]
+        r2 = np.where(keep, r2, 1.0)
+        ir2 = 1.0 / r2
+        ir6 = ir2 ** 3
+        fac = np.where(keep, 4.0 * ir2 * (12.0 * ir6 * ir6 - 6.0 * ir6), 0.0)
+        t = -d[:, :, 0] * fac * d[:, :, 1]
+        tot += float(np.sum(t))
+        mag += float(np.sum(np.abs(t)))
+    kin = -v[:, 0] * v[:, 1]
+    V = L ** 3
+    return (tot + float(np.sum(kin))) / V, (mag + float(np.sum(np.abs(kin)))) / V
+
+
+@pytest.mark.parametrize("N,rho,bc", [(2048, 0.85, ljamd.BC_PERIODIC), (2000, 0.3, ljamd.BC_WALL),
+                                      (1500, 0.2, ljamd.BC_NONE), (16384, 0.85, ljamd.BC_PERIODIC)])
+def test_shear_pressure_vs_reference_cpu(N, rho, bc):
+    """Pshear as the reference CPU path leaves it after CalculateForces and
+    CalculateParameters (MDSystem.cpp:299,309,335,353), on demand: against the
+    oracle's restatement (bit-exact to the reference) and an FP64 sum, within
+    1e-7 of the sum of the terms' magnitudes (measured on MI355X: at most
+    8.4e-9 from the oracle; the oracle 1.1e-9 from the FP64 sum).  The
+    observables keep Pshear = 0, as the reference's GPU branch does."""
+    L, pos, vel = perturbed_state(N, rho, 1.0, seed=29)
+    print(f"\n[case] shear pressure N={N} rho={rho} bc={bc}", flush=True)
+    with ljamd.LJSystem(N, rho, 1.0, boundary=bc) as s:
+        s.set_state(pos, vel)
+        ps = s.shear_pressure()
+        obs = s.CalculateParameters()
+    o = OracleSystem(N, 1.0, rho, ljamd.EVN, bc, nthreads=16)
+    o.set_state(pos, vel)
+    o.forces()
+    o.params()
+    ref = o.s.Pshear
+    p64, scale = _shear_fp64(pos, vel, L, bc == ljamd.BC_PERIODIC)
+    print(f"  GPU {ps:.10g}  oracle {ref:.10g}  FP64 {p64:.10g}  scale {scale:.4g}  "
+          f"|GPU - oracle| / scale {abs(ps - ref) / scale:.2e}")
+    assert abs(ps - ref) <= 1e-7 * scale and abs(ps - p64) <= 1e-7 * scale
+    assert abs(ref - p64) <= 1e-7 * scale
+    assert obs["Pshear"] == 0.0
 
