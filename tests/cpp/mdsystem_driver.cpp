@@ -55,6 +55,9 @@ int main(int argc, char** argv)
   try {
     ljhip::MDSystem s(cfg);
     std::printf("init %.17g %.17g %.17g %.17g %.17g\n", s.U, s.T, s.K, s.V, s.P);
+    // Pshear: 0 as on the reference's GPU branch, then the CPU path's value
+    const double ps_gpu_branch = s.Pshear;
+    std::printf("pshear %.17g %.17g\n", ps_gpu_branch, s.UpdateShearPressure());
     for (int k = 0; k < steps; ++k) {
       s.Integrate(dt);
       std::printf("step %d %.17g %.17g %.17g %.17g %.17g %.17g\n", k, s.t, s.U, s.T, s.K, s.V, s.P);

@@ -36,3 +36,29 @@ def force_errors(F, Fref):
 
 def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
+
+
+def shear_fp64(pos, vel, L, periodic, rows=256):
+    """Pshear of the state in FP64 (eps = sigma = 1): (sum over i < j of
+    -r_x F_y + sum_i -v_x v_y) / L^3, and the same with every term's
+    magnitude (the scale of the tolerance: the stress averages to zero)."""
+    x = xyz(pos)
+    v = xyz(vel)
+    N = x.shape[0]
+    tot, mag = 0.0, 0.0
+    for i0 in range(0, N, rows):
+        d = x[i0:i0 + rows, None, :] - x[None, :, :]
+        if periodic:
+            d -= L * np.round(d / L)
+        r2 = np.sum(d * d, axis=2)
+        keep = np.arange(N)[None, :] > np.arange(i0, min(N, i0 + rows))[:, None]
+        r2 = np.where(keep, r2, 1.0)
+        ir2 = 1.0 / r2
+        ir6 = ir2 ** 3
+        fac = np.where(keep, 4.0 * ir2 * (12.0 * ir6 * ir6 - 6.0 * ir6), 0.0)
+        t = -d[:, :, 0] * fac * d[:, :, 1]
+        tot += float(np.sum(t))
+        mag += float(np.sum(np.abs(t)))
+    kin = -v[:, 0] * v[:, 1]
+    V = L ** 3
+    return (tot + float(np.sum(kin))) / V, (mag + float(np.sum(np.abs(kin)))) / V

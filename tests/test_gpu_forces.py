@@ -21,7 +21,7 @@ import numpy as np
 import pytest
 
 import ljamd
-from helpers import force_errors, perturbed_state, rel, xyz
+from helpers import force_errors, perturbed_state, rel, shear_fp64, xyz
 from oracle_lib import ORC, OracleSystem, forces_fp64, forces_ref, lattice_state
 
 pytestmark = pytest.mark.gpu
@@ -233,32 +233,6 @@ def test_compensated_pair_once(N, bc):
     assert rel(obs["V"], E) <= 1e-6 and rel(obs["W"], W) <= 1e-5 and ew_err <= 1e-5
 
 
-def _shear_fp64(pos, vel, L, periodic, rows=256):
-    """Pshear of the state in FP64 (eps = sigma = 1): (sum over i < j of
-    -r_x F_y + sum_i -v_x v_y) / L^3, and the same with every term's
-    magnitude (the scale of the tolerance: the stress averages to zero)."""
-    x = xyz(pos)
-    v = xyz(vel)
-    N = x.shape[0]
-    tot, mag = 0.0, 0.0
-    for i0 in range(0, N, rows):
-        d = x[i0:i0 + rows, None, :] - x[None, :, :]
-        if periodic:
-            d -= L * np.round(d / L)
-        r2 = np.sum(d * d, axis=2)
-        keep = np.arange(N)[None, :] > np.arange(i0, min(N, i0 + rows))[:, None]
-        r2 = np.where(keep, r2, 1.0)
-        ir2 = 1.0 / r2
-        ir6 = ir2 ** 3
-        fac = np.where(keep, 4.0 * ir2 * (12.0 * ir6 * ir6 - 6.0 * ir6), 0.0)
-        t = -d[:, :, 0] * fac * d[:, :, 1]
-        tot += float(np.sum(t))
-        mag += float(np.sum(np.abs(t)))
-    kin = -v[:, 0] * v[:, 1]
-    V = L ** 3
-    return (tot + float(np.sum(kin))) / V, (mag + float(np.sum(np.abs(kin)))) / V
-
-
 @pytest.mark.parametrize("N,rho,bc", [(2048, 0.85, ljamd.BC_PERIODIC), (2000, 0.3, ljamd.BC_WALL),
                                       (1500, 0.2, ljamd.BC_NONE), (16384, 0.85, ljamd.BC_PERIODIC)])
 def test_shear_pressure_vs_reference_cpu(N, rho, bc):
@@ -279,7 +253,7 @@ def test_shear_pressure_vs_reference_cpu(N, rho, bc):
     o.forces()
     o.params()
     ref = o.s.Pshear
-    p64, scale = _shear_fp64(pos, vel, L, bc == ljamd.BC_PERIODIC)
+    p64, scale = shear_fp64(pos, vel, L, bc == ljamd.BC_PERIODIC)
     print(f"  GPU {ps:.10g}  oracle {ref:.10g}  FP64 {p64:.10g}  scale {scale:.4g}  "
           f"|GPU - oracle| / scale {abs(ps - ref) / scale:.2e}")
     assert abs(ps - ref) <= 1e-7 * scale and abs(ps - p64) <= 1e-7 * scale

@@ -12,6 +12,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from helpers import shear_fp64
 from oracle_lib import ORC, OracleSystem, lattice_state, rdf, velocity_histogram
 
 pytestmark = pytest.mark.gpu
@@ -50,6 +51,12 @@ def test_mdsystem_class_vs_reference_cpu():
     o.params()
     U0, T0g, K0, V0, P0 = out["init"]
     assert abs(U0 - o.s.U) <= 1e-6 * abs(o.s.U) and abs(V0 - o.s.V) <= 1e-5 * abs(o.s.V)
+    # Pshear: 0 as on the reference's GPU branch; UpdateShearPressure gives the
+    # CPU path's value (MDSystem.cpp:299,309,335,353)
+    ps0, ps = out["pshear"]
+    _, scale = shear_fp64(pos, vel, L, True)
+    print(f"  Pshear {ps:.10g} vs oracle {o.s.Pshear:.10g} (scale {scale:.3g})")
+    assert ps0 == 0.0 and abs(ps - o.s.Pshear) <= 1e-7 * scale
     ho = o.integrate(dt, steps)
     st = np.array(out["step"])
     assert np.array_equal(st[:, 0], np.arange(steps))
