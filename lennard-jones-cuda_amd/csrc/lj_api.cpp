@@ -364,6 +364,10 @@ int lj_destroy(lj_ctx* c)
   dev_free(c->step_ew);
   dev_free(c->vh);
   dev_free(c->shear);
+  dev_free(c->sh_fpart);
+  dev_free(c->sh_apart);
+  dev_free(c->sh_blk);
+  dev_free(c->sh_ev);
   dev_free(c->xd);
   dev_free(c->vd);
   dev_free(c->fd);
@@ -776,6 +780,79 @@ int lj_velocity_histogram(lj_ctx* c, double step, int nbins, uint64_t* counts)
   LJ_HIP(launch_velocity_hist(st, c->vel, c->cfg.N, step, nbins, c->vh));
   LJ_HIP(hipMemcpyAsync(counts, c->vh, sizeof(uint64_t) * nbins, hipMemcpyDeviceToHost, st));
   LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+namespace {
+// the plan and workspace of one shard's force pass
+int shard_setup(lj_ctx* c, int shard, int nshards, int jslices, const char* fn, ForcePlan* out)
+{
+  LJ_TRY(ctx_setdev(c));
+  const int N = c->cfg.N;
+  if (nshards < 1 || nshards > (N + 63) / 64 || shard < 0 || shard >= nshards)
+    return fail(LJ_ERR_ARG, "%s: need 0 <= shard < nshards <= ceil(N / 64) (got %d of %d)", fn,
+                shard, nshards);
+  if (jslices < 0 || jslices > N) return fail(LJ_ERR_ARG, "%s: jslices must be in [0, N]", fn);
+  if (is_fp64(c)) return fail(LJ_ERR_STATE, "%s: FP32 and compensated modes only", fn);
+  const ForcePlan pl = make_shard_plan(N, shard, nshards, jslices);
+  const size_t np = (size_t)pl.S * pl.Npad;
+  const int nb = (pl.iend() - pl.i0 + kBlock - 1) / kBlock;
+  if (np > c->sh_cap || nb > c->sh_blk_cap || !c->sh_ev) {
+    // grown to at least the old capacities (shares of other sizes come next)
+    const size_t cap = std::max(np, c->sh_cap);
+    const int bcap = std::max(nb, c->sh_blk_cap);
+    LJ_HIP(hipStreamSynchronize(c->stream));
+    dev_free(c->sh_fpart);
+    dev_free(c->sh_apart);
+    dev_free(c->sh_blk);
+    dev_free(c->sh_ev);
+    c->sh_cap = 0;
+    c->sh_blk_cap = 0;
+    LJ_TRY(dev_alloc(&c->sh_fpart, cap));
+    LJ_TRY(dev_alloc(&c->sh_apart, cap));
+    LJ_TRY(dev_alloc(&c->sh_blk, (size_t)bcap * kBlkFields));
+    LJ_TRY(dev_alloc(&c->sh_ev, 2));
+    c->sh_cap = cap;
+    c->sh_blk_cap = bcap;
+  }
+  if (c->ps.periodic) LJ_HIP(launch_prep_q(c->stream, c->pos, c->q, N, c->ps.qscale));
+  *out = pl;
+  return LJ_OK;
+}
+}  // namespace
+
+int lj_forces_shard(lj_ctx* c, int shard, int nshards, int jslices, float* d_force4, double* ev)
+{
+  ForcePlan pl;
+  LJ_TRY(shard_setup(c, shard, nshards, jslices, "lj_forces_shard", &pl));
+  hipStream_t st = c->stream;
+  const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
+  LJ_HIP(launch_force(st, pl, c->ps, c->q, c->pos, c->sh_fpart, c->sh_apart, comp, true));
+  float4* out = d_force4 ? reinterpret_cast<float4*>(d_force4) : c->force;
+  LJ_HIP(launch_reduce_forces(st, pl, c->ps, c->sh_fpart, c->sh_apart, out, c->sh_blk));
+  LJ_HIP(launch_finalize_forces(st, c->sh_blk, (pl.iend() - pl.i0 + kBlock - 1) / kBlock,
+                                c->cfg.eps, c->sh_ev, nullptr));
+  if (ev) LJ_HIP(hipMemcpyAsync(ev, c->sh_ev, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  LJ_HIP(hipStreamSynchronize(st));
+  return LJ_OK;
+}
+
+int lj_time_forces_shard(lj_ctx* c, int shard, int nshards, int jslices, int reps, double* us)
+{
+  if (reps <= 0 || !us) return fail(LJ_ERR_ARG, "lj_time_forces_shard: bad arguments");
+  ForcePlan pl;
+  LJ_TRY(shard_setup(c, shard, nshards, jslices, "lj_time_forces_shard", &pl));
+  hipStream_t st = c->stream;
+  const bool comp = c->cfg.precision == LJ_PREC_COMPENSATED;
+  LJ_HIP(launch_force(st, pl, c->ps, c->q, c->pos, c->sh_fpart, c->sh_apart, comp, false));  // warm
+  LJ_HIP(hipEventRecord(c->e0, st));
+  for (int k = 0; k < reps; ++k)
+    LJ_HIP(launch_force(st, pl, c->ps, c->q, c->pos, c->sh_fpart, c->sh_apart, comp, false));
+  LJ_HIP(hipEventRecord(c->e1, st));
+  LJ_HIP(hipEventSynchronize(c->e1));
+  float ms = 0.f;
+  LJ_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+  *us = 1000.0 * ms / reps;
   return LJ_OK;
 }
 

@@ -42,6 +42,14 @@ struct lj_ctx {
   int vh_cap = 0;
   double* shear = nullptr;  // shear-stress workgroup parts, then the result
   int shear_cap = 0;
+  // one shard's force pass (lj_forces_shard): partial entries, block rows of
+  // its reduction and its two totals
+  float4* sh_fpart = nullptr;
+  float* sh_apart = nullptr;
+  double* sh_blk = nullptr;
+  double* sh_ev = nullptr;
+  size_t sh_cap = 0;
+  int sh_blk_cap = 0;
   hipGraph_t graph = nullptr;        // one step
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph_run = nullptr;    // kGraphSteps steps back to back

@@ -34,6 +34,10 @@ struct ForcePlan {
   int Npad = 0;     // row pitch of the partial buffers
   int ipl = 1;      // i per lane
   int gx = 0;       // workgroups along i
+  // i rows [i0, iend()): all N (i1 = 0), or one shard's (make_shard_plan)
+  int i0 = 0, i1 = 0;
+  int iend() const { return i1 > 0 ? i1 : N; }
+  bool sharded() const { return i0 != 0 || iend() != N; }
   // pair-once kernel (lj_force_n3.hip): S = ng groups, one workgroup per item
   int n3 = 0;
   int ng = 0;
@@ -54,6 +58,11 @@ struct ForcePlan {
   int step_wgs() const { return nitems_step * split; }
 };
 ForcePlan make_force_plan(int N, int jslices_override, int ipl);
+// one GPU's share of a force pass sharded by i over nshards <= ceil(N / 64)
+// GPUs: rows [64 floor(shard T / nshards), 64 floor((shard + 1) T / nshards))
+// (capped at N), T = ceil(N / 64) tiles of 64, against all N on the sliced
+// grid, jslices j slices (0: enough to fill the device at that share)
+ForcePlan make_shard_plan(int N, int shard, int nshards, int jslices);
 ForcePlan make_n3_plan(int N, bool comp = false);
 // false when LJ_PAIR_ONCE_COMP=0 keeps the compensated mode on the sliced kernel
 bool n3_comp_enabled();

@@ -34,18 +34,20 @@ using namespace dev;
 using namespace rdfdev;
 
 // grid: (gx, S); block: kBlock.  fpart[s*Npad + i] = (fx, fy, fz, b), apart = a,
-// in internal units (force = FS * f).
+// in internal units (force = FS * f), for the i rows [ibase, iend): all N, or
+// one GPU's share of a pass sharded by i (lj_forces_shard).
 template <bool PERIODIC, bool GUARDALL, int IPL, bool COMP, bool TIES>
 __global__ __launch_bounds__(kBlock) void k_force(const uint4* __restrict__ q,
                                                   const float4* __restrict__ pf,
                                                   float4* __restrict__ fpart,
                                                   float* __restrict__ apart, int N, int Npad,
-                                                  int jchunk, float C1, float thr)
+                                                  int jchunk, float C1, float thr, int ibase,
+                                                  int iend)
 {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i0 = (blockIdx.x * kWaves + wv) * (64 * IPL);
-  if (i0 >= N) return;
+  const int i0 = ibase + (blockIdx.x * kWaves + wv) * (64 * IPL);
+  if (i0 >= iend) return;
   const int s = blockIdx.y;
   const int j0 = s * jchunk;
   const int j1 = min(N, j0 + jchunk);
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const uint4* __restrict__ q,
 #pragma unroll
   for (int k = 0; k < IPL; ++k) {
     ii[k] = i0 + k * 64 + lane;
-    pi[k] = load_pos<PERIODIC>(q, pf, min(ii[k], N - 1));
+    pi[k] = load_pos<PERIODIC>(q, pf, min(ii[k], iend - 1));
     acc_zero(hi[k]);
     acc_zero(lo[k]);
   }
@@ -77,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const uint4* __restrict__ q,
 
 #pragma unroll
   for (int k = 0; k < IPL; ++k) {
-    if (ii[k] < N) {
+    if (ii[k] < iend) {
       const size_t o = (size_t)s * Npad + ii[k];
       if constexpr (COMP) {
         store_part(fpart + o, hi[k].fx + lo[k].fx, hi[k].fy + lo[k].fy, hi[k].fz + lo[k].fz,
@@ -173,12 +175,12 @@ __global__ __launch_bounds__(kBlock) void k_prep_q(const float4* __restrict__ po
 __global__ __launch_bounds__(kBlock) void k_reduce_forces(const float4* __restrict__ fpart,
                                                           const float* __restrict__ apart,
                                                           float4* __restrict__ force,
-                                                          double* __restrict__ blk, int N, int S,
-                                                          int Npad, double FS)
+                                                          double* __restrict__ blk, int ibase,
+                                                          int iend, int S, int Npad, double FS)
 {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = ibase + blockIdx.x * kBlock + threadIdx.x;
   double v[2] = {0., 0.};
-  if (i < N) {
+  if (i < iend) {
     const Partial p = sum_slices(fpart, apart, i, S, Npad);
     const double e = 0.5 * (p.b - p.a);
     force[i] = make_float4((float)(FS * p.fx), (float)(FS * p.fy), (float)(FS * p.fz), (float)e);
@@ -873,7 +875,7 @@ hipError_t launch_force_t(hipStream_t st, const ForcePlan& pl, const PairScalars
                           const uint4* q, const float4* posf, float4* fpart, float* apart)
 {
   if constexpr (IPL == 1) {
-    if (force_split_waves(pl)) {
+    if (!pl.sharded() && force_split_waves(pl)) {
       // tiles padded to a multiple of the 8 XCDs: tile t is written on XCD
       // t mod 8 by every slice's workgroup, as the 64-particle row t of the
       // integrator kernels that read its partials is scheduled (round robin)
@@ -887,7 +889,7 @@ hipError_t launch_force_t(hipStream_t st, const ForcePlan& pl, const PairScalars
   dim3 grid(pl.gx, pl.S);
   hipExtLaunchKernelGGL((k_force<PERIODIC, GUARDALL, IPL, COMP, TIES>), grid, dim3(kBlock), 0, st,
                         tl_force_stamp.start, tl_force_stamp.stop, 0, q, posf, fpart, apart, pl.N,
-                        pl.Npad, pl.jchunk, ps.C1, ps.thr);
+                        pl.Npad, pl.jchunk, ps.C1, ps.thr, pl.i0, pl.iend());
   return hipGetLastError();
 }
 
@@ -931,6 +933,28 @@ ForcePlan make_force_plan(int N, int jslices_override, int ipl)
   pl.jchunk = jc;
   pl.S = (N + jc - 1) / jc;
   pl.Npad = (N + 63) & ~63;
+  return pl;
+}
+
+ForcePlan make_shard_plan(int N, int shard, int nshards, int jslices)
+{
+  // whole 64-particle tiles (the kernel's wave tiles), so that every wave of
+  // a share splits its slices' j ranges where the full pass does (the self
+  // pair's segment) and takes the same FP32 blocks: bit-identical rows
+  const long T = (N + 63) / 64;
+  const int i0 = (int)(64 * (shard * T / nshards));
+  const int i1 = (int)std::min<long>(N, 64 * ((shard + 1) * T / nshards));
+  const int gx = (i1 - i0 + kBlock - 1) / kBlock;
+  int S = jslices;
+  if (S <= 0) {
+    // about 8 resident waves per SIMD on 256 CUs x 4 SIMDs, as for a full pass
+    S = (8192 + gx * kWaves - 1) / (gx * kWaves);
+    S = max(1, min(S, (N + 63) / 64));
+  }
+  ForcePlan pl = make_force_plan(N, S, 1);  // the full pass's slicing for that count
+  pl.i0 = i0;
+  pl.i1 = i1;
+  pl.gx = gx;
   return pl;
 }
 
@@ -987,8 +1011,8 @@ hipError_t launch_reduce_forces(hipStream_t st, const ForcePlan& pl, const PairS
                                 const float4* fpart, const float* apart, float4* force,
                                 double* blk)
 {
-  hipLaunchKernelGGL(k_reduce_forces, dim3(nblocks(pl.N)), dim3(kBlock), 0, st, fpart, apart,
-                     force, blk, pl.N, pl.S, pl.Npad, ps.FS);
+  hipLaunchKernelGGL(k_reduce_forces, dim3(nblocks(pl.iend() - pl.i0)), dim3(kBlock), 0, st, fpart,
+                     apart, force, blk, pl.i0, pl.iend(), pl.S, pl.Npad, ps.FS);
   return hipGetLastError();
 }
 

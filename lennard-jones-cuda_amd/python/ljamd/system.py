@@ -182,6 +182,23 @@ class LJSystem:
         check(lib().lj_velocity_histogram(self.h, float(step), int(nbins), ptr(out)))
         return out
 
+    def forces_shard(self, shard, nshards, jslices=0):
+        """One GPU's share of a force pass sharded by i over nshards GPUs (at
+        most ceil(N/64)): the forces on its rows, whole 64-particle tiles
+        [64*(shard*T//nshards), 64*((shard+1)*T//nshards)) with T = ceil(N/64),
+        go to the context's force array; returns that share's (E_pot, W)
+        partial sums."""
+        ev = np.zeros(2, np.float64)
+        check(lib().lj_forces_shard(self.h, int(shard), int(nshards), int(jslices), None, ptr(ev)))
+        return float(ev[0]), float(ev[1])
+
+    def time_forces_shard(self, shard, nshards, jslices=0, reps=50):
+        """Microseconds per launch of one share's force kernel."""
+        us = C.c_double()
+        check(lib().lj_time_forces_shard(self.h, int(shard), int(nshards), int(jslices), int(reps),
+                                         C.byref(us)))
+        return us.value
+
     def shear_pressure(self):
         """Pshear of the current state as the reference CPU path computes it
         (MDSystem.cpp:299,309,335,353); one all-pairs pass."""
