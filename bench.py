@@ -24,6 +24,9 @@ reference does; the same code over 16 host threads beside it.
 precision: the FP32, compensated and FP64 modes on the precision workload
 (N = 131072, hard wall), force-pass time and E_pot / P deviation from FP64
 over the first 100 steps.
+shard: one GPU's share of the headline force pass sharded by i-particles over
+8 GPUs (lj_time_forces_shard: 2048 rows against all N on the sliced kernel)
+beside the whole pass on the same rule; the collectives are not run (one GPU).
 
 Clocks: the GPU's shader clock ramps up over the first ~0.1 s of sustained
 load, so every timed leg first runs force passes for at least SETTLE_MS
@@ -241,6 +244,20 @@ def rdf_leg(reps=20):
             "ref_cpu_256_r2_bins_us": us_c, "ref_cpu_pairs_per_s": pairs / (us_c * 1e-6),
             "hard_wall": {"rbins_500_to_half_box_us": wr, "ref_gpu_256_r2_bins_us": wg,
                           "ref_cpu_256_r2_bins_us": wc}}
+
+
+def shard_leg(nshards=8, reps=200):
+    """One share of the headline force pass sharded by i over nshards GPUs
+    against the whole pass with the same slicing rule (the sliced kernel)."""
+    import ljamd
+
+    with ljamd.LJSystem(N, RHO, T0, canonical=ljamd.TVN, seed=SEED) as s:
+        settle(s)
+        share = s.time_forces_shard(0, nshards, reps=reps)
+        whole = s.time_forces_shard(0, 1, reps=max(1, reps // 4))
+    return {"nshards": nshards, "rows": N // nshards, "share_force_us": share,
+            "whole_force_us": whole, "speedup": whole / share,
+            "kernel": "k_force (sliced) on a row range; rows of whole 64-particle tiles"}
 
 
 def precision_leg(steps=100):
@@ -471,6 +488,7 @@ def main():
     adjoint = adjoint_leg()
     rdf = rdf_leg()
     precision = precision_leg()
+    shard = shard_leg()
     cpu = None if a.no_cpu else cpu_leg()
     if cpu is not None:
         cpu["all_cores"] = cpu_parallel_leg()
@@ -532,6 +550,7 @@ def main():
         "adjoint": adjoint,
         "rdf": rdf,
         "precision": precision,
+        "shard": shard,
         "final_T": g["obs"]["T"],
         "final_u": g["obs"]["U"] / N,
     }
