@@ -443,6 +443,8 @@ def main():
                     help="only the batched-ensemble leg")
     ap.add_argument("--profile-precision", action="store_true",
                     help="only the precision modes' leg")
+    ap.add_argument("--profile-shard", action="store_true",
+                    help="only one GPU's share of the sharded force pass (for rocprofv3)")
     ap.add_argument("--profile-adjoint", action="store_true",
                     help="only the differentiable workload's leg (for rocprofv3)")
     ap.add_argument("--sweep-n", action="store_true",
@@ -453,6 +455,9 @@ def main():
         return
     if a.profile_force:
         profile_force(a.profile_force, boundary=a.profile_boundary, n=a.profile_n)
+        return
+    if a.profile_shard:
+        print(json.dumps({"profile": "shard", "N": N, **shard_leg()}))
         return
     if a.profile_rdf:
         print(json.dumps({"profile": "rdf", "N": N, **rdf_leg()}))
