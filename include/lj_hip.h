@@ -257,7 +257,9 @@ int lj_velocity_histogram(lj_ctx* ctx, double step, int nbins, uint64_t* counts)
  * positions, on
  * the sliced kernel with jslices j slices (0: enough slices to fill the device
  * at that share), written to rows i0..i1-1 of d_force4 (float4[N] on the
- * device; NULL: the context's force array), with that share's E_pot and W
+ * device; NULL: the context's force array, which the next lj_integrate's
+ * first drift reads, so run every share before stepping), with that share's
+ * E_pot and W
  * partial sums in ev[0], ev[1] (may be NULL): the all-reduce's inputs.  With
  * the jslices of a full sliced evaluation (lj_config.jslices) the rows are
  * bit-identical to it.  FP32 and compensated modes. */
