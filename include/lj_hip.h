@@ -251,18 +251,17 @@ int lj_velocity_histogram(lj_ctx* ctx, double step, int nbins, uint64_t* counts)
 
 /* One GPU's share of a force pass sharded by i-particles over nshards GPUs
  * (no counterpart in the reference, which runs on one device; the single-GPU
- * part of the multi-GPU design): the forces on particles [i0, i1), whole
- * 64-particle tiles: with T = ceil(N/64), i0 = 64*floor(shard*T/nshards), i1 =
- * min(N, 64*floor((shard+1)*T/nshards)), 1 <= nshards <= T, from all N current
- * positions, on
- * the sliced kernel with jslices j slices (0: enough slices to fill the device
- * at that share), written to rows i0..i1-1 of d_force4 (float4[N] on the
- * device; NULL: the context's force array, which the next lj_integrate's
- * first drift reads, so run every share before stepping), with that share's
- * E_pot and W
- * partial sums in ev[0], ev[1] (may be NULL): the all-reduce's inputs.  With
- * the jslices of a full sliced evaluation (lj_config.jslices) the rows are
- * bit-identical to it.  FP32 and compensated modes. */
+ * part of the multi-GPU design).  The share is the forces on particles
+ * [i0, i1), whole 64-particle tiles: with T = ceil(N/64),
+ * i0 = 64*floor(shard*T/nshards), i1 = min(N, 64*floor((shard+1)*T/nshards)),
+ * 1 <= nshards <= T, from all N current positions, on the sliced kernel with
+ * jslices j slices (0: enough slices to fill the device at that share).  The
+ * rows go to rows i0..i1-1 of d_force4 (float4[N] on the device; NULL: the
+ * context's force array, which the next lj_integrate's first drift reads, so
+ * run every share before stepping), and the share's E_pot and W partial sums
+ * to ev[0], ev[1] (may be NULL): the all-reduce's inputs.  With the jslices
+ * of a full sliced evaluation (lj_config.jslices) the rows are bit-identical
+ * to it.  FP32 and compensated modes. */
 int lj_forces_shard(lj_ctx* ctx, int shard, int nshards, int jslices, float* d_force4,
                     double* ev);
 /* the share's force kernel alone, averaged over reps launches (HIP events) */
