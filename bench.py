@@ -255,9 +255,15 @@ def shard_leg(nshards=8, reps=200):
         settle(s)
         share = s.time_forces_shard(0, nshards, reps=reps)
         whole = s.time_forces_shard(0, 1, reps=max(1, reps // 4))
+        # the pair-once step pass with its items dealt out instead
+        n3_share = s.time_force_items_share(0, nshards, reps=reps)
+        n3_whole = s.time_force_items_share(0, 1, reps=max(1, reps // 4))
     return {"nshards": nshards, "rows": N // nshards, "share_force_us": share,
             "whole_force_us": whole, "speedup": whole / share,
-            "kernel": "k_force (sliced) on a row range; rows of whole 64-particle tiles"}
+            "kernel": "k_force (sliced) on a row range; rows of whole 64-particle tiles",
+            "pair_once_items": {"share_force_us": n3_share, "whole_force_us": n3_whole,
+                                "speedup": n3_whole / n3_share,
+                                "kernel": "k_force_n3 step variant on 1/nshards of its items"}}
 
 
 def precision_leg(steps=100):

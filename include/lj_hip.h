@@ -266,6 +266,15 @@ int lj_forces_shard(lj_ctx* ctx, int shard, int nshards, int jslices, float* d_f
                     double* ev);
 /* the share's force kernel alone, averaged over reps launches (HIP events) */
 int lj_time_forces_shard(lj_ctx* ctx, int shard, int nshards, int jslices, int reps, double* us);
+/* The other way to shard (no counterpart in the reference): the time step's
+ * pair-once pass with its group-pair items dealt out, share `shard` of
+ * nshards taking the contiguous items [shard*M/nshards, (shard+1)*M/nshards)
+ * of the M workgroup items (every item carries the same work).  Each item
+ * writes its own partial entries, so across GPUs an all-reduce of the partial
+ * buffers would give the full pass's.  Times the share's launch (HIP events,
+ * reps launches; the context's partial buffers are overwritten).  FP32 mode,
+ * pair-once pass with paired diagonal items (N >= about 11000). */
+int lj_time_force_items_share(lj_ctx* ctx, int shard, int nshards, int reps, double* us);
 
 /* Shear stress of the current state as the reference's CPU path computes it
  * (MDSystem.cpp:299,309,335,353): (sum over ordered pairs of -r_x f_y / 2 +

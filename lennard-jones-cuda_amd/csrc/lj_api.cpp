@@ -856,6 +856,35 @@ int lj_time_forces_shard(lj_ctx* c, int shard, int nshards, int jslices, int rep
   return LJ_OK;
 }
 
+int lj_time_force_items_share(lj_ctx* c, int shard, int nshards, int reps, double* us)
+{
+  LJ_TRY(ctx_setdev(c));
+  const ForcePlan& pl = c->plan3;
+  if (reps <= 0 || !us) return fail(LJ_ERR_ARG, "lj_time_force_items_share: bad arguments");
+  if (pl.n3 != 2 || pl.split != 1 || pl.comp)
+    return fail(LJ_ERR_STATE, "lj_time_force_items_share: needs the FP32 pair-once step pass "
+                              "with paired diagonal items");
+  const int M = pl.nitems_step;
+  if (nshards < 1 || nshards > M || shard < 0 || shard >= nshards)
+    return fail(LJ_ERR_ARG, "lj_time_force_items_share: need 0 <= shard < nshards <= %d", M);
+  const int first = (int)((long)shard * M / nshards);
+  const int count = (int)((long)(shard + 1) * M / nshards) - first;
+  hipStream_t st = c->stream;
+  if (c->ps.periodic) LJ_HIP(launch_prep_q(st, c->pos, c->q, c->cfg.N, c->ps.qscale));
+  LJ_HIP(launch_force_n3_step_items(st, pl, c->ps, c->q, c->pos, c->fpart, c->step_ew, first,
+                                    count));  // warm
+  LJ_HIP(hipEventRecord(c->e0, st));
+  for (int k = 0; k < reps; ++k)
+    LJ_HIP(launch_force_n3_step_items(st, pl, c->ps, c->q, c->pos, c->fpart, c->step_ew, first,
+                                      count));
+  LJ_HIP(hipEventRecord(c->e1, st));
+  LJ_HIP(hipEventSynchronize(c->e1));
+  float ms = 0.f;
+  LJ_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+  *us = 1000.0 * ms / reps;
+  return LJ_OK;
+}
+
 int lj_shear_pressure(lj_ctx* c, double* pshear)
 {
   LJ_TRY(ctx_setdev(c));

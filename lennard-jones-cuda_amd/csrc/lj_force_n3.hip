@@ -1065,6 +1065,30 @@ hipError_t launch_batch_force_n3(hipStream_t st, const BatchArgs& a, bool period
   return hipGetLastError();
 }
 
+hipError_t launch_force_n3_step_items(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                                      const uint4* q, const float4* posf, float4* fpart, double* ew,
+                                      int first, int count)
+{
+  // the step variant's items [first, first + count), each writing its own
+  // partial entries and (E, W) totals: one GPU's share of a pass whose items
+  // are dealt out (W = 2, diagonal items paired, not split)
+  if (pl.n3 != 2 || pl.split != 1 || pl.comp || first < 0 || count < 1 ||
+      first + count > pl.nitems_step)
+    return hipErrorInvalidValue;
+  const LaunchStamp ts = tl_force_stamp;
+  const int2* items = pl.items_step + first;
+  double* e = ew + 2 * (size_t)first;
+  if (ps.periodic)
+    hipExtLaunchKernelGGL((k_force_n3<true, false, 2, false, false>), dim3(count), dim3(256), 0, st,
+                          ts.start, ts.stop, 0, q, posf, items, fpart, nullptr, e, pl.N, pl.Npad,
+                          ps.C1);
+  else
+    hipExtLaunchKernelGGL((k_force_n3<false, false, 2, false, false>), dim3(count), dim3(256), 0,
+                          st, ts.start, ts.stop, 0, q, posf, items, fpart, nullptr, e, pl.N, pl.Npad,
+                          ps.C1);
+  return hipGetLastError();
+}
+
 hipError_t launch_force_n3(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                            const uint4* q, const float4* posf, float4* fpart, float* apart,
                            bool ties, double* ew)

@@ -92,6 +92,11 @@ hipError_t launch_prep_q(hipStream_t st, const float4* pos, uint4* q, int N, dou
 // step_ew (pair-once plans only, may be null): the time step's variant, which
 // writes force-only partials and per-workgroup (E, W) totals to step_ew
 struct StepArgs;
+// the time step's pair-once pass on its items [first, first + count) (W = 2,
+// paired diagonal items, not split): one GPU's share when items are dealt out
+hipError_t launch_force_n3_step_items(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
+                                      const uint4* q, const float4* posf, float4* fpart, double* ew,
+                                      int first, int count);
 hipError_t launch_force(hipStream_t st, const ForcePlan& pl, const PairScalars& ps,
                         const uint4* q, const float4* posf, float4* fpart, float* apart,
                         bool compensated, bool ties, double* step_ew = nullptr);

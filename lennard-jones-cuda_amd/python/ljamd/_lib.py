@@ -105,6 +105,7 @@ SIGNATURES = [
     ("lj_shear_pressure", _i, [_vp, C.POINTER(_d)]),
     ("lj_forces_shard", _i, [_vp, _i, _i, _i, _vp, _vp]),
     ("lj_time_forces_shard", _i, [_vp, _i, _i, _i, _i, C.POINTER(_d)]),
+    ("lj_time_force_items_share", _i, [_vp, _i, _i, _i, C.POINTER(_d)]),
     ("lj_get_array_f64", _i, [_vp, _i, _vp]),
     ("lj_compute_forces", _i, [_vp, C.POINTER(LJObs)]),
     ("lj_compute_parameters", _i, [_vp, C.POINTER(LJObs)]),

@@ -199,6 +199,14 @@ class LJSystem:
                                          C.byref(us)))
         return us.value
 
+    def time_force_items_share(self, shard, nshards, reps=200):
+        """Microseconds per launch of one share of the pair-once step pass with
+        its items dealt out over nshards GPUs."""
+        us = C.c_double()
+        check(lib().lj_time_force_items_share(self.h, int(shard), int(nshards), int(reps),
+                                              C.byref(us)))
+        return us.value
+
     def shear_pressure(self):
         """Pshear of the current state as the reference CPU path computes it
         (MDSystem.cpp:299,309,335,353); one all-pairs pass."""

@@ -74,3 +74,23 @@ def test_shard_bad_args():
     with ljamd.LJSystem(512, 0.5, 1.0, precision=ljamd.PREC_FP64) as s:
         with pytest.raises(ljamd.LJError):
             s.forces_shard(0, 2)
+
+
+def test_pair_once_items_share_n16384():
+    """The other sharding: the time step's pair-once pass with its items dealt
+    out over 8 GPUs; one share's launch against the whole pass (timing only;
+    the partial entries of the items are disjoint by construction)."""
+    N, rho = 16384, 0.85
+    print(f"\n[case] pair-once item share N={N}", flush=True)
+    with ljamd.LJSystem(N, rho, 1.0, canonical=ljamd.TVN) as s:
+        assert s.force_plan() == (True, 64)
+        t8 = s.time_force_items_share(0, 8)
+        t1 = s.time_force_items_share(0, 1, reps=50)
+        with pytest.raises(ljamd.LJError):
+            s.time_force_items_share(8, 8)
+    print(f"  one share of 8: {t8:.2f} us, the whole pass: {t1:.2f} us (ratio {t1 / t8:.2f})")
+    assert t8 < t1
+    with ljamd.LJSystem(4096, rho, 1.0) as s:  # split items: no paired diagonals
+        with pytest.raises(ljamd.LJError):
+            s.time_force_items_share(0, 2)
+
