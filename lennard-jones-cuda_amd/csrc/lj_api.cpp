@@ -183,7 +183,18 @@ int enqueue_step_part(lj_ctx* c, hipStream_t st, double dt, bool first, bool las
   if (first) LJ_HIP(launch_drift(st, a));
   // the drift kernel already wrote the fixed-point copy for periodic boxes
   double* ew = const_cast<double*>(a.ew);
-  LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false, ew));
+  const ForcePlan& p3 = c->plan3;
+  if (c->emulate_shards > 1 && p3.n3 == 2 && p3.split == 1 && !p3.comp) {
+    // each item writes its own partial entries and totals: the G launches
+    // leave the buffers of the single launch
+    const int M = p3.nitems_step, G = std::min(c->emulate_shards, M);
+    for (int g = 0; g < G; ++g) {
+      const int f = (int)((long)g * M / G), e = (int)((long)(g + 1) * M / G);
+      LJ_HIP(launch_force_n3_step_items(st, p3, c->ps, c->q, c->pos, c->fpart, ew, f, e - f));
+    }
+  } else {
+    LJ_HIP(launch_force(st, fplan(c), c->ps, c->q, c->pos, c->fpart, c->apart, comp, false, ew));
+  }
   if (c->smp_rdf != LJ_SAMPLE_RDF_OFF) {
     // the positions the force pass saw (drifted, before the boundary
     // conditions), as CalculateForces bins them inside Integrate
@@ -410,6 +421,7 @@ int lj_create(const lj_config* cfg, lj_ctx** out)
   c->L = box_length(cfg->N, cfg->rho);
   refresh_scalars(c);
   c->plan = make_force_plan(cfg->N, cfg->jslices, 1);
+  if (const char* e = std::getenv("LJ_EMULATE_SHARDS")) c->emulate_shards = std::max(1, std::atoi(e));
   // the pair-once kernel serves the FP32 and compensated modes at sizes that
   // fill the GPU; an explicit j-slice count keeps the sliced kernel
   if (cfg->jslices == 0 && n3_eligible(cfg->N)) {

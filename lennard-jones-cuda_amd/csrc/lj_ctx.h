@@ -50,6 +50,10 @@ struct lj_ctx {
   double* sh_ev = nullptr;
   size_t sh_cap = 0;
   int sh_blk_cap = 0;
+  // > 1: the step's pair-once pass runs as that many consecutive launches on
+  // contiguous item ranges, as that many GPUs would run their shares
+  // (LJ_EMULATE_SHARDS, read at creation; a check of the sharded design)
+  int emulate_shards = 1;
   hipGraph_t graph = nullptr;        // one step
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph_run = nullptr;    // kGraphSteps steps back to back

@@ -4,12 +4,14 @@ about N / G of them, against all N.  With the slicing of a full sliced evaluatio
 reassemble it bit for bit and their E_pot / W partials add up to its totals;
 with the default slicing each share fills the device, and at N = 16384 over
 8 shares one share's kernel is timed against the whole pass."""
+import os
+
 import numpy as np
 import pytest
 
 import ljamd
 from helpers import force_errors, perturbed_state, xyz
-from oracle_lib import forces_fp64
+from oracle_lib import forces_fp64, lattice_state
 
 pytestmark = pytest.mark.gpu
 
@@ -93,4 +95,29 @@ def test_pair_once_items_share_n16384():
     with ljamd.LJSystem(4096, rho, 1.0) as s:  # split items: no paired diagonals
         with pytest.raises(ljamd.LJError):
             s.time_force_items_share(0, 2)
+
+
+@pytest.mark.parametrize("canon,bc", [(ljamd.TVN, ljamd.BC_PERIODIC), (ljamd.EVN, ljamd.BC_WALL)])
+def test_emulated_item_shards_step_bitwise(canon, bc):
+    """The time step with its pair-once pass run as 8 consecutive launches on
+    contiguous item ranges (LJ_EMULATE_SHARDS=8, as 8 GPUs would run their
+    shares): every item writes its own partial entries and totals, so the
+    trajectory is bit-identical to the single launch's."""
+    N, rho, T0 = 16384, 0.85, 1.0
+    L, pos, vel = lattice_state(N, rho, T0, seed=5)
+    print(f"\n[case] emulated item shards canon={canon} bc={bc}", flush=True)
+    res = {}
+    for g in ("1", "8"):
+        os.environ["LJ_EMULATE_SHARDS"] = g
+        try:
+            with ljamd.LJSystem(N, rho, T0, canonical=canon, boundary=bc) as s:
+                assert s.force_plan() == (True, 64)
+                s.set_state(pos, vel)
+                s.CalculateForces()
+                h = s.Integrate(0.005, 20)
+                res[g] = (h, s.pos.copy(), s.vel.copy())
+        finally:
+            os.environ.pop("LJ_EMULATE_SHARDS", None)
+    for a, b in zip(res["1"], res["8"]):
+        assert np.array_equal(a, b)
 
