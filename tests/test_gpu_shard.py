@@ -18,17 +18,20 @@ pytestmark = pytest.mark.gpu
 
 # slice counts whose j chunks are multiples of 8 (kept as given) and whose
 # full pass runs the plain sliced grid (at least four waves per SIMD)
-@pytest.mark.parametrize("N,bc,G,S", [(4096, ljamd.BC_PERIODIC, 4, 64), (6000, ljamd.BC_WALL, 3, 75),
-                                      (16384, ljamd.BC_PERIODIC, 8, 32),
-                                      (3001, ljamd.BC_NONE, 7, 94)])
-def test_shards_reassemble_full_pass(N, bc, G, S):
+@pytest.mark.parametrize("N,bc,G,S,prec", [(4096, ljamd.BC_PERIODIC, 4, 64, ljamd.PREC_FP32),
+                                           (6000, ljamd.BC_WALL, 3, 75, ljamd.PREC_FP32),
+                                           (16384, ljamd.BC_PERIODIC, 8, 32, ljamd.PREC_FP32),
+                                           (3001, ljamd.BC_NONE, 7, 94, ljamd.PREC_FP32),
+                                           (6000, ljamd.BC_PERIODIC, 5, 75, ljamd.PREC_COMPENSATED)])
+def test_shards_reassemble_full_pass(N, bc, G, S, prec):
     """Shares with the full pass's j slices: bit-identical force rows
     (Force.w included), E_pot and W partials summing to the totals; shares
-    of uneven size (6000 / 3: 1984 + 1984 + 2032 rows; 3001 / 7)."""
+    of uneven size (6000 / 3: 1984 + 1984 + 2032 rows; 3001 / 7); the
+    compensated mode's sliced kernel (TwoSum blocks) alike."""
     rho = 0.85
     L, pos, _ = perturbed_state(N, rho, 1.0, seed=31)
-    print(f"\n[case] shards N={N} bc={bc} G={G} S={S}", flush=True)
-    with ljamd.LJSystem(N, rho, 1.0, boundary=bc, jslices=S) as s:
+    print(f"\n[case] shards N={N} bc={bc} G={G} S={S} prec={prec}", flush=True)
+    with ljamd.LJSystem(N, rho, 1.0, boundary=bc, jslices=S, precision=prec) as s:
         assert s.force_plan() == (False, S)
         s.set_state(pos, np.zeros_like(pos))
         obs = s.CalculateForces()
